@@ -1,0 +1,336 @@
+/*
+ * pmmc_hip.h -- C ABI of the MI355X-native pMMC annealing hot path.
+ *
+ * Pure C (stdint.h only): host C++ that includes this header compiles with
+ * g++ and never sees a HIP type.  All pointers are caller-owned HOST memory;
+ * device memory is owned by the opaque pmmc_ctx (grow-only arenas).
+ *
+ * Drop-in boundary (reference entry points this ABI replaces):
+ *   pmmc_heatmap_anneal  <- float LooperSolver::ParallelMonteCarloHeatmap(float)
+ *                           LooperSolver.hpp:469, ParallelMonteCarloHeatmap.cu:365
+ *   pmmc_heatmap_sweep   <- MonteCarloHeatmapKernel's algorithm,
+ *                           ParallelMonteCarloHeatmap.cu:217-363 (race-free form)
+ *   pmmc_arcs_anneal     <- float LooperSolver::parallelMonteCarloArcs(float)
+ *                           LooperSolver.hpp:682, ParallelMonteCarloArcs.cu:680
+ *   pmmc_smooth_anneal   <- float LooperSolver::ParallelMonteCarloSmooth(float, ull)
+ *                           LooperSolver.hpp:482, ParallelMonteCarloSmooth.cu:357
+ *   pmmc_mark_arcs       <- kernel<<<256,1024>>> of InteractionArcs::parallelMarkArcs
+ *                           ParallelMarkArcs.cu:36-64, :69-148
+ *   pmmc_group_arcs      <- host grouping of InteractionArcs::parallelMarkArcs
+ *                           ParallelMarkArcs.cu:150-243
+ *   pmmc_set_gpu_seed    <- void setGpuSeed(unsigned) platform.h:106,
+ *                           ParallelMonteCarloHeatmap.cu:37
+ *   pmmc_ctx_create/destroy/reserve <- GpuResourceCache::init/cleanup and its
+ *                           grow-only buffers, LooperSolver.hpp:54-106,
+ *                           ParallelMonteCarloArcs.cu:84-213
+ * Test hook (no reference counterpart, needed to check per-move ΔE on injected
+ * proposals):  pmmc_delta_energy.
+ * Batch / serve forms (no reference counterpart; the batched and served
+ * annealing forms):  pmmc_anneal_batch, pmmc_server_*.
+ *
+ * Errors: every entry point returns pmmc_status; nothing calls exit().  The
+ * message of the last failure is available from pmmc_last_error(ctx).
+ */
+#ifndef PMMC_HIP_H
+#define PMMC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  PMMC_OK = 0,
+  PMMC_ERR_INVALID = 1,   /* bad argument, size or layout                   */
+  PMMC_ERR_HIP = 2,       /* a HIP runtime call failed                      */
+  PMMC_ERR_NOMEM = 3,     /* device allocation failed / budget exceeded     */
+  PMMC_ERR_NODEVICE = 4,  /* no usable gfx950 device                        */
+  PMMC_ERR_BUSY = 5       /* server has no free slot / memory for the job   */
+} pmmc_status;
+
+typedef enum { PMMC_LEVEL_HEATMAP = 1, PMMC_LEVEL_ARCS = 2, PMMC_LEVEL_SMOOTH = 3 } pmmc_level;
+
+/* Stop rules of the milestone test (see pmmc_schedule). */
+typedef enum {
+  PMMC_STOP_FIRST_FAIL = 0,   /* CPU annealers: stop at the first failing milestone
+                                 (LooperSolver.hpp:1543-1548, :4775-4779, :5027-5031) */
+  PMMC_STOP_MISSES_RESET = 1, /* GPU arcs: misses reset on improvement
+                                 (ParallelMonteCarloArcs.cu:636-655)                 */
+  PMMC_STOP_MISSES_ACCUM = 2  /* GPU heatmap / smooth: misses never reset
+                                 (ParallelMonteCarloHeatmap.cu:343-352,
+                                  ParallelMonteCarloSmooth.cu:320-335)               */
+} pmmc_stop_rule;
+
+typedef enum {
+  PMMC_STOP_RUNNING = 0, PMMC_STOP_RULE = 1, PMMC_STOP_SMALL = 2, PMMC_STOP_CAP = 3,
+  PMMC_STOP_GUARD = 4, PMMC_STOP_NOFREE = 5, PMMC_STOP_NONFINITE = 6
+} pmmc_stop_reason;
+
+/* Schedule of one Metropolis chain.  One chain performs blocks of `block`
+ * moves.  Move m (0-based, counted) of chain c:
+ *   bead p  = free[ pick(B, n_free) ], B = Philox(key, {m>>2, c>>pick_shift, segment,
+ *             level<<28|1})[m&3]  (pick_shift = 6: the 64 chains of one wavefront
+ *             share bead picks; displacement and acceptance stay per chain)
+ *   {dx,dy,dz,U} = Philox(key, {m, c, segment, level<<28|0}), d = (2*u01-1)*step
+ *   ΔE = local(p, x_p + d) - local(p, x_p)       (FP32, level semantics)
+ *   E' = E + ΔE                                   (FP64 running total)
+ *   accept if E' <= E (E' < E when strict_accept), else if T > 0 (and E > 0 when
+ *   positive_guard): U < tjs * exp(-tjc * (E'/E) / T)
+ *   T *= dt after every move (or once per block when cool_per_block)
+ * After each block: step *= step_decay; when (iters % milestone_period) < block
+ * a milestone runs: E is re-summed over the current positions (FP32 terms,
+ * FP64 sum) and the stop rule is applied with
+ *   no_improvement = (E > improvement*ms && successes < min_successes)
+ *                    || E < small_energy || (ratio_stop > 0 && ms > 1e-10 && E/ms > ratio_stop)
+ * Chains stop when the rule fires, E is non-finite, or iters >= max_moves
+ * (checked before each block, as the reference GPU kernels do).
+ * Presets reproducing each reference annealer: pmmc_schedule_preset().      */
+typedef struct {
+  double T0;              /* maxTemp*                                           */
+  double dt;              /* dtTemp*                                            */
+  double improvement;     /* MCstopConditionImprovement*                        */
+  double small_energy;    /* 1e-6 (CPU heatmap/smooth) / 1e-5 (arcs) / 1e-4 (GPU heatmap) */
+  double ratio_stop;      /* 0.9999 for arcs, 0 = off                           */
+  double guard_energy;    /* skip annealing when E0 < guard or NaN (heatmap 1e-6) */
+  float tjs;              /* tempJumpScale*                                     */
+  float tjc;              /* tempJumpCoef*                                      */
+  float step;             /* displacement half width, already scaled            */
+  float step_decay;       /* per block (0.95 for the GPU heatmap schedule)      */
+  float sphere_radius;    /* Settings::sphere_radius, 0 = off                   */
+  int32_t block;          /* moves per block                                    */
+  int32_t milestone_period;
+  int32_t min_successes;  /* MCstopConditionMinSuccesses*                       */
+  int32_t fails_threshold;/* [cuda] milestone_fails                             */
+  int32_t stop_rule;      /* pmmc_stop_rule                                     */
+  int32_t strict_accept;
+  int32_t positive_guard;
+  int32_t cool_per_block;
+  int32_t use2d;
+  int32_t resum;          /* 1: re-sum E at milestones (default)                */
+  int64_t max_moves;      /* hard cap per chain                                 */
+} pmmc_schedule;
+
+/* Which reference annealer a preset reproduces. */
+typedef enum { PMMC_PRESET_CPU = 0, PMMC_PRESET_GPU = 1 } pmmc_preset;
+
+/* Reference settings consumed by the presets ([cuda] and [simulation_*] INI
+ * sections, Settings.hpp:649-654, :865-941, defaults :350-515). */
+typedef struct {
+  double max_temp, dt_temp, improvement;
+  float tjs, tjc;
+  int32_t min_successes, steps;   /* stop_condition_successes / stop_condition_steps */
+  int32_t milestone_fails;        /* [cuda] milestone_fails                            */
+  float sphere_radius;
+  int32_t use2d;
+} pmmc_level_settings;
+
+/* Fill *out with the schedule that reproduces the reference annealer of
+ * `level` under `preset` (CPU: LooperSolver::MonteCarlo*; GPU: the .cu
+ * kernels), for a caller step size `step_size` (the value the reference level
+ * driver passes).  Returns PMMC_ERR_INVALID for an unknown level/preset. */
+pmmc_status pmmc_schedule_preset(int level, int preset, const pmmc_level_settings *s,
+                                 float step_size, pmmc_schedule *out);
+
+/* ---------------- problems (host memory, caller owned) ---------------- */
+
+/* Heatmap level: LooperSolver state read by ParallelMonteCarloHeatmap. */
+typedef struct {
+  int32_t n;                 /* active region size == heatmap size          */
+  const float *pos;          /* [n*3] initial xyz                           */
+  const uint8_t *fixed;      /* [n] or NULL                                  */
+  const float *dist;         /* [n*n] heatmap_dist row-major                */
+  int32_t diagonal;          /* heatmap_dist.diagonal_size                  */
+  const int32_t *boundaries; /* heatmap_chromosome_boundaries or NULL       */
+  int32_t n_boundaries;
+  float skip_below;          /* 1e-6 (CPU, LooperSolver.hpp:3604); 1e-3 (GPU, .cu:176) */
+} pmmc_heatmap_problem;
+
+/* Arc level: state read by parallelMonteCarloArcs. */
+typedef struct {
+  int32_t n;
+  const float *pos;          /* [n*3]                                       */
+  const uint8_t *fixed;      /* [n] or NULL                                  */
+  const float *exp_dist;     /* [n*n] heatmap_exp_dist_anchor, -1 = no arc  */
+  int32_t n_loops;
+  const int32_t *loop_pairs; /* [2*n_loops] active-region indices            */
+  const float *loop_params;  /* [2*n_loops] {stiffness, eq_distance}         */
+  float k_stretch, k_squeeze;/* springConstantStretchArcs / SqueezeArcs     */
+} pmmc_arcs_problem;
+
+/* Smooth level: state read by ParallelMonteCarloSmooth. */
+typedef struct {
+  int32_t n;
+  const float *pos;          /* [n*3]                                       */
+  const uint8_t *fixed;      /* [n] or NULL                                  */
+  const float *dist_to_next; /* [n]                                          */
+  float k_stretch, k_squeeze, k_angular; /* [springs]                        */
+  float w_dist, w_angle;     /* weightDistSmooth, weightAngleSmooth          */
+} pmmc_smooth_problem;
+
+/* How many chains and which random stream. */
+typedef struct {
+  uint64_t seed;             /* Philox key                                   */
+  uint32_t segment;          /* Philox counter word 2 (problem / segment id) */
+  uint32_t chain0;           /* first chain id                               */
+  int32_t n_chains;          /* 0 = library default for the level            */
+} pmmc_run;
+
+typedef struct {
+  double energy;             /* best chain: final running total (FP64)        */
+  double energy_resummed;    /* best chain: level return value re-summed in FP64:
+                                heatmap calcScoreHeatmapActiveRegion(),
+                                arcs running score, smooth calcScoreStructureSmooth(true,true) */
+  double best_energy;        /* lowest running total seen by the best chain    */
+  int64_t moves;             /* counted moves, summed over all chains          */
+  int64_t accepted;          /* accepted moves, summed over all chains         */
+  int64_t best_chain_moves;  /* moves of the returned chain                    */
+  int64_t best_chain_accepted;
+  int32_t best_chain;        /* chain id of the returned chain                 */
+  int32_t n_chains;
+  int32_t milestones;        /* of the returned chain                          */
+  int32_t stop_reason;       /* of the returned chain (pmmc_stop_reason)      */
+  double kernel_ms;          /* device time of the annealing launches          */
+} pmmc_anneal_stats;
+
+/* ---------------- context ---------------- */
+typedef struct pmmc_ctx pmmc_ctx;
+
+pmmc_status pmmc_ctx_create(int device, pmmc_ctx **out);
+void pmmc_ctx_destroy(pmmc_ctx *ctx);
+/* Grow the device arena to at least `bytes` (HBM-budget admission). */
+pmmc_status pmmc_ctx_reserve(pmmc_ctx *ctx, size_t bytes);
+const char *pmmc_last_error(const pmmc_ctx *ctx);
+/* Number of compute units of the context's device. */
+int pmmc_ctx_cu_count(const pmmc_ctx *ctx);
+
+/* setGpuSeed: default seed used when a run passes seed == 0. */
+void pmmc_set_gpu_seed(pmmc_ctx *ctx, uint64_t seed);
+
+/* ---------------- plain (one problem per call) ---------------- */
+/* Anneal n_chains chains from problem->pos, write the positions of the chain
+ * with the lowest final running energy (ties: lowest chain id) to pos_out
+ * [n*3].  For the smooth level only non-fixed beads are written back, as
+ * ParallelMonteCarloSmooth.cu:485-491 does; pos_out must hold the input
+ * positions for the fixed beads. */
+pmmc_status pmmc_heatmap_anneal(pmmc_ctx *ctx, const pmmc_heatmap_problem *p,
+                                const pmmc_schedule *s, const pmmc_run *run,
+                                float *pos_out, pmmc_anneal_stats *stats);
+pmmc_status pmmc_arcs_anneal(pmmc_ctx *ctx, const pmmc_arcs_problem *p,
+                             const pmmc_schedule *s, const pmmc_run *run,
+                             float *pos_out, pmmc_anneal_stats *stats);
+pmmc_status pmmc_smooth_anneal(pmmc_ctx *ctx, const pmmc_smooth_problem *p,
+                               const pmmc_schedule *s, const pmmc_run *run,
+                               float *pos_out, pmmc_anneal_stats *stats);
+
+/* ---------------- heatmap sweep (reference GPU heatmap kernel) ---------------- */
+/* Each iteration: every non-fixed bead b gets `proposers` independent
+ * proposers; proposer r runs `inner` Metropolis proposals on bead b against a
+ * snapshot of all other beads (score = local(b) + sphere, GPU skip threshold),
+ * starting from the snapshot position and from the previous iteration's best
+ * score (the initial full score in the first iteration).  The proposer with the
+ * lowest final score (ties: lowest r) replaces bead b when its local score is
+ * below the bead's current one.  All beads update together.  Then T *= dt,
+ * step *= step_decay and the full score is re-summed in FP64; misses counts
+ * iterations with E > improvement * previous (never reset); stop when
+ * misses >= fails, E < small_energy, E is non-finite, or max_iters is reached.
+ * Proposer (b, r) uses the per-chain stream with chain = b*proposers + r and
+ * move = iteration*inner + k at level 4. */
+typedef struct {
+  double T0, dt, improvement, small_energy;
+  float tjs, tjc, step, step_decay, sphere_radius;
+  int32_t inner;             /* 512 in the reference                           */
+  int32_t proposers;         /* multiple of 64                                 */
+  int32_t fails_threshold;
+  int32_t max_iters;
+  int32_t use2d;
+} pmmc_sweep_schedule;
+
+pmmc_status pmmc_heatmap_sweep(pmmc_ctx *ctx, const pmmc_heatmap_problem *p,
+                               const pmmc_sweep_schedule *s, uint64_t seed, uint32_t segment,
+                               float *pos_inout, pmmc_anneal_stats *stats);
+
+/* ---------------- per-move ΔE test hook ---------------- */
+/* Evaluate ΔE (FP32, the kernels' arithmetic) for n_prop injected proposals
+ * (beads[i], disp[3*i..3*i+2]) against the fixed state `pos` of the problem.
+ * problem points to the level's problem struct; sphere_radius as in the
+ * schedule. */
+pmmc_status pmmc_delta_energy(pmmc_ctx *ctx, int level, const void *problem,
+                              float sphere_radius, int32_t n_prop, const int32_t *beads,
+                              const float *disp, float *delta_out);
+
+/* ---------------- arc marking ---------------- */
+/* For each raw arc i: out_start[i] = last anchor j with length_j > 1 that
+ * contains arc_start[i] (-1 if none), likewise out_end.  Anchor length is 0
+ * when start == end == 0, else end - start + 1 (Anchor.hpp:81-87). */
+pmmc_status pmmc_mark_arcs(pmmc_ctx *ctx, const int32_t *anchor_start,
+                           const int32_t *anchor_end, int32_t n_anchors,
+                           const int32_t *arc_start, const int32_t *arc_end, int32_t n_arcs,
+                           int32_t *out_start, int32_t *out_end);
+
+/* One emitted arc (InteractionArc fields, InteractionArc.hpp:86-92). */
+typedef struct {
+  int32_t start, end, genomic_start, genomic_end, score, eff_score, factor;
+} pmmc_arc;
+
+/* Whole InteractionArcs::parallelMarkArcs for n_chr chromosomes in order:
+ * sort raw arcs (start, end, factor), mark on the GPU, group by start anchor
+ * and merge per factor exactly as the reference host loop does, including the
+ * std::unordered_map emission order and the cross-chromosome last_start.
+ * raw: [4 * total] (genomic start, genomic end, score, factor) per chromosome
+ * slice raw_off[c]..raw_off[c+1]; anchors likewise via anc_off.  out must
+ * hold 2 * total arcs; out_off[n_chr+1] receives the per-chromosome slices;
+ * nonmatching[n_chr] (nullable) the skipped-arc counts. */
+pmmc_status pmmc_group_arcs(pmmc_ctx *ctx, int32_t n_chr, const int32_t *anc_off,
+                            const int32_t *anc_start, const int32_t *anc_end,
+                            const int32_t *raw_off, const int32_t *raw, pmmc_arc *out,
+                            int32_t *out_off, int32_t *nonmatching);
+
+/* ---------------- batched form ---------------- */
+typedef struct {
+  int32_t level;             /* pmmc_level                                     */
+  const void *problem;       /* pmmc_{heatmap,arcs,smooth}_problem*            */
+  pmmc_schedule schedule;
+  pmmc_run run;
+  float *pos_out;            /* [n*3] result (smooth: non-fixed beads only)     */
+  pmmc_anneal_stats *stats;  /* nullable                                        */
+} pmmc_job;
+
+typedef struct {
+  int32_t blocks_per_step;   /* blocks of moves per step launch (0 = autotune)  */
+  int32_t steps_per_graph;   /* step launches captured per hipGraph (0 = auto)  */
+  int32_t use_graph;         /* 1: replay captured step loops                  */
+} pmmc_batch_opts;
+
+/* Run all jobs concurrently (one device-resident chain pool, step launches
+ * replayed from a captured hipGraph until every chain has stopped). */
+pmmc_status pmmc_anneal_batch(pmmc_ctx *ctx, const pmmc_job *jobs, int32_t n_jobs,
+                              const pmmc_batch_opts *opts);
+
+/* ---------------- served form (continuous batching) ---------------- */
+typedef struct pmmc_server pmmc_server;
+
+typedef struct {
+  size_t hbm_budget;         /* bytes of chain state + problem data admitted   */
+  int32_t max_jobs;          /* resident job slots                              */
+  int32_t blocks_per_step;   /* blocks of moves per step launch                 */
+} pmmc_server_opts;
+
+pmmc_status pmmc_server_create(pmmc_ctx *ctx, const pmmc_server_opts *opts, pmmc_server **out);
+void pmmc_server_destroy(pmmc_server *srv);
+/* Queue a job (inputs are copied).  *job_id identifies it in poll. */
+pmmc_status pmmc_server_submit(pmmc_server *srv, const pmmc_job *job, uint64_t *job_id);
+/* Admit queued jobs that fit the budget, run one step launch per level over
+ * every resident chain, retire finished jobs (results written to the job's
+ * pos_out / stats).  *n_done = jobs finished in this step; *n_resident and
+ * *n_queued describe the pool afterwards (nullable). */
+pmmc_status pmmc_server_step(pmmc_server *srv, int32_t *n_done, int32_t *n_resident,
+                             int32_t *n_queued);
+/* 1 when job_id has finished (its outputs are written). */
+int pmmc_server_done(const pmmc_server *srv, uint64_t job_id);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

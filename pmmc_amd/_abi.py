@@ -1,0 +1,169 @@
+"""ctypes mirror of include/pmmc_hip.h (field order and types match the C
+structs exactly) and the loader of libpmmc_hip.so.
+
+The library is the product: if it is missing this module raises, there is no
+fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpmmc_hip.so")
+
+LEVEL_HEATMAP, LEVEL_ARCS, LEVEL_SMOOTH = 1, 2, 3
+PRESET_CPU, PRESET_GPU = 0, 1
+STOP_FIRST_FAIL, STOP_MISSES_RESET, STOP_MISSES_ACCUM = 0, 1, 2
+STATUS = {0: "PMMC_OK", 1: "PMMC_ERR_INVALID", 2: "PMMC_ERR_HIP", 3: "PMMC_ERR_NOMEM",
+          4: "PMMC_ERR_NODEVICE", 5: "PMMC_ERR_BUSY"}
+STOP_REASONS = {0: "running", 1: "rule", 2: "small", 3: "cap", 4: "guard", 5: "nofree",
+                6: "nonfinite"}
+
+f32p = C.POINTER(C.c_float)
+i32p = C.POINTER(C.c_int32)
+u8p = C.POINTER(C.c_uint8)
+
+
+class Schedule(C.Structure):
+    _fields_ = [
+        ("T0", C.c_double), ("dt", C.c_double), ("improvement", C.c_double),
+        ("small_energy", C.c_double), ("ratio_stop", C.c_double), ("guard_energy", C.c_double),
+        ("tjs", C.c_float), ("tjc", C.c_float), ("step", C.c_float), ("step_decay", C.c_float),
+        ("sphere_radius", C.c_float),
+        ("block", C.c_int32), ("milestone_period", C.c_int32), ("min_successes", C.c_int32),
+        ("fails_threshold", C.c_int32), ("stop_rule", C.c_int32), ("strict_accept", C.c_int32),
+        ("positive_guard", C.c_int32), ("cool_per_block", C.c_int32), ("use2d", C.c_int32),
+        ("resum", C.c_int32), ("max_moves", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LevelSettings(C.Structure):
+    _fields_ = [
+        ("max_temp", C.c_double), ("dt_temp", C.c_double), ("improvement", C.c_double),
+        ("tjs", C.c_float), ("tjc", C.c_float),
+        ("min_successes", C.c_int32), ("steps", C.c_int32), ("milestone_fails", C.c_int32),
+        ("sphere_radius", C.c_float), ("use2d", C.c_int32),
+    ]
+
+
+class HeatmapProblem(C.Structure):
+    _fields_ = [("n", C.c_int32), ("pos", f32p), ("fixed", u8p), ("dist", f32p),
+                ("diagonal", C.c_int32), ("boundaries", i32p), ("n_boundaries", C.c_int32),
+                ("skip_below", C.c_float)]
+
+
+class ArcsProblem(C.Structure):
+    _fields_ = [("n", C.c_int32), ("pos", f32p), ("fixed", u8p), ("exp_dist", f32p),
+                ("n_loops", C.c_int32), ("loop_pairs", i32p), ("loop_params", f32p),
+                ("k_stretch", C.c_float), ("k_squeeze", C.c_float)]
+
+
+class SmoothProblem(C.Structure):
+    _fields_ = [("n", C.c_int32), ("pos", f32p), ("fixed", u8p), ("dist_to_next", f32p),
+                ("k_stretch", C.c_float), ("k_squeeze", C.c_float), ("k_angular", C.c_float),
+                ("w_dist", C.c_float), ("w_angle", C.c_float)]
+
+
+class Run(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("segment", C.c_uint32), ("chain0", C.c_uint32),
+                ("n_chains", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("energy", C.c_double), ("energy_resummed", C.c_double),
+                ("best_energy", C.c_double), ("moves", C.c_int64), ("accepted", C.c_int64),
+                ("best_chain_moves", C.c_int64), ("best_chain_accepted", C.c_int64),
+                ("best_chain", C.c_int32), ("n_chains", C.c_int32), ("milestones", C.c_int32),
+                ("stop_reason", C.c_int32), ("kernel_ms", C.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["stop_reason"] = STOP_REASONS.get(d["stop_reason"], d["stop_reason"])
+        return d
+
+
+class Job(C.Structure):
+    _fields_ = [("level", C.c_int32), ("problem", C.c_void_p), ("schedule", Schedule),
+                ("run", Run), ("pos_out", f32p), ("stats", C.POINTER(Stats))]
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("blocks_per_step", C.c_int32), ("steps_per_graph", C.c_int32),
+                ("use_graph", C.c_int32)]
+
+
+class SweepSchedule(C.Structure):
+    _fields_ = [("T0", C.c_double), ("dt", C.c_double), ("improvement", C.c_double),
+                ("small_energy", C.c_double),
+                ("tjs", C.c_float), ("tjc", C.c_float), ("step", C.c_float),
+                ("step_decay", C.c_float), ("sphere_radius", C.c_float),
+                ("inner", C.c_int32), ("proposers", C.c_int32), ("fails_threshold", C.c_int32),
+                ("max_iters", C.c_int32), ("use2d", C.c_int32)]
+
+
+class Arc(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in
+                ("start", "end", "genomic_start", "genomic_end", "score", "eff_score", "factor")]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_jobs", C.c_int32),
+                ("blocks_per_step", C.c_int32)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/pmmc_hip.h
+SIGNATURES = {
+    "pmmc_schedule_preset": (C.c_int, [C.c_int, C.c_int, C.POINTER(LevelSettings), C.c_float,
+                                       C.POINTER(Schedule)]),
+    "pmmc_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "pmmc_ctx_destroy": (None, [C.c_void_p]),
+    "pmmc_ctx_reserve": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "pmmc_last_error": (C.c_char_p, [C.c_void_p]),
+    "pmmc_ctx_cu_count": (C.c_int, [C.c_void_p]),
+    "pmmc_set_gpu_seed": (None, [C.c_void_p, C.c_uint64]),
+    "pmmc_heatmap_anneal": (C.c_int, [C.c_void_p, C.POINTER(HeatmapProblem), C.POINTER(Schedule),
+                                      C.POINTER(Run), f32p, C.POINTER(Stats)]),
+    "pmmc_arcs_anneal": (C.c_int, [C.c_void_p, C.POINTER(ArcsProblem), C.POINTER(Schedule),
+                                   C.POINTER(Run), f32p, C.POINTER(Stats)]),
+    "pmmc_smooth_anneal": (C.c_int, [C.c_void_p, C.POINTER(SmoothProblem), C.POINTER(Schedule),
+                                     C.POINTER(Run), f32p, C.POINTER(Stats)]),
+    "pmmc_heatmap_sweep": (C.c_int, [C.c_void_p, C.POINTER(HeatmapProblem),
+                                     C.POINTER(SweepSchedule), C.c_uint64, C.c_uint32, f32p,
+                                     C.POINTER(Stats)]),
+    "pmmc_delta_energy": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_float, C.c_int32, i32p,
+                                    f32p, f32p]),
+    "pmmc_mark_arcs": (C.c_int, [C.c_void_p, i32p, i32p, C.c_int32, i32p, i32p, C.c_int32, i32p,
+                                 i32p]),
+    "pmmc_group_arcs": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, i32p, i32p, i32p,
+                                  C.POINTER(Arc), i32p, i32p]),
+    "pmmc_anneal_batch": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int32, C.POINTER(BatchOpts)]),
+    "pmmc_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
+    "pmmc_server_destroy": (None, [C.c_void_p]),
+    "pmmc_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_uint64)]),
+    "pmmc_server_step": (C.c_int, [C.c_void_p, i32p, i32p, i32p]),
+    "pmmc_server_done": (C.c_int, [C.c_void_p, C.c_uint64]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libpmmc_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} not found: build it with `python -m pmmc_amd.build` "
+            "(the HIP library is required; there is no fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

@@ -1,0 +1,258 @@
+"""numpy-facing wrappers over the C ABI (include/pmmc_hip.h).
+
+Every call goes through libpmmc_hip.so; arrays are converted to contiguous
+float32 / int32 / uint8 and kept alive for the duration of the call.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi as A
+
+
+class PmmcError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{A.STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _u8(a):
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+def _ptr(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def settings(max_temp=5.0, dt_temp=0.9999, improvement=0.99, tjs=50.0, tjc=20.0,
+             min_successes=10, steps=50000, milestone_fails=3, sphere_radius=0.0, use2d=0):
+    """pmmc_level_settings from [simulation_*] / [cuda] values."""
+    return A.LevelSettings(max_temp, dt_temp, improvement, tjs, tjc, min_successes, steps,
+                           milestone_fails, sphere_radius, use2d)
+
+
+def schedule_preset(level: int, preset: int, s: A.LevelSettings, step_size: float) -> A.Schedule:
+    lib = A.load()
+    out = A.Schedule()
+    st = lib.pmmc_schedule_preset(level, preset, C.byref(s), step_size, C.byref(out))
+    if st != 0:
+        raise PmmcError(st, "bad level or preset")
+    return out
+
+
+@dataclass
+class Heatmap:
+    pos: np.ndarray
+    dist: np.ndarray
+    fixed: np.ndarray | None = None
+    diagonal: int = 1
+    boundaries: np.ndarray | None = None
+    skip_below: float = 1e-6
+    _keep: list = field(default_factory=list, repr=False)
+
+    level = A.LEVEL_HEATMAP
+
+    def c(self):
+        pos, dist = _f32(self.pos).reshape(-1), _f32(self.dist).reshape(-1)
+        fx = None if self.fixed is None else _u8(self.fixed)
+        bd = None if self.boundaries is None else _i32(self.boundaries)
+        self._keep = [pos, dist, fx, bd]
+        n = pos.size // 3
+        return A.HeatmapProblem(n, _ptr(pos, A.f32p), _ptr(fx, A.u8p), _ptr(dist, A.f32p),
+                                self.diagonal, _ptr(bd, A.i32p), 0 if bd is None else bd.size,
+                                self.skip_below)
+
+
+@dataclass
+class Arcs:
+    pos: np.ndarray
+    exp_dist: np.ndarray
+    fixed: np.ndarray | None = None
+    loop_pairs: np.ndarray | None = None
+    loop_params: np.ndarray | None = None
+    k_stretch: float = 1.0
+    k_squeeze: float = 1.0
+    _keep: list = field(default_factory=list, repr=False)
+
+    level = A.LEVEL_ARCS
+
+    def c(self):
+        pos, ed = _f32(self.pos).reshape(-1), _f32(self.exp_dist).reshape(-1)
+        fx = None if self.fixed is None else _u8(self.fixed)
+        lp = None if self.loop_pairs is None else _i32(self.loop_pairs).reshape(-1)
+        lq = None if self.loop_params is None else _f32(self.loop_params).reshape(-1)
+        self._keep = [pos, ed, fx, lp, lq]
+        nl = 0 if lp is None else lp.size // 2
+        return A.ArcsProblem(pos.size // 3, _ptr(pos, A.f32p), _ptr(fx, A.u8p), _ptr(ed, A.f32p),
+                             nl, _ptr(lp, A.i32p), _ptr(lq, A.f32p), self.k_stretch,
+                             self.k_squeeze)
+
+
+@dataclass
+class Smooth:
+    pos: np.ndarray
+    dist_to_next: np.ndarray
+    fixed: np.ndarray | None = None
+    k_stretch: float = 0.1
+    k_squeeze: float = 0.1
+    k_angular: float = 0.1
+    w_dist: float = 1.0
+    w_angle: float = 1.0
+    _keep: list = field(default_factory=list, repr=False)
+
+    level = A.LEVEL_SMOOTH
+
+    def c(self):
+        pos, dt = _f32(self.pos).reshape(-1), _f32(self.dist_to_next)
+        fx = None if self.fixed is None else _u8(self.fixed)
+        self._keep = [pos, dt, fx]
+        return A.SmoothProblem(pos.size // 3, _ptr(pos, A.f32p), _ptr(fx, A.u8p),
+                               _ptr(dt, A.f32p), self.k_stretch, self.k_squeeze, self.k_angular,
+                               self.w_dist, self.w_angle)
+
+
+class Context:
+    """One pmmc_ctx on one device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = A.load()
+        h = C.c_void_p()
+        st = self.lib.pmmc_ctx_create(device, C.byref(h))
+        if st != 0:
+            raise PmmcError(st, f"pmmc_ctx_create(device={device}) failed")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.pmmc_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != 0:
+            raise PmmcError(st, self.lib.pmmc_last_error(self.h).decode())
+
+    @property
+    def cu_count(self) -> int:
+        return self.lib.pmmc_ctx_cu_count(self.h)
+
+    def set_gpu_seed(self, seed: int):
+        self.lib.pmmc_set_gpu_seed(self.h, seed)
+
+    # ---- plain ----
+    def anneal(self, prob, sched: A.Schedule, seed=1, segment=0, chain0=0, n_chains=0):
+        """Anneal one problem; returns (positions [n,3], stats dict)."""
+        cp = prob.c()
+        n = cp.n
+        out = _f32(prob.pos).reshape(n, 3).copy()
+        stats = A.Stats()
+        run = A.Run(seed, segment, chain0, n_chains)
+        fn = {A.LEVEL_HEATMAP: self.lib.pmmc_heatmap_anneal, A.LEVEL_ARCS: self.lib.pmmc_arcs_anneal,
+              A.LEVEL_SMOOTH: self.lib.pmmc_smooth_anneal}[prob.level]
+        self._check(fn(self.h, C.byref(cp), C.byref(sched), C.byref(run),
+                       out.ctypes.data_as(A.f32p), C.byref(stats)))
+        return out, stats.as_dict()
+
+    def delta_energy(self, prob, beads, disp, sphere_radius=0.0):
+        cp = prob.c()
+        b = _i32(beads)
+        d = _f32(disp).reshape(-1)
+        out = np.empty(b.size, np.float32)
+        self._check(self.lib.pmmc_delta_energy(self.h, prob.level, C.cast(C.pointer(cp), C.c_void_p),
+                                               sphere_radius, b.size, _ptr(b, A.i32p),
+                                               _ptr(d, A.f32p), _ptr(out, A.f32p)))
+        return out
+
+    def heatmap_sweep(self, prob: Heatmap, sched: A.SweepSchedule, seed=1, segment=0):
+        cp = prob.c()
+        out = _f32(prob.pos).reshape(-1, 3).copy()
+        stats = A.Stats()
+        self._check(self.lib.pmmc_heatmap_sweep(self.h, C.byref(cp), C.byref(sched), seed, segment,
+                                                out.ctypes.data_as(A.f32p), C.byref(stats)))
+        return out, stats.as_dict()
+
+    # ---- arcs marking ----
+    def mark_arcs(self, anc_s, anc_e, arc_s, arc_e):
+        a_s, a_e, r_s, r_e = _i32(anc_s), _i32(anc_e), _i32(arc_s), _i32(arc_e)
+        o_s = np.empty(r_s.size, np.int32)
+        o_e = np.empty(r_s.size, np.int32)
+        self._check(self.lib.pmmc_mark_arcs(self.h, _ptr(a_s, A.i32p), _ptr(a_e, A.i32p), a_s.size,
+                                            _ptr(r_s, A.i32p), _ptr(r_e, A.i32p), r_s.size,
+                                            _ptr(o_s, A.i32p), _ptr(o_e, A.i32p)))
+        return o_s, o_e
+
+    def group_arcs(self, chromosomes):
+        """chromosomes: list of (anchors [A,2], raw arcs [R,4]) ->
+        list of arrays [K,7] (start, end, gstart, gend, score, eff, factor),
+        list of non-matching counts."""
+        anc_off, raw_off = [0], [0]
+        anc, raw = [], []
+        for a, r in chromosomes:
+            a = np.asarray(a, np.int32).reshape(-1, 2)
+            r = np.asarray(r, np.int32).reshape(-1, 4)
+            anc.append(a)
+            raw.append(r)
+            anc_off.append(anc_off[-1] + len(a))
+            raw_off.append(raw_off[-1] + len(r))
+        A_ = np.concatenate(anc) if anc else np.zeros((0, 2), np.int32)
+        R_ = np.concatenate(raw) if raw else np.zeros((0, 4), np.int32)
+        a_s, a_e = _i32(A_[:, 0]), _i32(A_[:, 1])
+        rr = _i32(R_).reshape(-1)
+        ao, ro = _i32(anc_off), _i32(raw_off)
+        out = (A.Arc * max(1, 2 * raw_off[-1]))()
+        out_off = np.zeros(len(chromosomes) + 1, np.int32)
+        nonm = np.zeros(max(1, len(chromosomes)), np.int32)
+        self._check(self.lib.pmmc_group_arcs(self.h, len(chromosomes), _ptr(ao, A.i32p),
+                                             _ptr(a_s, A.i32p), _ptr(a_e, A.i32p), _ptr(ro, A.i32p),
+                                             _ptr(rr, A.i32p), out, _ptr(out_off, A.i32p),
+                                             _ptr(nonm, A.i32p)))
+        flat = np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_int32)),
+                                     shape=(max(1, 2 * raw_off[-1]) * 7,)).reshape(-1, 7)
+        res = [flat[out_off[c]:out_off[c + 1]].copy() for c in range(len(chromosomes))]
+        return res, nonm[:len(chromosomes)].copy()
+
+    # ---- batched ----
+    def anneal_batch(self, problems, schedules, seeds, segments=None, n_chains=0,
+                     blocks_per_step=0, steps_per_graph=0, use_graph=True):
+        """Run many problems concurrently; returns list of (pos, stats)."""
+        k = len(problems)
+        jobs = (A.Job * k)()
+        keep, outs, stats = [], [], []
+        for i, p in enumerate(problems):
+            cp = p.c()
+            keep.append(cp)
+            o = _f32(p.pos).reshape(-1, 3).copy()
+            s = A.Stats()
+            outs.append(o)
+            stats.append(s)
+            seg = i if segments is None else segments[i]
+            nc = n_chains[i] if isinstance(n_chains, (list, tuple, np.ndarray)) else n_chains
+            sch = schedules[i] if isinstance(schedules, (list, tuple)) else schedules
+            jobs[i] = A.Job(p.level, C.cast(C.pointer(cp), C.c_void_p), sch,
+                            A.Run(int(seeds[i]), int(seg), 0, int(nc)),
+                            o.ctypes.data_as(A.f32p), C.pointer(s))
+        opts = A.BatchOpts(blocks_per_step, steps_per_graph, 1 if use_graph else 0)
+        self._check(self.lib.pmmc_anneal_batch(self.h, jobs, k, C.byref(opts)))
+        return [(outs[i], stats[i].as_dict()) for i in range(k)]

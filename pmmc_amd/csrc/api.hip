@@ -1,0 +1,311 @@
+// api.hip -- context lifetime, schedule presets and problem preparation.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctx.hpp"
+#include "engine_host.hpp"
+
+using namespace pmmc;
+
+// ------------------------------------------------------------------ //
+// context
+// ------------------------------------------------------------------ //
+extern "C" pmmc_status pmmc_ctx_create(int device, pmmc_ctx **out) {
+  if (!out) return PMMC_ERR_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return PMMC_ERR_NODEVICE;
+  if (device < 0 || device >= count) return PMMC_ERR_INVALID;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return PMMC_ERR_NODEVICE;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return PMMC_ERR_NODEVICE;
+  if (hipSetDevice(device) != hipSuccess) return PMMC_ERR_NODEVICE;
+  pmmc_ctx *c = new pmmc_ctx();
+  c->device = device;
+  c->cu_count = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return PMMC_ERR_HIP;
+  }
+  *out = c;
+  return PMMC_OK;
+}
+
+extern "C" void pmmc_ctx_destroy(pmmc_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->arena) hipFree(c->arena);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" pmmc_status pmmc_ctx_reserve(pmmc_ctx *c, size_t bytes) {
+  if (!c) return PMMC_ERR_INVALID;
+  return arena_reserve(c, bytes);
+}
+
+extern "C" const char *pmmc_last_error(const pmmc_ctx *c) {
+  return c ? c->last_error.c_str() : "null context";
+}
+
+extern "C" int pmmc_ctx_cu_count(const pmmc_ctx *c) { return c ? c->cu_count : 0; }
+
+extern "C" void pmmc_set_gpu_seed(pmmc_ctx *c, uint64_t seed) {
+  if (c) c->default_seed = seed;
+}
+
+// ------------------------------------------------------------------ //
+// schedule presets: the reference annealers as pmmc_schedule
+// ------------------------------------------------------------------ //
+extern "C" pmmc_status pmmc_schedule_preset(int level, int preset, const pmmc_level_settings *s,
+                                            float step_size, pmmc_schedule *o) {
+  if (!s || !o) return PMMC_ERR_INVALID;
+  std::memset(o, 0, sizeof(*o));
+  o->T0 = s->max_temp;
+  o->dt = s->dt_temp;
+  o->improvement = s->improvement;
+  o->tjs = s->tjs;
+  o->tjc = s->tjc;
+  o->sphere_radius = s->sphere_radius;
+  o->use2d = s->use2d;
+  o->min_successes = s->min_successes;
+  o->fails_threshold = s->milestone_fails;
+  o->step_decay = 1.0f;
+  o->resum = 1;
+  o->guard_energy = -INFINITY;
+  o->max_moves = INT32_MAX;
+  const int steps = s->steps > 0 ? s->steps : 1;
+  if (preset == PMMC_PRESET_CPU) {
+    // milestone every `steps` counted moves, stop at the first failing one
+    o->block = steps;
+    o->milestone_period = steps;
+    o->stop_rule = PMMC_STOP_FIRST_FAIL;
+  }
+  switch (level) {
+    case PMMC_LEVEL_HEATMAP:
+      o->guard_energy = 1e-6;  // ParallelMonteCarloHeatmap.cu:465; LooperSolver.hpp:1465
+      if (preset == PMMC_PRESET_CPU) {
+        o->step = 0.5f * step_size;  // LooperSolver.hpp:1426
+        o->small_energy = 1e-6;      // :1550
+      } else if (preset == PMMC_PRESET_GPU) {
+        o->step = 0.75f * step_size;  // ParallelMonteCarloHeatmap.cu:478
+        o->step_decay = 0.95f;        // :286
+        o->block = 512;               // :247
+        o->milestone_period = 512;
+        o->cool_per_block = 1;        // :285
+        o->stop_rule = PMMC_STOP_MISSES_ACCUM;
+        o->min_successes = INT_MAX;   // misses counted on E > thr*ms alone (:343-345)
+        o->small_energy = 1e-4;       // :349
+      } else {
+        return PMMC_ERR_INVALID;
+      }
+      return PMMC_OK;
+    case PMMC_LEVEL_ARCS:
+      o->step = step_size;
+      o->small_energy = 1e-5;  // LooperSolver.hpp:4778; .cu:641
+      o->ratio_stop = 0.9999;  // :4779; .cu:642
+      if (preset == PMMC_PRESET_GPU) {
+        o->block = 256;  // ParallelMonteCarloArcs.cu:520
+        o->milestone_period = 256;
+        o->stop_rule = PMMC_STOP_MISSES_RESET;
+        o->positive_guard = 1;  // .cu:599
+        o->max_moves = steps;   // .cu:522
+      } else if (preset != PMMC_PRESET_CPU) {
+        return PMMC_ERR_INVALID;
+      }
+      return PMMC_OK;
+    case PMMC_LEVEL_SMOOTH:
+      o->step = step_size;
+      o->small_energy = 1e-6;  // LooperSolver.hpp:5030; .cu:325
+      o->strict_accept = 1;    // LooperSolver.hpp:4974; .cu:297
+      if (preset == PMMC_PRESET_GPU) {
+        o->block = 256;  // ParallelMonteCarloSmooth.cu:269
+        o->milestone_period = steps;  // .cu:320
+        o->stop_rule = PMMC_STOP_MISSES_ACCUM;
+        o->max_moves = steps;  // .cu:267
+      } else if (preset != PMMC_PRESET_CPU) {
+        return PMMC_ERR_INVALID;
+      }
+      return PMMC_OK;
+    default:
+      return PMMC_ERR_INVALID;
+  }
+}
+
+namespace pmmc {
+
+pmmc_status check_schedule(pmmc_ctx *ctx, const pmmc_schedule &s) {
+  if (s.block <= 0 || s.milestone_period <= 0) return fail(ctx, PMMC_ERR_INVALID, "schedule: block and milestone_period must be > 0");
+  if (s.max_moves < 0 || s.max_moves > 0xFFFFFFFFll - s.block) return fail(ctx, PMMC_ERR_INVALID, "schedule: max_moves out of range");
+  if (!(s.dt > 0.0) || !std::isfinite(s.T0)) return fail(ctx, PMMC_ERR_INVALID, "schedule: bad temperature");
+  if (!std::isfinite(s.step) || s.step < 0.f) return fail(ctx, PMMC_ERR_INVALID, "schedule: bad step");
+  if (s.stop_rule < 0 || s.stop_rule > 2) return fail(ctx, PMMC_ERR_INVALID, "schedule: bad stop rule");
+  return PMMC_OK;
+}
+
+int default_chains(const pmmc_ctx *ctx, int /*level*/) {
+  // one 64-chain ensemble per compute unit
+  const int cus = ctx && ctx->cu_count > 0 ? ctx->cu_count : 256;
+  return cus * kLanes;
+}
+
+// heatmap_chromosome_boundaries lookup, LooperSolver.hpp:6034-6050
+static void block_of(const int32_t *b, int nb, int n, int p, int &st, int &en) {
+  st = 0;
+  en = n - 1;
+  if (nb <= 0 || !b) return;
+  if (p < 0 || p > b[nb - 1]) return;
+  for (int i = 0; i + 1 < nb; ++i)
+    if (b[i] <= p && p < b[i + 1]) {
+      st = b[i];
+      en = b[i + 1] - 1;
+      return;
+    }
+}
+
+static pmmc_status prep_heatmap(pmmc_ctx *ctx, const pmmc_heatmap_problem *p, HostProblem &h) {
+  const int n = p->n;
+  if (n < 1 || !p->pos || !p->dist) return fail(ctx, PMMC_ERR_INVALID, "heatmap: n < 1 or null arrays");
+  if (p->diagonal < 0 || p->n_boundaries < 0 || (p->n_boundaries > 0 && !p->boundaries))
+    return fail(ctx, PMMC_ERR_INVALID, "heatmap: bad diagonal or boundaries");
+  const size_t nn = static_cast<size_t>(n) * n;
+  for (size_t i = 0; i < nn; ++i)
+    if (std::isnan(p->dist[i])) return fail(ctx, PMMC_ERR_INVALID, "heatmap: NaN in dist");
+  std::vector<int> st(n), en(n);
+  for (int m = 0; m < n; ++m) block_of(p->boundaries, p->n_boundaries, n, m, st[m], en[m]);
+  const float thr = p->skip_below;
+  auto valid = [&](int i, int m) -> bool {  // term i of local(m)
+    if (i < st[m] || i > en[m]) return false;
+    if (i == m) return false;
+    if (std::abs(i - m) < p->diagonal) return false;
+    return p->dist[static_cast<size_t>(i) * n + m] >= thr;
+  };
+  h.symmetric = true;
+  for (int i = 0; i < n && h.symmetric; ++i)
+    for (int j = i + 1; j < n; ++j)
+      if (std::memcmp(&p->dist[static_cast<size_t>(i) * n + j], &p->dist[static_cast<size_t>(j) * n + i],
+                      sizeof(float)) != 0) {
+        h.symmetric = false;
+        break;
+      }
+  // block partition must be symmetric for the double-counting identity
+  h.W.assign(nn, 0.f);
+  h.Hc.assign(nn, 0.f);
+  if (!h.symmetric) h.W2.assign(nn, 0.f);
+  for (int q = 0; q < n; ++q)
+    for (int j = 0; j < n; ++j) {
+      if (valid(j, q)) {
+        const float hv = p->dist[static_cast<size_t>(j) * n + q];
+        h.W[static_cast<size_t>(q) * n + j] = 1.0f / hv;
+        h.Hc[static_cast<size_t>(q) * n + j] = hv;
+      }
+      if (!h.symmetric && valid(q, j))
+        h.W2[static_cast<size_t>(q) * n + j] = 1.0f / p->dist[static_cast<size_t>(q) * n + j];
+    }
+  h.pos0.assign(p->pos, p->pos + 3 * static_cast<size_t>(n));
+  h.fixed.assign(n, 0);
+  if (p->fixed) h.fixed.assign(p->fixed, p->fixed + n);
+  return PMMC_OK;
+}
+
+static pmmc_status prep_arcs(pmmc_ctx *ctx, const pmmc_arcs_problem *p, HostProblem &h) {
+  const int n = p->n;
+  if (n < 1 || !p->pos || !p->exp_dist) return fail(ctx, PMMC_ERR_INVALID, "arcs: n < 1 or null arrays");
+  if (p->n_loops < 0 || (p->n_loops > 0 && (!p->loop_pairs || !p->loop_params)))
+    return fail(ctx, PMMC_ERR_INVALID, "arcs: bad loop arrays");
+  const float *ed = p->exp_dist;
+  std::vector<std::vector<ArcEntry>> rows(n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const float e = ed[static_cast<size_t>(i) * n + j];
+      if (std::isnan(e)) return fail(ctx, PMMC_ERR_INVALID, "arcs: NaN in exp_dist");
+      if (j == i) continue;
+      if (e >= 1e-6f) {
+        const float e2 = e * e;
+        rows[i].push_back(ArcEntry{j, e, p->k_stretch / e2, p->k_squeeze / e2});
+      } else if (e >= 0.f && j > i) {
+        h.zpairs.push_back(i);
+        h.zpairs.push_back(j);
+      }
+    }
+  for (int k = 0; k < p->n_loops; ++k) {
+    const int a = p->loop_pairs[2 * k], b = p->loop_pairs[2 * k + 1];
+    if (a < 0 || a >= n || b < 0 || b >= n) return fail(ctx, PMMC_ERR_INVALID, "arcs: loop index out of range");
+    if (a == b) continue;  // distance stays 0: contributes a constant, no delta
+    const float kk = p->loop_params[2 * k], r0 = p->loop_params[2 * k + 1];
+    rows[a].push_back(ArcEntry{static_cast<int32_t>(b | INT32_MIN), r0, kk, kk});
+    rows[b].push_back(ArcEntry{static_cast<int32_t>(a | INT32_MIN), r0, kk, kk});
+  }
+  h.off.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) h.off[i + 1] = h.off[i] + static_cast<int32_t>(rows[i].size());
+  h.ent.reserve(h.off[n]);
+  for (int i = 0; i < n; ++i) h.ent.insert(h.ent.end(), rows[i].begin(), rows[i].end());
+  h.ks = p->k_stretch;
+  h.kq = p->k_squeeze;
+  h.pos0.assign(p->pos, p->pos + 3 * static_cast<size_t>(n));
+  h.fixed.assign(n, 0);
+  if (p->fixed) h.fixed.assign(p->fixed, p->fixed + n);
+  return PMMC_OK;
+}
+
+static pmmc_status prep_smooth(pmmc_ctx *ctx, const pmmc_smooth_problem *p, HostProblem &h) {
+  const int n = p->n;
+  if (n < 1 || !p->pos || !p->dist_to_next) return fail(ctx, PMMC_ERR_INVALID, "smooth: n < 1 or null arrays");
+  h.dtn.assign(p->dist_to_next, p->dist_to_next + n);
+  h.ks = p->k_stretch;
+  h.kq = p->k_squeeze;
+  h.ka = p->k_angular;
+  h.wd = p->w_dist;
+  h.wa = p->w_angle;
+  h.pos0.assign(p->pos, p->pos + 3 * static_cast<size_t>(n));
+  h.fixed.assign(n, 0);
+  if (p->fixed) h.fixed.assign(p->fixed, p->fixed + n);
+  return PMMC_OK;
+}
+
+pmmc_status prepare_problem(pmmc_ctx *ctx, int level, const void *problem, const pmmc_schedule &s,
+                            const pmmc_run &run, HostProblem &h) {
+  if (!problem) return fail(ctx, PMMC_ERR_INVALID, "null problem");
+  pmmc_status ps = check_schedule(ctx, s);
+  if (ps != PMMC_OK) return ps;
+  h.level = level;
+  switch (level) {
+    case PMMC_LEVEL_HEATMAP: ps = prep_heatmap(ctx, static_cast<const pmmc_heatmap_problem *>(problem), h); break;
+    case PMMC_LEVEL_ARCS: ps = prep_arcs(ctx, static_cast<const pmmc_arcs_problem *>(problem), h); break;
+    case PMMC_LEVEL_SMOOTH: ps = prep_smooth(ctx, static_cast<const pmmc_smooth_problem *>(problem), h); break;
+    default: return fail(ctx, PMMC_ERR_INVALID, "unknown level");
+  }
+  if (ps != PMMC_OK) return ps;
+  h.n = static_cast<int>(h.pos0.size() / 3);
+  for (int i = 0; i < h.n; ++i)
+    if (!h.fixed[i]) h.free_list.push_back(i);
+  h.sched = s;
+  h.seed = run.seed ? run.seed : ctx->default_seed;
+  h.segment = run.segment;
+  if (run.chain0 % kLanes != 0) return fail(ctx, PMMC_ERR_INVALID, "run.chain0 must be a multiple of 64");
+  h.chain0 = run.chain0;
+  h.n_chains = run.n_chains > 0 ? run.n_chains : default_chains(ctx, level);
+  if (static_cast<uint64_t>(h.chain0) + h.n_chains > 0x7FFFFFFFull)
+    return fail(ctx, PMMC_ERR_INVALID, "chain ids exceed 2^31");
+  return PMMC_OK;
+}
+
+// Blocks of moves per step launch: about 8k moves per chain per step, so a
+// step is long enough to amortise the launch yet short enough for admission.
+int autotune_blocks_per_step(const std::vector<HostProblem> &hp) {
+  int blk = 1;
+  for (const auto &h : hp) blk = std::max(blk, h.sched.block);
+  const int b = 8192 / blk;
+  return b < 1 ? 1 : b;
+}
+
+}  // namespace pmmc

@@ -1,0 +1,87 @@
+// ctx.hpp -- the opaque pmmc_ctx (device, stream, grow-only arena) and the
+// error plumbing of the C ABI.  Replaces the reference's GpuResourceCache
+// (LooperSolver.hpp:54-106, ParallelMonteCarloArcs.cu:84-213): one context
+// owns every device buffer, buffers only grow, and nothing is freed between
+// calls.  Errors become pmmc_status codes (the reference exit()s,
+// ParallelMonteCarloArcs.cu:60-70).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "../../include/pmmc_hip.h"
+
+struct pmmc_ctx {
+  int device = 0;
+  int cu_count = 0;
+  hipStream_t stream = nullptr;
+  uint64_t default_seed = 0;
+  std::string last_error;
+  // grow-only device arena, carved per call
+  char *arena = nullptr;
+  size_t arena_cap = 0;
+  size_t arena_top = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace pmmc {
+
+inline pmmc_status fail(pmmc_ctx *c, pmmc_status st, const std::string &msg) {
+  if (c) c->last_error = msg;
+  return st;
+}
+
+#define PMMC_HIP_TRY(ctx, call)                                                          \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return ::pmmc::fail((ctx), PMMC_ERR_HIP,                                           \
+                          std::string(#call) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+// Ensure the arena holds `bytes`; contents are discarded when it grows.
+inline pmmc_status arena_reserve(pmmc_ctx *c, size_t bytes) {
+  if (bytes <= c->arena_cap) return PMMC_OK;
+  if (c->arena) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, PMMC_ERR_HIP, hipGetErrorString(e));
+    hipFree(c->arena);
+    c->arena = nullptr;
+    c->arena_cap = 0;
+  }
+  size_t cap = bytes + (bytes >> 3) + (1u << 20);
+  if (hipMalloc(&c->arena, cap) != hipSuccess) {
+    c->arena = nullptr;
+    return fail(c, PMMC_ERR_NOMEM, "device arena allocation failed");
+  }
+  c->arena_cap = cap;
+  return PMMC_OK;
+}
+
+// Bump allocator over a reserved span (256-byte aligned).
+struct Carver {
+  char *base;
+  size_t top = 0;
+  size_t cap;
+  template <class T>
+  T *take(size_t count) {
+    top = (top + 255) & ~size_t(255);
+    T *p = reinterpret_cast<T *>(base + top);
+    top += sizeof(T) * count;
+    return p;
+  }
+};
+
+// Bytes a Carver needs for a list of (sizeof, count) requests.
+struct Sizer {
+  size_t top = 0;
+  template <class T>
+  void add(size_t count) {
+    top = (top + 255) & ~size_t(255);
+    top += sizeof(T) * count;
+  }
+};
+
+}  // namespace pmmc

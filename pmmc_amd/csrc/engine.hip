@@ -1,0 +1,928 @@
+// engine.hip -- lockstep-ensemble simulated annealing for the heatmap, arc
+// and smooth levels of pMMC, plus the plain and batched C entry points.
+//
+// Reference kernels this replaces:
+//   MonteCarloHeatmapKernel  ParallelMonteCarloHeatmap.cu:217-363 (chain form:
+//                            the CPU annealer LooperSolver.hpp:1424-1565 on the GPU)
+//   MonteCarloArcsKernel     ParallelMonteCarloArcs.cu:407-675
+//   MonteCarloSmoothKernelMultiWarp ParallelMonteCarloSmooth.cu:231-349
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctx.hpp"
+#include "engine_host.hpp"
+#include "kernels.hpp"
+
+namespace pmmc {
+
+// ------------------------------------------------------------------ //
+// FP64 exact energies (initial totals and the returned energies)
+// ------------------------------------------------------------------ //
+__device__ __forceinline__ double block_sum_d(double v, double *red) {
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+  for (int s = blockDim.x >> 1; s > 0; s >>= 1) {
+    if (t < s) red[t] += red[t + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// heatmap full score (calcScoreHeatmapActiveRegion(), LooperSolver.hpp:3583-3617)
+// over AoS positions, FP64 terms.
+__device__ double exact_heatmap(const DevProblem &P, const float *x, double *red) {
+  const int n = P.n;
+  double s = 0.0;
+  for (int m = threadIdx.x; m < n; m += blockDim.x) {
+    const int j0 = P.symmetric ? m + 1 : 0;
+    for (int j = j0; j < n; ++j) {
+      const float h = P.Hc[static_cast<size_t>(m) * n + j];
+      if (h == 0.f) continue;
+      const double d = dist3d(x[3 * m], x[3 * m + 1], x[3 * m + 2], x[3 * j], x[3 * j + 1],
+                              x[3 * j + 2]);
+      const double c = (d - h) / h;
+      s += c * c;
+    }
+  }
+  double tot = block_sum_d(s, red);
+  return P.symmetric ? 2.0 * tot : tot;
+}
+
+__device__ double exact_sphere(const DevProblem &P, const float *x, float r, double *red) {
+  double s = 0.0;
+  if (r > 0.f)
+    for (int m = threadIdx.x; m < P.n; m += blockDim.x)
+      s += sphere_penalty_d(x[3 * m], x[3 * m + 1], x[3 * m + 2], r);
+  return block_sum_d(s, red);
+}
+
+// arc + loop terms, FP64 (LooperSolver.hpp:3215-3246 without repulsion, :3339-3353)
+__device__ double exact_arcs_terms(const DevProblem &P, const float *x, double *red) {
+  double s = 0.0;
+  for (int p = threadIdx.x; p < P.n; p += blockDim.x) {
+    for (int e = P.off[p]; e < P.off[p + 1]; ++e) {
+      const ArcEntry en = P.ent[e];
+      const int j = en.j & 0x7fffffff;
+      if (j <= p) continue;
+      const double d = dist3d(x[3 * p], x[3 * p + 1], x[3 * p + 2], x[3 * j], x[3 * j + 1],
+                              x[3 * j + 2]);
+      if (en.j < 0) {  // loop constraint
+        const double t = d - en.c;
+        s += static_cast<double>(en.ka) * t * t;
+      } else {
+        const double diff = (d - en.c) / en.c;
+        s += diff * diff * (diff >= 0.0 ? P.ks : P.kq);
+      }
+    }
+  }
+  return block_sum_d(s, red);
+}
+
+// repulsion sum over pairs with exp_dist < 0 (LooperSolver.hpp:3228-3231):
+// all pairs minus the pairs that carry an arc or an explicit zero.
+__device__ double exact_repulsion(const DevProblem &P, const float *x, double *red) {
+  const int n = P.n;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+    for (int j = i + 1; j < n; ++j) {
+      const double dx = xi - x[3 * j], dy = yi - x[3 * j + 1], dz = zi - x[3 * j + 2];
+      const double d = sqrt(dx * dx + dy * dy + dz * dz);
+      s += 1.0 / (d > 1e-6 ? d : 1e-6);
+    }
+    for (int e = P.off[i]; e < P.off[i + 1]; ++e) {
+      const ArcEntry en = P.ent[e];
+      if (en.j < 0) continue;  // loops do not replace repulsion
+      const int j = en.j;
+      if (j <= i) continue;
+      const double d = dist3d(x[3 * i], x[3 * i + 1], x[3 * i + 2], x[3 * j], x[3 * j + 1],
+                              x[3 * j + 2]);
+      s -= 1.0 / (d > 1e-6 ? d : 1e-6);
+    }
+  }
+  for (int k = threadIdx.x; k < P.nz; k += blockDim.x) {
+    const int i = P.zpairs[2 * k], j = P.zpairs[2 * k + 1];
+    const double d = dist3d(x[3 * i], x[3 * i + 1], x[3 * i + 2], x[3 * j], x[3 * j + 1],
+                            x[3 * j + 2]);
+    s -= 1.0 / (d > 1e-6 ? d : 1e-6);
+  }
+  return block_sum_d(s, red);
+}
+
+// smooth bonds and angles, FP64 (LooperSolver.hpp:3414-3451); out[0] = bonds,
+// out[1] = angles.
+__device__ void exact_smooth(const DevProblem &P, const float *x, double *red, double *out) {
+  const int n = P.n;
+  double sb = 0.0, sa = 0.0;
+  for (int i = threadIdx.x; i + 1 < n; i += blockDim.x) {
+    double d0 = P.dtn[i];
+    if (d0 < 1e-6) d0 = 1e-6;
+    const double d = dist3d(x[3 * i], x[3 * i + 1], x[3 * i + 2], x[3 * i + 3], x[3 * i + 4],
+                            x[3 * i + 5]);
+    const double diff = (d - d0) / d0;
+    sb += diff * diff * (diff >= 0.0 ? P.ks : P.kq);
+    if (i >= 1) {
+      const double ux = (double)x[3 * i - 3] - x[3 * i], uy = (double)x[3 * i - 2] - x[3 * i + 1],
+                   uz = (double)x[3 * i - 1] - x[3 * i + 2];
+      const double wx = (double)x[3 * i] - x[3 * i + 3], wy = (double)x[3 * i + 1] - x[3 * i + 4],
+                   wz = (double)x[3 * i + 2] - x[3 * i + 5];
+      const double lu = sqrt(ux * ux + uy * uy + uz * uz), lw = sqrt(wx * wx + wy * wy + wz * wz);
+      if (lu > 1e-6 && lw > 1e-6) {
+        double cs = (ux * wx + uy * wy + uz * wz) / (lu * lw);
+        cs = cs < -1.0 ? -1.0 : (cs > 1.0 ? 1.0 : cs);
+        const double ang = (1.0 - cs) * 0.5;
+        sa += ang * ang * ang * P.ka;
+      }
+    }
+  }
+  out[0] = block_sum_d(sb, red);
+  out[1] = block_sum_d(sa, red);
+}
+
+// One workgroup (256 threads) per problem: initial running total and offset.
+__global__ __launch_bounds__(256) void k_init_problem(DevProblem *probs) {
+  __shared__ double red[256];
+  DevProblem &P = probs[blockIdx.x];
+  const float *x = P.pos0;
+  const float r = P.sched.sphere_radius;
+  double E0 = 0.0, off = 0.0;
+  if (P.level == PMMC_LEVEL_HEATMAP) {
+    E0 = exact_heatmap(P, x, red) + exact_sphere(P, x, r, red);
+  } else if (P.level == PMMC_LEVEL_ARCS) {
+    off = exact_repulsion(P, x, red);
+    E0 = off + exact_arcs_terms(P, x, red) + exact_sphere(P, x, r, red);
+  } else {
+    double bs[2];
+    exact_smooth(P, x, red, bs);
+    const double sph = exact_sphere(P, x, r, red);
+    const double W0 = bs[0] * P.wd + bs[1] * P.wa + sph;
+    const double U0 = bs[0] + bs[1] + sph;
+    E0 = W0;
+    off = W0 - U0;
+  }
+  if (threadIdx.x == 0) {
+    *P.E0 = E0;
+    *P.offset = off;
+  }
+}
+
+// One wavefront per ensemble: replicate pos0 into the SoA lanes, init state.
+__global__ __launch_bounds__(64) void k_init_ens(const DevProblem *__restrict__ probs,
+                                                 const DevEns *__restrict__ ens) {
+  const DevEns it = ens[blockIdx.x];
+  const DevProblem &P = probs[it.problem];
+  const int lane = threadIdx.x;
+  const int n3 = 3 * P.n;
+  for (int i = 0; i < n3; ++i) it.pos[static_cast<size_t>(i) * kLanes + lane] = P.pos0[i];
+  EnsState *st = it.st;
+  const double E0 = *P.E0;
+  int status = 0;
+  if (lane >= it.nlanes) status = -1;
+  else if (!(E0 == E0) || E0 < P.sched.guard_energy) status = PMMC_STOP_GUARD;
+  else if (P.nfree == 0) status = PMMC_STOP_NOFREE;
+  st->E[lane] = E0;
+  st->T[lane] = P.sched.T0;
+  st->ms[lane] = E0;
+  st->best[lane] = E0;
+  st->step[lane] = P.sched.step;
+  st->iters[lane] = 0;
+  st->acc[lane] = 0;
+  st->msucc[lane] = 0;
+  st->misses[lane] = 0;
+  st->milestones[lane] = 0;
+  st->status[lane] = status;
+}
+
+// ------------------------------------------------------------------ //
+// the annealing step kernel
+// ------------------------------------------------------------------ //
+template <int LEVEL, bool STAGE>
+__global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ probs,
+                                               const DevEns *__restrict__ ens, int max_blocks) {
+  extern __shared__ float lds[];
+  const DevEns it = ens[blockIdx.x];
+  const DevProblem &P = probs[it.problem];
+  const int lane = threadIdx.x;
+  EnsState *__restrict__ st = it.st;
+  int status = st->status[lane];
+  if (__ballot(status == 0) == 0) return;
+
+  const int n = P.n;
+  PosView X;
+  if constexpr (STAGE) {
+    for (int i = 0; i < 3 * n; ++i) lds[i * kLanes + lane] = it.pos[static_cast<size_t>(i) * kLanes + lane];
+    X.p = lds + lane;
+  } else {
+    X.p = it.pos + lane;
+  }
+
+  const pmmc_schedule S = P.sched;
+  const uint64_t seed = P.seed;
+  const uint32_t segment = P.segment;
+  const int nfree = P.nfree;
+  const float r = S.sphere_radius;
+  const double offset = (LEVEL == PMMC_LEVEL_HEATMAP) ? 0.0 : *P.offset;
+
+  double E = st->E[lane], T = st->T[lane], ms = st->ms[lane], best = st->best[lane];
+  float step = st->step[lane];
+  uint32_t iters = st->iters[lane], acc = st->acc[lane];
+  int msucc = st->msucc[lane], misses = st->misses[lane], miles = st->milestones[lane];
+  bool active = (status == 0);
+  const uint32_t chain = it.chain0 + static_cast<uint32_t>(lane);
+  const uint32_t group = it.chain0 >> kPickShift;
+
+  for (int b = 0; b < max_blocks; ++b) {
+    if (active && static_cast<int64_t>(iters) >= S.max_moves) {
+      status = PMMC_STOP_CAP;
+      active = false;
+    }
+    const uint64_t am = __ballot(active);
+    if (am == 0) break;
+    const uint32_t m0 = readlane_u32(iters, __builtin_ctzll(am));
+    U4 rb = {0, 0, 0, 0};
+    for (int k = 0; k < S.block; ++k) {
+      const uint32_t m = m0 + static_cast<uint32_t>(k);
+      if (k == 0 || (m & 3u) == 0) rb = draw(seed, m >> 2, group, segment, LEVEL, kSubBead);
+      const uint32_t sel = m & 3u;
+      const uint32_t word = sel == 0 ? rb.x : (sel == 1 ? rb.y : (sel == 2 ? rb.z : rb.w));
+      const int p = P.free_list[pick(word, static_cast<uint32_t>(nfree))];
+      const U4 rc = draw(seed, m, chain, segment, LEVEL, kSubChain);
+      const float dx = disp(rc.x, step), dy = disp(rc.y, step);
+      const float dz = S.use2d ? 0.f : disp(rc.z, step);
+      const float px = X.get(p, 0), py = X.get(p, 1), pz = X.get(p, 2);
+      const float nx = px + dx, ny = py + dy, nz = pz + dz;
+      float dE;
+      if constexpr (LEVEL == PMMC_LEVEL_HEATMAP) {
+        dE = heatmap_delta(P, X, p, px, py, pz, nx, ny, nz);
+      } else if constexpr (LEVEL == PMMC_LEVEL_ARCS) {
+        dE = arcs_delta(P, X, p, px, py, pz, nx, ny, nz);
+      } else {
+        float qx[5], qy[5], qz[5];
+#pragma unroll
+        for (int o = 0; o < 5; ++o) {
+          const int idx = p - 2 + o;
+          const bool in = (o != 2) && idx >= 0 && idx < n;
+          qx[o] = in ? X.get(idx, 0) : 0.f;
+          qy[o] = in ? X.get(idx, 1) : 0.f;
+          qz[o] = in ? X.get(idx, 2) : 0.f;
+        }
+        const float d0m = p > 0 ? fmaxf(P.dtn[p - 1], 1e-6f) : 1.f;
+        const float d0p = p < n - 1 ? fmaxf(P.dtn[p], 1e-6f) : 1.f;
+        const float eo = smooth_local(P, p, qx, qy, qz, px, py, pz, d0m, d0p);
+        const float en = smooth_local(P, p, qx, qy, qz, nx, ny, nz, d0m, d0p);
+        dE = en - eo;
+      }
+      if (r > 0.f) dE += sphere_penalty(nx, ny, nz, r) - sphere_penalty(px, py, pz, r);
+      const double En = E + static_cast<double>(dE);
+      const bool ok = active && metropolis(S, E, En, T, u01(rc.w));
+      if (ok) {
+        X.set(p, 0, nx);
+        X.set(p, 1, ny);
+        X.set(p, 2, nz);
+        E = En;
+        ++msucc;
+        ++acc;
+        best = E < best ? E : best;
+      }
+      if (active && !S.cool_per_block) T *= S.dt;
+    }
+    if (active) {
+      iters += static_cast<uint32_t>(S.block);
+      if (S.cool_per_block) T *= S.dt;
+      step *= S.step_decay;
+      if ((iters % static_cast<uint32_t>(S.milestone_period)) < static_cast<uint32_t>(S.block)) {
+        ++miles;
+        if (S.resum) {
+          if constexpr (LEVEL == PMMC_LEVEL_HEATMAP) E = resum_heatmap(P, X, r);
+          else if constexpr (LEVEL == PMMC_LEVEL_ARCS) E = resum_arcs(P, X, r, offset);
+          else E = resum_smooth(P, X, r, offset);
+        }
+        if (!isfinite(E)) {
+          status = PMMC_STOP_NONFINITE;
+          active = false;
+        } else {
+          const bool no_imp = (E > S.improvement * ms && msucc < S.min_successes) ||
+                              E < S.small_energy ||
+                              (S.ratio_stop > 0.0 && ms > 1e-10 && E / ms > S.ratio_stop);
+          if (S.stop_rule == PMMC_STOP_FIRST_FAIL) {
+            if (no_imp) {
+              status = E < S.small_energy ? PMMC_STOP_SMALL : PMMC_STOP_RULE;
+              active = false;
+            }
+          } else {
+            if (no_imp) ++misses;
+            else if (S.stop_rule == PMMC_STOP_MISSES_RESET) misses = 0;
+            if (E < S.small_energy) {
+              status = PMMC_STOP_SMALL;
+              active = false;
+            } else if (misses >= S.fails_threshold) {
+              status = PMMC_STOP_RULE;
+              active = false;
+            }
+          }
+          if (active) {
+            ms = E;
+            msucc = 0;
+          }
+        }
+      }
+    }
+  }
+  if (status != -1) {
+    st->E[lane] = E;
+    st->T[lane] = T;
+    st->ms[lane] = ms;
+    st->best[lane] = best;
+    st->step[lane] = step;
+    st->iters[lane] = iters;
+    st->acc[lane] = acc;
+    st->msucc[lane] = msucc;
+    st->misses[lane] = misses;
+    st->milestones[lane] = miles;
+    st->status[lane] = status;
+  }
+  if constexpr (STAGE) {
+    for (int i = 0; i < 3 * n; ++i) it.pos[static_cast<size_t>(i) * kLanes + lane] = lds[i * kLanes + lane];
+  }
+}
+
+// Count lanes still running (for step loops).
+__global__ __launch_bounds__(64) void k_count_active(const DevEns *__restrict__ ens, int n_ens,
+                                                     int *count) {
+  int c = 0;
+  for (int e = blockIdx.x; e < n_ens; e += gridDim.x) c += ens[e].st->status[threadIdx.x] == 0;
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  if (threadIdx.x == 0 && c) atomicAdd(count, c);
+}
+
+__global__ __launch_bounds__(64) void k_zero_int(int *p) {
+  if (threadIdx.x == 0) *p = 0;
+}
+
+__global__ __launch_bounds__(64) void k_zero_n(int *p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// running lanes per problem index (counts zeroed first)
+__global__ __launch_bounds__(64) void k_count_per_problem(const DevEns *__restrict__ ens, int n_ens,
+                                                          int *counts) {
+  for (int e = blockIdx.x; e < n_ens; e += gridDim.x) {
+    int c = ens[e].st->status[threadIdx.x] == 0;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if (threadIdx.x == 0 && c) atomicAdd(&counts[ens[e].problem], c);
+  }
+}
+
+hipError_t launch_count_per_problem(hipStream_t s, const DevEns *ens, int n_ens, int *counts,
+                                    int n_counts) {
+  hipLaunchKernelGGL(k_zero_n, dim3(std::max(1, std::min(64, (n_counts + 63) / 64))), dim3(64), 0, s,
+                     counts, n_counts);
+  if (n_ens > 0)
+    hipLaunchKernelGGL(k_count_per_problem, dim3(std::min(n_ens, 1024)), dim3(64), 0, s, ens, n_ens,
+                       counts);
+  return hipGetLastError();
+}
+
+// One workgroup (256 threads) per problem: best chain, AoS gather, FP64
+// return value, move counts.
+__global__ __launch_bounds__(256) void k_finalize(DevProblem *probs, const DevEns *__restrict__ ens) {
+  __shared__ double red[256];
+  __shared__ double bE[256];
+  __shared__ int bC[256];
+  __shared__ long long sm[256], sa[256];
+  DevProblem &P = probs[blockIdx.x];
+  const int t = threadIdx.x;
+  double myE = INFINITY;
+  int myC = INT_MAX;
+  long long mv = 0, ac = 0;
+  const int total_lanes = P.n_ens * kLanes;
+  for (int q = t; q < total_lanes; q += blockDim.x) {
+    const DevEns &it = ens[P.ens0 + q / kLanes];
+    const int lane = q % kLanes;
+    if (lane >= it.nlanes) continue;
+    const EnsState *st = it.st;
+    const int c = static_cast<int>(it.chain0) + lane;
+    mv += st->iters[lane];
+    ac += st->acc[lane];
+    const double e = st->E[lane];
+    if (myC == INT_MAX || e < myE || (e == myE && c < myC)) {
+      myE = e;
+      myC = c;
+    }
+  }
+  bE[t] = myE;
+  bC[t] = myC;
+  sm[t] = mv;
+  sa[t] = ac;
+  __syncthreads();
+  for (int s = blockDim.x >> 1; s > 0; s >>= 1) {
+    if (t < s) {
+      const double e2 = bE[t + s];
+      const int c2 = bC[t + s];
+      const bool take = (bC[t] == INT_MAX) || (c2 != INT_MAX && (e2 < bE[t] || (e2 == bE[t] && c2 < bC[t])));
+      if (take) {
+        bE[t] = e2;
+        bC[t] = c2;
+      }
+      sm[t] += sm[t + s];
+      sa[t] += sa[t + s];
+    }
+    __syncthreads();
+  }
+  const int bc = bC[0];
+  const int q = bc - static_cast<int>(P.chain0);
+  const int be = P.ens0 + q / kLanes, bl = q % kLanes;
+  const DevEns &bit = ens[be];
+  for (int i = t; i < 3 * P.n; i += blockDim.x)
+    P.out_pos[i] = bit.pos[static_cast<size_t>(i) * kLanes + bl];
+  __syncthreads();
+  const float *x = P.out_pos;
+  double ret;
+  if (P.level == PMMC_LEVEL_HEATMAP) {
+    ret = exact_heatmap(P, x, red);
+  } else if (P.level == PMMC_LEVEL_ARCS) {
+    ret = *P.offset + exact_arcs_terms(P, x, red) + exact_sphere(P, x, P.sched.sphere_radius, red);
+  } else {
+    double bs[2];
+    exact_smooth(P, x, red, bs);
+    ret = bs[0] * P.wd + bs[1] * P.wa;
+  }
+  if (t == 0) {
+    DevResult *R = P.result;
+    const EnsState *st = bit.st;
+    R->energy = st->E[bl];
+    R->resummed = ret;
+    R->best = st->best[bl];
+    R->moves = sm[0];
+    R->accepted = sa[0];
+    R->best_moves = st->iters[bl];
+    R->best_accepted = st->acc[bl];
+    R->best_chain = bc;
+    R->milestones = st->milestones[bl];
+    R->stop_reason = st->status[bl];
+    R->n_chains = P.n_chains;
+  }
+}
+
+// ------------------------------------------------------------------ //
+// host side
+// ------------------------------------------------------------------ //
+static constexpr int kStageMaxBytes = 64 * 1024;
+
+bool stage_ok(int n) { return static_cast<size_t>(n) * 3 * kLanes * sizeof(float) <= kStageMaxBytes; }
+
+template <int LEVEL>
+static hipError_t launch_level(hipStream_t s, const DevProblem *probs, const DevEns *ens, int n_ens,
+                               int max_blocks, bool stage, int n_for_stage) {
+  if (n_ens <= 0) return hipSuccess;
+  if (stage) {
+    const size_t lds = static_cast<size_t>(n_for_stage) * 3 * kLanes * sizeof(float);
+    hipLaunchKernelGGL((k_anneal<LEVEL, true>), dim3(n_ens), dim3(kLanes), lds, s, probs, ens,
+                       max_blocks);
+  } else {
+    hipLaunchKernelGGL((k_anneal<LEVEL, false>), dim3(n_ens), dim3(kLanes), 0, s, probs, ens,
+                       max_blocks);
+  }
+  return hipGetLastError();
+}
+
+// Everything a batch needs on the device.
+hipError_t launch_groups(hipStream_t s, const DevProblem *probs, const DevEns *ens,
+                         const std::vector<LaunchGroup> &groups, int max_blocks) {
+  hipError_t err = hipSuccess;
+  for (const auto &g : groups) {
+    const DevEns *e = ens + g.ens0;
+    switch (g.level) {
+      case PMMC_LEVEL_HEATMAP:
+        err = launch_level<PMMC_LEVEL_HEATMAP>(s, probs, e, g.n_ens, max_blocks, g.stage, g.n);
+        break;
+      case PMMC_LEVEL_ARCS:
+        err = launch_level<PMMC_LEVEL_ARCS>(s, probs, e, g.n_ens, max_blocks, g.stage, g.n);
+        break;
+      default:
+        err = launch_level<PMMC_LEVEL_SMOOTH>(s, probs, e, g.n_ens, max_blocks, g.stage, g.n);
+    }
+    if (err != hipSuccess) return err;
+  }
+  return err;
+}
+
+hipError_t launch_init(hipStream_t s, DevProblem *probs, int n_probs, const DevEns *ens, int n_ens) {
+  hipLaunchKernelGGL(k_init_problem, dim3(n_probs), dim3(256), 0, s, probs);
+  hipLaunchKernelGGL(k_init_ens, dim3(n_ens), dim3(kLanes), 0, s, probs, ens);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(hipStream_t s, DevProblem *probs, int n_probs, const DevEns *ens) {
+  hipLaunchKernelGGL(k_finalize, dim3(n_probs), dim3(256), 0, s, probs, ens);
+  return hipGetLastError();
+}
+
+static void size_problem(Sizer &z, const HostProblem &h) {
+  z.add<int32_t>(std::max<size_t>(1, h.free_list.size()));
+  z.add<float>(h.pos0.size());
+  z.add<float>(h.W.size());
+  z.add<float>(h.W2.size());
+  z.add<float>(h.Hc.size());
+  z.add<int32_t>(h.off.size());
+  z.add<ArcEntry>(std::max<size_t>(1, h.ent.size()));
+  z.add<int32_t>(std::max<size_t>(1, h.zpairs.size()));
+  z.add<float>(h.dtn.size());
+  z.add<double>(2);
+  z.add<float>(h.pos0.size());  // out_pos
+  const int ne = (h.n_chains + kLanes - 1) / kLanes;
+  z.add<EnsState>(ne);
+  z.add<float>(static_cast<size_t>(ne) * h.pos0.size() * kLanes);
+}
+
+template <class T>
+static T *put(Carver &cv, hipStream_t s, const std::vector<T> &v, hipError_t &err, size_t min_count = 0) {
+  T *p = cv.take<T>(std::max(v.size(), min_count));
+  if (!v.empty() && err == hipSuccess)
+    err = hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s);
+  return p;
+}
+
+size_t problem_bytes(const HostProblem &h) {
+  Sizer z;
+  size_problem(z, h);
+  return z.top + 256;
+}
+
+// Upload one problem into carved device memory; append its ensembles (with
+// problem index `index`) to `ens`.
+void fill_problem(Carver &cv, hipStream_t s, const HostProblem &h, int index, DevResult *result,
+                  DevProblem &P, std::vector<DevEns> &ens, hipError_t &err) {
+  std::memset(&P, 0, sizeof(P));
+  P.level = h.level;
+  P.n = h.n;
+  P.nfree = static_cast<int32_t>(h.free_list.size());
+  P.symmetric = h.symmetric ? 1 : 0;
+  P.free_list = put(cv, s, h.free_list, err, 1);
+  P.pos0 = put(cv, s, h.pos0, err);
+  P.W = put(cv, s, h.W, err);
+  P.W2 = h.W2.empty() ? nullptr : put(cv, s, h.W2, err);
+  P.Hc = put(cv, s, h.Hc, err);
+  P.off = put(cv, s, h.off, err);
+  P.ent = put(cv, s, h.ent, err, 1);
+  P.zpairs = put(cv, s, h.zpairs, err, 1);
+  P.nnz = static_cast<int32_t>(h.ent.size());
+  P.nz = static_cast<int32_t>(h.zpairs.size() / 2);
+  P.ks = h.ks;
+  P.kq = h.kq;
+  P.dtn = put(cv, s, h.dtn, err);
+  P.ka = h.ka;
+  P.wd = h.wd;
+  P.wa = h.wa;
+  double *e0 = cv.take<double>(2);
+  P.E0 = e0;
+  P.offset = e0 + 1;
+  P.sched = h.sched;
+  P.seed = h.seed;
+  P.segment = h.segment;
+  P.chain0 = h.chain0;
+  P.n_chains = h.n_chains;
+  P.out_pos = cv.take<float>(h.pos0.size());
+  P.result = result;
+  const int ne = (h.n_chains + kLanes - 1) / kLanes;
+  EnsState *sts = cv.take<EnsState>(ne);
+  float *pos = cv.take<float>(static_cast<size_t>(ne) * h.pos0.size() * kLanes);
+  P.ens0 = static_cast<int32_t>(ens.size());
+  P.n_ens = ne;
+  for (int e = 0; e < ne; ++e) {
+    DevEns d;
+    d.problem = index;
+    d.chain0 = h.chain0 + static_cast<uint32_t>(e * kLanes);
+    d.nlanes = std::min(kLanes, h.n_chains - e * kLanes);
+    d.pad = 0;
+    d.pos = pos + static_cast<size_t>(e) * h.pos0.size() * kLanes;
+    d.st = sts + e;
+    ens.push_back(d);
+  }
+}
+
+pmmc_status batch_setup(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &bd) {
+  Sizer z;
+  z.add<DevProblem>(hp.size());
+  z.add<DevResult>(hp.size());
+  z.add<int>(64);
+  size_t n_ens = 0;
+  for (auto &h : hp) {
+    size_problem(z, h);
+    n_ens += (h.n_chains + kLanes - 1) / kLanes;
+  }
+  z.add<DevEns>(n_ens);
+  pmmc_status ps = arena_reserve(ctx, z.top + 4096);
+  if (ps != PMMC_OK) return ps;
+  Carver cv{ctx->arena, 0, ctx->arena_cap};
+  hipStream_t s = ctx->stream;
+  hipError_t err = hipSuccess;
+  bd.probs = cv.take<DevProblem>(hp.size());
+  bd.results = cv.take<DevResult>(hp.size());
+  bd.counter = cv.take<int>(64);
+  bd.ens = cv.take<DevEns>(n_ens);
+  bd.hprobs.resize(hp.size());
+  bd.hens.clear();
+  bd.groups.clear();
+  // order ensembles by (level, stage, n) so each launch group is contiguous
+  std::vector<int> order(hp.size());
+  for (size_t i = 0; i < hp.size(); ++i) order[i] = static_cast<int>(i);
+  auto key = [&](int i) {
+    const HostProblem &h = hp[i];
+    const bool stg = stage_ok(h.n);
+    return std::make_tuple(h.level, stg ? 1 : 0, stg ? h.n : 0);
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
+  for (int oi : order) {
+    HostProblem &h = hp[oi];
+    const bool stg = stage_ok(h.n);
+    if (bd.groups.empty() || bd.groups.back().level != h.level || bd.groups.back().stage != stg ||
+        (stg && bd.groups.back().n != h.n))
+      bd.groups.push_back({h.level, stg, stg ? h.n : 0, static_cast<int>(bd.hens.size()), 0});
+    const size_t before = bd.hens.size();
+    fill_problem(cv, s, h, oi, bd.results + oi, bd.hprobs[oi], bd.hens, err);
+    bd.groups.back().n_ens += static_cast<int>(bd.hens.size() - before);
+  }
+  bd.n_ens_total = static_cast<int>(bd.hens.size());
+  if (err == hipSuccess)
+    err = hipMemcpyAsync(bd.probs, bd.hprobs.data(), sizeof(DevProblem) * hp.size(),
+                         hipMemcpyHostToDevice, s);
+  if (err == hipSuccess)
+    err = hipMemcpyAsync(bd.ens, bd.hens.data(), sizeof(DevEns) * bd.hens.size(),
+                         hipMemcpyHostToDevice, s);
+  if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("upload: ") + hipGetErrorString(err));
+  err = launch_init(s, bd.probs, static_cast<int>(hp.size()), bd.ens, bd.n_ens_total);
+  if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("init: ") + hipGetErrorString(err));
+  return PMMC_OK;
+}
+
+hipError_t batch_step(pmmc_ctx *ctx, const BatchDev &bd, int max_blocks) {
+  return launch_groups(ctx->stream, bd.probs, bd.ens, bd.groups, max_blocks);
+}
+
+hipError_t batch_count_active(pmmc_ctx *ctx, const BatchDev &bd, int *host_count) {
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, ctx->stream, bd.counter);
+  const int grid = std::min(bd.n_ens_total, 1024);
+  hipLaunchKernelGGL(k_count_active, dim3(grid), dim3(64), 0, ctx->stream, bd.ens, bd.n_ens_total, bd.counter);
+  hipError_t err = hipMemcpyAsync(host_count, bd.counter, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+  if (err != hipSuccess) return err;
+  return hipStreamSynchronize(ctx->stream);
+}
+
+pmmc_status batch_finish(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &bd,
+                         std::vector<float *> &outs, std::vector<pmmc_anneal_stats *> &stats,
+                         double kernel_ms) {
+  hipStream_t s = ctx->stream;
+  hipError_t err = launch_finalize(s, bd.probs, static_cast<int>(hp.size()), bd.ens);
+  std::vector<DevResult> res(hp.size());
+  if (err == hipSuccess)
+    err = hipMemcpyAsync(res.data(), bd.results, sizeof(DevResult) * hp.size(), hipMemcpyDeviceToHost, s);
+  std::vector<std::vector<float>> tmp(hp.size());
+  for (size_t i = 0; i < hp.size() && err == hipSuccess; ++i) {
+    tmp[i].resize(hp[i].pos0.size());
+    err = hipMemcpyAsync(tmp[i].data(), bd.hprobs[i].out_pos, sizeof(float) * tmp[i].size(),
+                         hipMemcpyDeviceToHost, s);
+  }
+  if (err == hipSuccess) err = hipStreamSynchronize(s);
+  if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("finalize: ") + hipGetErrorString(err));
+  for (size_t i = 0; i < hp.size(); ++i) {
+    if (outs[i]) {
+      if (hp[i].level == PMMC_LEVEL_SMOOTH) {
+        // only non-fixed beads are written back (ParallelMonteCarloSmooth.cu:485-491)
+        for (int b = 0; b < hp[i].n; ++b)
+          if (!hp[i].fixed[b])
+            for (int k = 0; k < 3; ++k) outs[i][3 * b + k] = tmp[i][3 * b + k];
+      } else {
+        std::memcpy(outs[i], tmp[i].data(), sizeof(float) * tmp[i].size());
+      }
+    }
+    if (stats[i]) {
+      pmmc_anneal_stats &S = *stats[i];
+      const DevResult &R = res[i];
+      S.energy = R.energy;
+      S.energy_resummed = R.resummed;
+      S.best_energy = R.best;
+      S.moves = R.moves;
+      S.accepted = R.accepted;
+      S.best_chain_moves = R.best_moves;
+      S.best_chain_accepted = R.best_accepted;
+      S.best_chain = R.best_chain;
+      S.n_chains = R.n_chains;
+      S.milestones = R.milestones;
+      S.stop_reason = R.stop_reason;
+      S.kernel_ms = kernel_ms;
+    }
+  }
+  return PMMC_OK;
+}
+
+// Run a set of prepared problems to completion.  blocks_per_launch <= 0:
+// a single launch per level group that runs every chain to its end.
+pmmc_status run_problems(pmmc_ctx *ctx, std::vector<HostProblem> &hp, std::vector<float *> &outs,
+                         std::vector<pmmc_anneal_stats *> &stats, int blocks_per_launch,
+                         bool use_graph, int steps_per_graph) {
+  BatchDev bd;
+  pmmc_status ps = batch_setup(ctx, hp, bd);
+  if (ps != PMMC_OK) return ps;
+  hipStream_t s = ctx->stream;
+  PMMC_HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+  if (blocks_per_launch <= 0) {
+    hipError_t err = batch_step(ctx, bd, INT_MAX);
+    if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("anneal: ") + hipGetErrorString(err));
+  } else {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    if (use_graph) {
+      if (steps_per_graph <= 0) steps_per_graph = 8;
+      PMMC_HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      hipError_t cerr = hipSuccess;
+      for (int i = 0; i < steps_per_graph && cerr == hipSuccess; ++i) cerr = batch_step(ctx, bd, blocks_per_launch);
+      hipError_t eerr = hipStreamEndCapture(s, &graph);
+      if (cerr != hipSuccess || eerr != hipSuccess)
+        return fail(ctx, PMMC_ERR_HIP, "graph capture failed");
+      PMMC_HIP_TRY(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    for (;;) {
+      hipError_t err;
+      if (use_graph) err = hipGraphLaunch(exec, s);
+      else err = batch_step(ctx, bd, blocks_per_launch);
+      if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("step: ") + hipGetErrorString(err));
+      int active = 0;
+      err = batch_count_active(ctx, bd, &active);
+      if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("count: ") + hipGetErrorString(err));
+      if (active == 0) break;
+    }
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+  }
+  PMMC_HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
+  PMMC_HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  return batch_finish(ctx, hp, bd, outs, stats, ms);
+}
+
+}  // namespace pmmc
+
+// ------------------------------------------------------------------ //
+// C ABI
+// ------------------------------------------------------------------ //
+using namespace pmmc;
+
+static pmmc_status plain(pmmc_ctx *ctx, int level, const void *prob, const pmmc_schedule *s,
+                         const pmmc_run *run, float *pos_out, pmmc_anneal_stats *stats) {
+  if (!ctx || !prob || !s || !run || !pos_out) return fail(ctx, PMMC_ERR_INVALID, "null argument");
+  std::vector<HostProblem> hp(1);
+  pmmc_status ps = prepare_problem(ctx, level, prob, *s, *run, hp[0]);
+  if (ps != PMMC_OK) return ps;
+  std::vector<float *> outs{pos_out};
+  std::vector<pmmc_anneal_stats *> st{stats};
+  return run_problems(ctx, hp, outs, st, 0, false, 0);
+}
+
+extern "C" pmmc_status pmmc_heatmap_anneal(pmmc_ctx *ctx, const pmmc_heatmap_problem *p,
+                                           const pmmc_schedule *s, const pmmc_run *run,
+                                           float *pos_out, pmmc_anneal_stats *stats) {
+  return plain(ctx, PMMC_LEVEL_HEATMAP, p, s, run, pos_out, stats);
+}
+
+extern "C" pmmc_status pmmc_arcs_anneal(pmmc_ctx *ctx, const pmmc_arcs_problem *p,
+                                        const pmmc_schedule *s, const pmmc_run *run,
+                                        float *pos_out, pmmc_anneal_stats *stats) {
+  return plain(ctx, PMMC_LEVEL_ARCS, p, s, run, pos_out, stats);
+}
+
+extern "C" pmmc_status pmmc_smooth_anneal(pmmc_ctx *ctx, const pmmc_smooth_problem *p,
+                                          const pmmc_schedule *s, const pmmc_run *run,
+                                          float *pos_out, pmmc_anneal_stats *stats) {
+  return plain(ctx, PMMC_LEVEL_SMOOTH, p, s, run, pos_out, stats);
+}
+
+extern "C" pmmc_status pmmc_anneal_batch(pmmc_ctx *ctx, const pmmc_job *jobs, int32_t n_jobs,
+                                         const pmmc_batch_opts *opts) {
+  if (!ctx || (!jobs && n_jobs > 0) || n_jobs < 0) return fail(ctx, PMMC_ERR_INVALID, "bad batch");
+  if (n_jobs == 0) return PMMC_OK;
+  std::vector<HostProblem> hp(n_jobs);
+  std::vector<float *> outs(n_jobs);
+  std::vector<pmmc_anneal_stats *> st(n_jobs);
+  for (int i = 0; i < n_jobs; ++i) {
+    if (!jobs[i].pos_out) return fail(ctx, PMMC_ERR_INVALID, "job without pos_out");
+    pmmc_status ps = prepare_problem(ctx, jobs[i].level, jobs[i].problem, jobs[i].schedule, jobs[i].run, hp[i]);
+    if (ps != PMMC_OK) return ps;
+    outs[i] = jobs[i].pos_out;
+    st[i] = jobs[i].stats;
+  }
+  int bps = opts ? opts->blocks_per_step : 0;
+  int spg = opts ? opts->steps_per_graph : 0;
+  bool graph = opts ? opts->use_graph != 0 : false;
+  if (bps <= 0) bps = autotune_blocks_per_step(hp);
+  return run_problems(ctx, hp, outs, st, bps, graph, spg);
+}
+
+// ------------------------------------------------------------------ //
+// per-move ΔE on injected proposals (test hook): the same device functions
+// as k_anneal, evaluated against one fixed state
+// ------------------------------------------------------------------ //
+namespace pmmc {
+
+template <int LEVEL>
+__global__ __launch_bounds__(64) void k_delta(const DevProblem *__restrict__ probs,
+                                              const DevEns *__restrict__ ens, int n_prop,
+                                              const int32_t *__restrict__ beads,
+                                              const float *__restrict__ dsp, float *__restrict__ out) {
+  const DevProblem &P = probs[0];
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x * kLanes + lane;
+  if (i >= n_prop) return;
+  PosView X;
+  X.p = ens[0].pos + lane;
+  const int n = P.n;
+  const int p = beads[i];
+  const float px = X.get(p, 0), py = X.get(p, 1), pz = X.get(p, 2);
+  const float nx = px + dsp[3 * i], ny = py + dsp[3 * i + 1], nz = pz + dsp[3 * i + 2];
+  float dE;
+  if constexpr (LEVEL == PMMC_LEVEL_HEATMAP) {
+    dE = heatmap_delta(P, X, p, px, py, pz, nx, ny, nz);
+  } else if constexpr (LEVEL == PMMC_LEVEL_ARCS) {
+    dE = arcs_delta(P, X, p, px, py, pz, nx, ny, nz);
+  } else {
+    float qx[5], qy[5], qz[5];
+    for (int o = 0; o < 5; ++o) {
+      const int idx = p - 2 + o;
+      const bool in = (o != 2) && idx >= 0 && idx < n;
+      qx[o] = in ? X.get(idx, 0) : 0.f;
+      qy[o] = in ? X.get(idx, 1) : 0.f;
+      qz[o] = in ? X.get(idx, 2) : 0.f;
+    }
+    const float d0m = p > 0 ? fmaxf(P.dtn[p - 1], 1e-6f) : 1.f;
+    const float d0p = p < n - 1 ? fmaxf(P.dtn[p], 1e-6f) : 1.f;
+    dE = smooth_local(P, p, qx, qy, qz, nx, ny, nz, d0m, d0p) -
+         smooth_local(P, p, qx, qy, qz, px, py, pz, d0m, d0p);
+  }
+  const float r = P.sched.sphere_radius;
+  if (r > 0.f) dE += sphere_penalty(nx, ny, nz, r) - sphere_penalty(px, py, pz, r);
+  out[i] = dE;
+}
+
+}  // namespace pmmc
+
+extern "C" pmmc_status pmmc_delta_energy(pmmc_ctx *ctx, int level, const void *problem,
+                                         float sphere_radius, int32_t n_prop, const int32_t *beads,
+                                         const float *disp, float *delta_out) {
+  if (!ctx || !problem || n_prop < 0 || (n_prop > 0 && (!beads || !disp || !delta_out)))
+    return fail(ctx, PMMC_ERR_INVALID, "delta: bad arguments");
+  if (n_prop == 0) return PMMC_OK;
+  pmmc_schedule s;
+  std::memset(&s, 0, sizeof(s));
+  s.T0 = 1.0;
+  s.dt = 1.0;
+  s.block = 1;
+  s.milestone_period = 1;
+  s.sphere_radius = sphere_radius;
+  s.guard_energy = -INFINITY;
+  pmmc_run run{1, 0, 0, kLanes};
+  std::vector<HostProblem> hp(1);
+  pmmc_status ps = prepare_problem(ctx, level, problem, s, run, hp[0]);
+  if (ps != PMMC_OK) return ps;
+  for (int i = 0; i < n_prop; ++i)
+    if (beads[i] < 0 || beads[i] >= hp[0].n) return fail(ctx, PMMC_ERR_INVALID, "delta: bead out of range");
+  BatchDev bd;
+  ps = batch_setup(ctx, hp, bd);
+  if (ps != PMMC_OK) return ps;
+  hipStream_t st = ctx->stream;
+  // proposals live after the batch in a second span of the arena; grow it by
+  // re-running setup if needed
+  const size_t need = (sizeof(int32_t) + 4 * sizeof(float)) * static_cast<size_t>(n_prop) + 4096;
+  int32_t *d_b = nullptr;
+  float *d_d = nullptr, *d_o = nullptr;
+  PMMC_HIP_TRY(ctx, hipMallocAsync(reinterpret_cast<void **>(&d_b), need, st));
+  d_d = reinterpret_cast<float *>(reinterpret_cast<char *>(d_b) + ((sizeof(int32_t) * n_prop + 255) & ~size_t(255)));
+  d_o = d_d + ((3 * static_cast<size_t>(n_prop) + 63) & ~size_t(63));
+  PMMC_HIP_TRY(ctx, hipMemcpyAsync(d_b, beads, sizeof(int32_t) * n_prop, hipMemcpyHostToDevice, st));
+  PMMC_HIP_TRY(ctx, hipMemcpyAsync(d_d, disp, sizeof(float) * 3 * n_prop, hipMemcpyHostToDevice, st));
+  const int grid = (n_prop + kLanes - 1) / kLanes;
+  switch (level) {
+    case PMMC_LEVEL_HEATMAP:
+      hipLaunchKernelGGL(k_delta<PMMC_LEVEL_HEATMAP>, dim3(grid), dim3(kLanes), 0, st, bd.probs, bd.ens, n_prop, d_b, d_d, d_o);
+      break;
+    case PMMC_LEVEL_ARCS:
+      hipLaunchKernelGGL(k_delta<PMMC_LEVEL_ARCS>, dim3(grid), dim3(kLanes), 0, st, bd.probs, bd.ens, n_prop, d_b, d_d, d_o);
+      break;
+    default:
+      hipLaunchKernelGGL(k_delta<PMMC_LEVEL_SMOOTH>, dim3(grid), dim3(kLanes), 0, st, bd.probs, bd.ens, n_prop, d_b, d_d, d_o);
+  }
+  PMMC_HIP_TRY(ctx, hipGetLastError());
+  PMMC_HIP_TRY(ctx, hipMemcpyAsync(delta_out, d_o, sizeof(float) * n_prop, hipMemcpyDeviceToHost, st));
+  PMMC_HIP_TRY(ctx, hipFreeAsync(d_b, st));
+  PMMC_HIP_TRY(ctx, hipStreamSynchronize(st));
+  return PMMC_OK;
+}

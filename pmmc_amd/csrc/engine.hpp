@@ -1,0 +1,87 @@
+// engine.hpp -- device data layout of the lockstep-ensemble annealer.
+//
+// Layout in HBM:
+//   * one ensemble = 64 chains of one problem = one wavefront; lane = chain.
+//   * ensemble positions are SoA  pos[(3*bead + axis) * 64 + lane]  (FP32), so
+//     every position access of a wavefront is one coalesced 256-byte row.
+//   * ensemble state is SoA over the 64 lanes (EnsState, 3.75 KB).
+//   * per-problem static data (heatmap weight rows, arc CSR, dist_to_next,
+//     free-bead list) is read with wave-uniform addresses, i.e. scalar loads.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/pmmc_hip.h"
+
+namespace pmmc {
+
+constexpr int kLanes = 64;
+constexpr int kPickShift = 6;  // chains c and c' share bead picks iff c>>6 == c'>>6
+
+// Philox level field of each annealer.
+constexpr uint32_t kLevelHeatmap = 1, kLevelArcs = 2, kLevelSmooth = 3, kLevelSweep = 4;
+
+struct EnsState {
+  double E[kLanes];     // running total (FP64)
+  double T[kLanes];     // temperature (FP64)
+  double ms[kLanes];    // energy at the previous milestone
+  double best[kLanes];  // lowest running total
+  float step[kLanes];
+  uint32_t iters[kLanes];
+  uint32_t acc[kLanes];
+  int32_t msucc[kLanes];
+  int32_t misses[kLanes];
+  int32_t milestones[kLanes];
+  int32_t status[kLanes];  // 0 running, >0 pmmc_stop_reason, -1 unused lane
+};
+
+// arc CSR entry: partner | (loop << 31), centre c, stretch and squeeze
+// coefficients (arcs: k/c^2, loops: k)
+struct ArcEntry {
+  int32_t j;
+  float c, ka, kb;
+};
+
+struct DevProblem {
+  int32_t level, n, nfree, symmetric;
+  const int32_t *free_list;  // [nfree]
+  const float *pos0;         // [n*3] initial positions (AoS)
+  // heatmap: W[p*n+j] = valid(j,p) ? 1/h[j][p] : 0 ; W2 likewise with h[p][j]
+  // (only for non-symmetric heatmaps); Hc[p*n+j] = valid(j,p) ? h[j][p] : 0
+  const float *W, *W2, *Hc;
+  // arcs
+  const int32_t *off;        // [n+1]
+  const ArcEntry *ent;       // [nnz]
+  const int32_t *zpairs;     // [2*nz] off-diagonal pairs with 0 <= e < 1e-6 (i<j)
+  int32_t nnz, nz;
+  float ks, kq;
+  // smooth
+  const float *dtn;
+  float ka, wd, wa, pad0;
+  // derived at init (device written)
+  double *E0;                // [1] initial running total
+  double *offset;            // [1] arcs: frozen repulsion; smooth: W0 - U0
+  pmmc_schedule sched;
+  uint64_t seed;
+  uint32_t segment, chain0;
+  int32_t ens0, n_ens, n_chains, pad1;
+  float *out_pos;            // [n*3]
+  struct DevResult *result;
+};
+
+struct DevEns {
+  int32_t problem;
+  uint32_t chain0;  // chain id of lane 0 (multiple of 64)
+  int32_t nlanes;
+  int32_t pad;
+  float *pos;       // [n*3*64]
+  EnsState *st;
+};
+
+struct DevResult {
+  double energy, resummed, best;
+  int64_t moves, accepted, best_moves, best_accepted;
+  int32_t best_chain, milestones, stop_reason, n_chains;
+};
+
+}  // namespace pmmc

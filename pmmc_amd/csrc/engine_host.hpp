@@ -1,0 +1,73 @@
+// engine_host.hpp -- host-side problem preparation for the annealer.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "ctx.hpp"
+#include "engine.hpp"
+
+namespace pmmc {
+
+// A validated problem with its derived device tables, still in host memory.
+struct HostProblem {
+  int level = 0, n = 0;
+  std::vector<int32_t> free_list;
+  std::vector<uint8_t> fixed;
+  std::vector<float> pos0;
+  // heatmap tables (see DevProblem)
+  std::vector<float> W, W2, Hc;
+  bool symmetric = true;
+  // arcs CSR
+  std::vector<int32_t> off;
+  std::vector<ArcEntry> ent;
+  std::vector<int32_t> zpairs;
+  float ks = 0.f, kq = 0.f;
+  // smooth
+  std::vector<float> dtn;
+  float ka = 0.f, wd = 1.f, wa = 1.f;
+  pmmc_schedule sched{};
+  uint64_t seed = 0;
+  uint32_t segment = 0, chain0 = 0;
+  int n_chains = 0;
+};
+
+pmmc_status prepare_problem(pmmc_ctx *ctx, int level, const void *problem, const pmmc_schedule &s,
+                            const pmmc_run &run, HostProblem &out);
+pmmc_status check_schedule(pmmc_ctx *ctx, const pmmc_schedule &s);
+int autotune_blocks_per_step(const std::vector<HostProblem> &hp);
+int default_chains(const pmmc_ctx *ctx, int level);
+
+// contiguous range of ensembles launched together: one (level, LDS staging, n)
+struct LaunchGroup {
+  int level;
+  bool stage;
+  int n;
+  int ens0, n_ens;
+};
+
+// everything a batch needs on the device
+struct BatchDev {
+  DevProblem *probs = nullptr;
+  DevEns *ens = nullptr;
+  DevResult *results = nullptr;
+  int *counter = nullptr;
+  std::vector<DevProblem> hprobs;
+  std::vector<DevEns> hens;
+  std::vector<LaunchGroup> groups;
+  int n_ens_total = 0;
+};
+
+bool stage_ok(int n);
+size_t problem_bytes(const HostProblem &h);
+void fill_problem(Carver &cv, hipStream_t s, const HostProblem &h, int index, DevResult *result,
+                  DevProblem &P, std::vector<DevEns> &ens, hipError_t &err);
+hipError_t launch_groups(hipStream_t s, const DevProblem *probs, const DevEns *ens,
+                         const std::vector<LaunchGroup> &groups, int max_blocks);
+hipError_t launch_init(hipStream_t s, DevProblem *probs, int n_probs, const DevEns *ens, int n_ens);
+hipError_t launch_finalize(hipStream_t s, DevProblem *probs, int n_probs, const DevEns *ens);
+hipError_t launch_count_per_problem(hipStream_t s, const DevEns *ens, int n_ens, int *counts,
+                                    int n_counts);
+
+}  // namespace pmmc

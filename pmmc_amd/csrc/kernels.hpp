@@ -1,0 +1,236 @@
+// kernels.hpp -- device code of the lockstep-ensemble annealer (gfx950).
+//
+// One workgroup = one wavefront = one ensemble of 64 chains (lane = chain).
+// Per move, the bead index is wave-uniform (shared pick stream), so all
+// static problem data for the move is read through scalar loads, while each
+// lane reads its own copy of the partner positions from a coalesced SoA row.
+// There is no cross-lane reduction on the move path: every lane sums its own
+// chain's partners; ballots only decide when the ensemble has finished.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "engine.hpp"
+
+namespace pmmc {
+
+// ------------------------------------------------------------------ //
+// helpers
+// ------------------------------------------------------------------ //
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int lane) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), lane));
+}
+
+// Position accessor: element (bead j, axis k) of this lane's chain.
+struct PosView {
+  float *p;  // base + lane
+  __device__ __forceinline__ float get(int j, int k) const { return p[(3 * j + k) * kLanes]; }
+  __device__ __forceinline__ void set(int j, int k, float v) const { p[(3 * j + k) * kLanes] = v; }
+};
+
+__device__ __forceinline__ bool metropolis(const pmmc_schedule &S, double E, double En, double T,
+                                           float U) {
+  bool ok = S.strict_accept ? (En < E) : (En <= E);
+  if (!ok && T > 0.0 && (!S.positive_guard || E > 0.0)) {
+    // tp = tjs * exp(-tjc * (E'/E) / T)   (ParallelMonteCarloArcs.cu:600-603)
+    const float ratio = static_cast<float>(En / E);
+    const float tp = S.tjs * __expf(-S.tjc * ratio / static_cast<float>(T));
+    ok = U < tp;
+  }
+  return ok;
+}
+
+// ------------------------------------------------------------------ //
+// level energies, FP32 move deltas
+// ------------------------------------------------------------------ //
+
+// heatmap: exact change of the full (double-counted) score for moving p from
+// xp to xn; LooperSolver.hpp:3583-3617 summed over all beads.
+__device__ __forceinline__ float heatmap_delta(const DevProblem &P, const PosView &X, int p,
+                                               float px, float py, float pz, float nx, float ny,
+                                               float nz) {
+  const int n = P.n;
+  const float *__restrict__ w1 = P.W + static_cast<size_t>(p) * n;
+  float acc0 = 0.f, acc1 = 0.f;
+  int j = 0;
+  for (; j + 1 < n; j += 2) {
+    const float wa = w1[j], wb = w1[j + 1];
+    const float xa = X.get(j, 0), ya = X.get(j, 1), za = X.get(j, 2);
+    const float xb = X.get(j + 1, 0), yb = X.get(j + 1, 1), zb = X.get(j + 1, 2);
+    const float a0 = fmaf(dist3(px, py, pz, xa, ya, za), wa, -1.f);
+    const float a1 = fmaf(dist3(nx, ny, nz, xa, ya, za), wa, -1.f);
+    const float b0 = fmaf(dist3(px, py, pz, xb, yb, zb), wb, -1.f);
+    const float b1 = fmaf(dist3(nx, ny, nz, xb, yb, zb), wb, -1.f);
+    acc0 = fmaf(a1 - a0, a1 + a0, acc0);
+    acc1 = fmaf(b1 - b0, b1 + b0, acc1);
+  }
+  if (j < n) {
+    const float wa = w1[j];
+    const float xa = X.get(j, 0), ya = X.get(j, 1), za = X.get(j, 2);
+    const float a0 = fmaf(dist3(px, py, pz, xa, ya, za), wa, -1.f);
+    const float a1 = fmaf(dist3(nx, ny, nz, xa, ya, za), wa, -1.f);
+    acc0 = fmaf(a1 - a0, a1 + a0, acc0);
+  }
+  float d = acc0 + acc1;
+  if (P.symmetric) return 2.f * d;
+  const float *__restrict__ w2 = P.W2 + static_cast<size_t>(p) * n;
+  float acc2 = 0.f;
+  for (int k = 0; k < n; ++k) {
+    const float w = w2[k];
+    const float xa = X.get(k, 0), ya = X.get(k, 1), za = X.get(k, 2);
+    const float a0 = fmaf(dist3(px, py, pz, xa, ya, za), w, -1.f);
+    const float a1 = fmaf(dist3(nx, ny, nz, xa, ya, za), w, -1.f);
+    acc2 = fmaf(a1 - a0, a1 + a0, acc2);
+  }
+  return d + acc2;
+}
+
+// arcs: change of arc + loop terms of bead p (ParallelMonteCarloArcs.cu:294-351,
+// LooperSolver.hpp:3250-3281, :3355-3371).  Term (d-c)^2 * (d >= c ? ka : kb).
+__device__ __forceinline__ float arcs_delta(const DevProblem &P, const PosView &X, int p, float px,
+                                            float py, float pz, float nx, float ny, float nz) {
+  const int e0 = P.off[p], e1 = P.off[p + 1];
+  float acc = 0.f;
+  for (int e = e0; e < e1; ++e) {
+    const ArcEntry en = P.ent[e];
+    const int j = en.j & 0x7fffffff;
+    const float xj = X.get(j, 0), yj = X.get(j, 1), zj = X.get(j, 2);
+    const float t0 = dist3(px, py, pz, xj, yj, zj) - en.c;
+    const float t1 = dist3(nx, ny, nz, xj, yj, zj) - en.c;
+    const float k0 = t0 >= 0.f ? en.ka : en.kb;
+    const float k1 = t1 >= 0.f ? en.ka : en.kb;
+    acc += t1 * t1 * k1 - t0 * t0 * k0;
+  }
+  return acc;
+}
+
+// smooth: bond term ((d - d0)/d0)^2 * k_{stretch|squeeze}, d0 = max(dtn, 1e-6)
+__device__ __forceinline__ float bond_f(float d, float d0, float ks, float kq) {
+  const float diff = (d - d0) / d0;
+  return diff * diff * (diff >= 0.f ? ks : kq);
+}
+
+// angle term at a vertex with bond vectors u (prev -> vertex) and w (vertex ->
+// next) given as u = a - b, w = b - c: ((1 - cos)/2)^3 * ka, skipped when a
+// bond is shorter than 1e-6 (ParallelMonteCarloSmooth.cu:126-147).  1 - cos is
+// evaluated as |u x w|^2 / (|u||w| (|u||w| + u.w)) when u.w > 0, which keeps
+// FP32 precision for nearly straight chains.
+__device__ __forceinline__ float angle_f(float ux, float uy, float uz, float wx, float wy,
+                                         float wz, float ka) {
+  const float lu = sqrtf(fmaf(ux, ux, fmaf(uy, uy, uz * uz)));
+  const float lw = sqrtf(fmaf(wx, wx, fmaf(wy, wy, wz * wz)));
+  if (!(lu > 1e-6f && lw > 1e-6f)) return 0.f;
+  const float dot = fmaf(ux, wx, fmaf(uy, wy, uz * wz));
+  const float ll = lu * lw;
+  float omc;
+  if (dot > 0.f) {
+    const float cx = uy * wz - uz * wy, cy = uz * wx - ux * wz, cz = ux * wy - uy * wx;
+    const float cr = fmaf(cx, cx, fmaf(cy, cy, cz * cz));
+    omc = cr / (ll * (ll + dot));
+  } else {
+    omc = 1.f - dot / ll;
+  }
+  omc = fminf(fmaxf(omc, 0.f), 2.f);
+  const float ang = 0.5f * omc;
+  return ang * ang * ang * ka;
+}
+
+// local window of bead p with centre position (cx,cy,cz); q* hold beads
+// p-2..p+2 (index 2 = p, ignored).  Unweighted (ParallelMonteCarloSmooth.cu:86-154).
+__device__ __forceinline__ float smooth_local(const DevProblem &P, int p, const float *qx,
+                                              const float *qy, const float *qz, float cx, float cy,
+                                              float cz, float d0m, float d0p) {
+  const int n = P.n;
+  float s = 0.f;
+  if (p > 0) s += bond_f(dist3(qx[1], qy[1], qz[1], cx, cy, cz), d0m, P.ks, P.kq);
+  if (p < n - 1) s += bond_f(dist3(cx, cy, cz, qx[3], qy[3], qz[3]), d0p, P.ks, P.kq);
+  if (p >= 2)  // vertex p-1
+    s += angle_f(qx[0] - qx[1], qy[0] - qy[1], qz[0] - qz[1], qx[1] - cx, qy[1] - cy, qz[1] - cz,
+                 P.ka);
+  if (p >= 1 && p + 1 <= n - 1)  // vertex p
+    s += angle_f(qx[1] - cx, qy[1] - cy, qz[1] - cz, cx - qx[3], cy - qy[3], cz - qz[3], P.ka);
+  if (p + 2 <= n - 1)  // vertex p+1
+    s += angle_f(cx - qx[3], cy - qy[3], cz - qz[3], qx[3] - qx[4], qy[3] - qy[4], qz[3] - qz[4],
+                 P.ka);
+  return s;
+}
+
+// ------------------------------------------------------------------ //
+// milestone re-summation (FP32 terms, FP64 sum), per lane
+// ------------------------------------------------------------------ //
+__device__ double resum_heatmap(const DevProblem &P, const PosView &X, float r) {
+  const int n = P.n;
+  double tot = 0.0;
+  for (int m = 0; m < n; ++m) {
+    const float xm = X.get(m, 0), ym = X.get(m, 1), zm = X.get(m, 2);
+    const float *__restrict__ wr = P.W + static_cast<size_t>(m) * n;
+    float part = 0.f;
+    const int j0 = P.symmetric ? m + 1 : 0;
+    for (int j = j0; j < n; ++j) {
+      const float w = wr[j];
+      if (w == 0.f) continue;
+      const float t = fmaf(dist3(xm, ym, zm, X.get(j, 0), X.get(j, 1), X.get(j, 2)), w, -1.f);
+      part = fmaf(t, t, part);
+    }
+    tot += static_cast<double>(part);
+  }
+  if (P.symmetric) tot *= 2.0;
+  if (r > 0.f)
+    for (int m = 0; m < n; ++m) tot += sphere_penalty(X.get(m, 0), X.get(m, 1), X.get(m, 2), r);
+  return tot;
+}
+
+__device__ double resum_arcs(const DevProblem &P, const PosView &X, float r, double offset) {
+  const int n = P.n;
+  double tot = offset;
+  for (int p = 0; p < n; ++p) {
+    const int e0 = P.off[p], e1 = P.off[p + 1];
+    if (e0 == e1) continue;
+    const float xp = X.get(p, 0), yp = X.get(p, 1), zp = X.get(p, 2);
+    float part = 0.f;
+    for (int e = e0; e < e1; ++e) {
+      const ArcEntry en = P.ent[e];
+      const int j = en.j & 0x7fffffff;
+      if (j <= p) continue;
+      const float t = dist3(xp, yp, zp, X.get(j, 0), X.get(j, 1), X.get(j, 2)) - en.c;
+      part = fmaf(t * t, t >= 0.f ? en.ka : en.kb, part);
+    }
+    tot += static_cast<double>(part);
+  }
+  if (r > 0.f)
+    for (int m = 0; m < n; ++m) tot += sphere_penalty(X.get(m, 0), X.get(m, 1), X.get(m, 2), r);
+  return tot;
+}
+
+__device__ double resum_smooth(const DevProblem &P, const PosView &X, float r, double offset) {
+  const int n = P.n;
+  double tot = offset;
+  float part = 0.f;
+  int cnt = 0;
+  // a = bead i-1, b = bead i, c = bead i+1
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  float bx = X.get(0, 0), by = X.get(0, 1), bz = X.get(0, 2);
+  for (int i = 0; i + 1 < n; ++i) {
+    const float cx = X.get(i + 1, 0), cy = X.get(i + 1, 1), cz = X.get(i + 1, 2);
+    part += bond_f(dist3(bx, by, bz, cx, cy, cz), fmaxf(P.dtn[i], 1e-6f), P.ks, P.kq);
+    if (i >= 1) part += angle_f(ax - bx, ay - by, az - bz, bx - cx, by - cy, bz - cz, P.ka);
+    ax = bx;
+    ay = by;
+    az = bz;
+    bx = cx;
+    by = cy;
+    bz = cz;
+    if (++cnt == 256) {  // flush FP32 partials into the FP64 total
+      tot += static_cast<double>(part);
+      part = 0.f;
+      cnt = 0;
+    }
+  }
+  tot += static_cast<double>(part);
+  if (r > 0.f)
+    for (int m = 0; m < n; ++m) tot += sphere_penalty(X.get(m, 0), X.get(m, 1), X.get(m, 2), r);
+  return tot;
+}
+
+}  // namespace pmmc

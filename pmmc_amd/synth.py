@@ -1,0 +1,153 @@
+"""Seeded synthetic inputs for the annealing workloads.
+
+heatmap_problem:  chromosome of N beads; target distances from a random-walk
+                  polymer, freq = 1/(d^3 + 0.1), distance heatmap by the rule
+                  of LooperSolver::createDistanceHeatmap (LooperSolver.hpp:3042-3085):
+                  h = freq_dist_scale * f^freq_dist_power, 0 where f < 1e-6,
+                  -1 inside the diagonal band, clamped at avg * stretching
+                  (avg = Heatmap::getAvg, Heatmap.hpp:440-451, FP32 sequential sum).
+arcs_problem:     one segment of n anchors with ~3 arcs per anchor, separation
+                  log-uniform in [1, 200] anchors, PET counts round(LogNormal(2, 1))
+                  clamped >= 4, duplicate pairs merged, expected distance by the
+                  power-law branch of LooperSolver::freqToDistance
+                  (LooperSolver.hpp:3956-3962): base + scale * (count + shift)^a.
+smooth_problem:   anchors fixed, `add` interpolated sub-anchor beads between
+                  consecutive anchors (LooperSolver::densifyActiveRegion,
+                  LooperSolver.hpp:3836-3898), dist_to_next = 1.
+All randomness comes from numpy's PCG64 seeded per call.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .api import Arcs, Heatmap, Smooth
+
+
+def random_walk(n: int, rng: np.random.Generator, step: float = 1.0) -> np.ndarray:
+    d = rng.normal(size=(n - 1, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([np.zeros((1, 3)), np.cumsum(d * step, axis=0)]).astype(np.float32)
+
+
+def distance_heatmap(freq: np.ndarray, diagonal: int = 1, scale: float = 25.0,
+                     power: float = -0.6, stretching: float = 2.5) -> np.ndarray:
+    n = freq.shape[0]
+    f = freq.astype(np.float32)
+    h = np.zeros((n, n), np.float32)
+    iu = np.triu_indices(n)
+    v = f[iu]
+    band = np.abs(iu[0] - iu[1]) < diagonal
+    val = np.where(v < 1e-6, np.float32(0.0),
+                   np.where(band, np.float32(-1.0),
+                            (np.float64(scale) * np.power(v.astype(np.float64), power))
+                            .astype(np.float32)))
+    h[iu] = val
+    h.T[iu] = val
+    # getAvg: from the first diagonal holding a value > 1e-6
+    w0 = 0
+    for w in range(n):
+        if np.any(np.diagonal(h, w) > 1e-6):
+            w0 = w
+            break
+    # FP32 sequential accumulation in row-major upper-triangle order
+    seq = h[np.triu_indices(n, w0)].astype(np.float32)
+    avg = np.float32(np.cumsum(seq, dtype=np.float32)[-1] / np.float32(seq.size)) if seq.size else 0
+    mx = np.float32(avg * np.float32(stretching))
+    h = np.where(h > mx, mx, h).astype(np.float32)
+    return h
+
+
+def heatmap_problem(n: int = 64, seed: int = 0, diagonal: int = 1, cube: float = 10.0,
+                    skip_below: float = 1e-6) -> tuple[Heatmap, np.ndarray]:
+    """Config 0: returns (problem, ground-truth polymer)."""
+    rng = np.random.default_rng(seed)
+    pos = (rng.random((n, 3)) * cube).astype(np.float32)
+    truth = random_walk(n, np.random.default_rng(seed + 1))
+    d = np.linalg.norm(truth[:, None, :] - truth[None, :, :], axis=2).astype(np.float64)
+    f = 1.0 / (d ** 3 + 0.1)
+    np.fill_diagonal(f, 0.0)
+    h = distance_heatmap(f, diagonal=diagonal)
+    return Heatmap(pos=pos, dist=h, diagonal=diagonal, skip_below=skip_below), truth
+
+
+def arcs_inputs(n: int = 400, n_arcs: int = 1200, seed: int = 0, segment_bp: int = 2_000_000,
+                max_sep: int = 200, count_a: float = -1.0 / 3.0, count_scale: float = 2.0,
+                count_shift: float = 0.0, count_base: float = 0.0):
+    """Anchors (genomic), merged arcs (i, j, count) and the expected-distance
+    matrix (-1 = no arc, 0 on the diagonal)."""
+    rng = np.random.default_rng(seed)
+    starts = np.sort(rng.choice(segment_bp - 2000, size=n, replace=False)).astype(np.int64) + 1
+    anchors = np.stack([starts, starts + 1000], axis=1).astype(np.int32)
+    sep = np.exp(rng.uniform(0.0, np.log(max_sep), size=n_arcs)).astype(np.int64)
+    sep = np.clip(sep, 1, max(1, n - 1))
+    i = rng.integers(0, n, size=n_arcs)
+    j = i + sep
+    over = j >= n
+    i[over] = np.maximum(0, i[over] - sep[over])
+    j[over] = np.minimum(n - 1, i[over] + sep[over])
+    cnt = np.maximum(4, np.round(rng.lognormal(2.0, 1.0, size=n_arcs))).astype(np.int64)
+    merged: dict[tuple[int, int], int] = {}
+    for a, b, c in zip(i.tolist(), j.tolist(), cnt.tolist()):
+        if a == b:
+            continue
+        key = (min(a, b), max(a, b))
+        merged[key] = merged.get(key, 0) + c
+    ed = np.full((n, n), -1.0, np.float32)
+    np.fill_diagonal(ed, 0.0)
+    arcs = []
+    for (a, b), c in sorted(merged.items()):
+        f = max(1.0, c + count_shift)
+        e = np.float32(count_base + count_scale * f ** count_a)
+        ed[a, b] = ed[b, a] = e
+        arcs.append((a, b, c))
+    return anchors, np.array(arcs, np.int64).reshape(-1, 3), ed
+
+
+def arcs_problem(n: int = 400, n_arcs: int = 1200, seed: int = 0, noise: float = 1.0,
+                 init_seed: int | None = None, **kw) -> Arcs:
+    """Config 1: arc level of one segment; initial positions = a random walk
+    plus uniform noise (the level driver's noise init)."""
+    _, _, ed = arcs_inputs(n, n_arcs, seed, **kw)
+    rng = np.random.default_rng(seed + 7 if init_seed is None else init_seed)
+    pos = random_walk(n, rng, step=0.5) + rng.uniform(-noise, noise, size=(n, 3)).astype(np.float32)
+    return Arcs(pos=pos.astype(np.float32), exp_dist=ed)
+
+
+def smooth_problem(n_anchors: int = 400, add: int = 49, seed: int = 0,
+                   anchor_pos: np.ndarray | None = None, noise: float = 0.5) -> Smooth:
+    """Config 2: anchors fixed, `add` beads between consecutive anchors
+    (n = n_anchors + (n_anchors - 1) * add; 400 anchors, add = 49 -> 19,951)."""
+    rng = np.random.default_rng(seed)
+    if anchor_pos is None:
+        anchor_pos = random_walk(n_anchors, rng, step=float(add + 1) ** 0.5)
+    a = np.asarray(anchor_pos, np.float32)
+    beads, fixed = [], []
+    for k in range(len(a) - 1):
+        beads.append(a[k])
+        fixed.append(1)
+        for t in range(1, add + 1):
+            beads.append(a[k] + (a[k + 1] - a[k]) * np.float32(t / (add + 1)))
+            fixed.append(0)
+    beads.append(a[-1])
+    fixed.append(1)
+    pos = np.array(beads, np.float32)
+    pos[np.array(fixed) == 0] += rng.uniform(-noise, noise, size=(int((np.array(fixed) == 0).sum()), 3)).astype(np.float32)
+    n = len(pos)
+    return Smooth(pos=pos, dist_to_next=np.ones(n, np.float32), fixed=np.array(fixed, np.uint8))
+
+
+def batch_arcs_problems(n_segments: int = 32, n_seeds: int = 16, seed: int = 0,
+                        n_min: int = 20, n_max: int = 2000, arcs_per_anchor: int = 3):
+    """Config 3: segments with anchor counts log-uniform in [n_min, n_max],
+    each with ensemble seeds 0..n_seeds-1 (different initial positions)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(n_min), np.log(n_max), size=n_segments)).astype(int)
+    probs, meta = [], []
+    for s, n in enumerate(sizes.tolist()):
+        base = arcs_problem(n=n, n_arcs=arcs_per_anchor * n, seed=1000 + s)
+        for e in range(n_seeds):
+            rng_e = np.random.default_rng(10_000 * (s + 1) + e)
+            pos = random_walk(n, rng_e, step=0.5) + rng_e.uniform(-1, 1, size=(n, 3)).astype(np.float32)
+            probs.append(Arcs(pos=pos.astype(np.float32), exp_dist=base.exp_dist))
+            meta.append((s, e, n))
+    return probs, meta

@@ -1,0 +1,225 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+Per-move ΔE tolerance (stated per level): |ΔE_gpu - ΔE_fp64| <=
+1e-5 * (|local_before| + |local_after|) + 1e-7, i.e. 1e-5 relative to the
+magnitude of the two local sums the delta is formed from (the delta itself
+is a difference and can be arbitrarily small).  Trajectories: the kernels and
+the oracle consume the same Philox stream, so chain positions are compared
+bit-for-bit; a chain may only differ when an FP32 vs FP64 accept decision
+flips on a near-tie, which the test bounds.
+"""
+import numpy as np
+import pytest
+
+from oracle import binding as O
+from pmmc_amd import synth
+from pmmc_amd.api import Arcs, Heatmap, Smooth, schedule_preset, settings
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _proposals(rng, n, k, scale):
+    beads = rng.integers(0, n, size=k).astype(np.int32)
+    disp = rng.uniform(-scale, scale, size=(k, 3)).astype(np.float32)
+    return beads, disp
+
+
+def _check_delta(ctx, prob, k=2048, scale=0.5, r=0.0, seed=0):
+    rng = np.random.default_rng(seed)
+    P = O.Problem(prob)
+    n = P.n
+    beads, disp = _proposals(rng, n, k, scale)
+    got = ctx.delta_energy(prob, beads, disp, sphere_radius=r)
+    pos = P.pos0.reshape(-1, 3)
+    worst = 0.0
+    for i in range(k):
+        p = int(beads[i])
+        q = pos[p] + disp[i]
+        want = P.delta(pos, p, q, r)
+        a, b = P.local_pair(pos, p, q, r)
+        err = abs(got[i] - want) / (abs(a) + abs(b) + 1e-7 / TOL)
+        worst = max(worst, err)
+    assert worst <= TOL, f"worst relative ΔE error {worst:.3e}"
+    return worst
+
+
+def test_delta_heatmap(ctx):
+    prob, _ = synth.heatmap_problem(n=64, seed=1)
+    _check_delta(ctx, prob)
+    prob, _ = synth.heatmap_problem(n=150, seed=2, diagonal=2)
+    prob.boundaries = np.array([0, 60, 150], np.int32)
+    _check_delta(ctx, prob, r=4.0)
+
+
+def test_delta_heatmap_asymmetric(ctx):
+    rng = np.random.default_rng(3)
+    n = 40
+    h = rng.uniform(0.5, 5, (n, n)).astype(np.float32)
+    np.fill_diagonal(h, 0)
+    prob = Heatmap(pos=rng.normal(size=(n, 3)).astype(np.float32) * 3, dist=h)
+    _check_delta(ctx, prob)
+
+
+def test_delta_arcs(ctx):
+    prob = synth.arcs_problem(n=400, n_arcs=1200, seed=1)
+    prob.loop_pairs = np.array([[0, 10], [5, 300], [7, 7]], np.int32)
+    prob.loop_params = np.array([[1.0, 1.5], [0.5, 3.0], [1.0, 1.0]], np.float32)
+    prob.k_stretch, prob.k_squeeze = 1.3, 0.8
+    _check_delta(ctx, prob, r=6.0)
+
+
+def test_delta_smooth(ctx):
+    prob = synth.smooth_problem(n_anchors=40, add=9, seed=1)
+    prob.w_dist, prob.w_angle = 1.5, 0.5
+    _check_delta(ctx, prob, scale=0.3, r=20.0)
+
+
+def test_delta_smooth_nearly_straight(ctx):
+    n = 64
+    pos = np.stack([np.arange(n, dtype=np.float32), np.zeros(n, np.float32),
+                    np.zeros(n, np.float32)], 1)
+    pos += np.random.default_rng(4).normal(scale=1e-3, size=pos.shape).astype(np.float32)
+    prob = Smooth(pos=pos, dist_to_next=np.ones(n, np.float32), k_angular=5.0)
+    _check_delta(ctx, prob, scale=0.01)
+
+
+# ---------------- trajectories ----------------
+def _run_pair(ctx, prob, sched, seed=7, segment=3, n_chains=64):
+    """GPU plain anneal (64 chains = one ensemble) vs the oracle's 64 chains."""
+    pos, st = ctx.anneal(prob, sched, seed=seed, segment=segment, chain0=0, n_chains=n_chains)
+    P = O.Problem(prob)
+    osched = O.schedule_from(sched)
+    res = [P.anneal(osched, seed, segment, c) for c in range(n_chains)]
+    energies = np.array([r[1]["energy"] for r in res])
+    best = int(np.argmin(energies))
+    return pos, st, res, best
+
+
+def _assert_traj(pos, st, res, best, total_moves_rel=0.0):
+    ox, ost = res[st["best_chain"]]
+    assert st["best_chain"] == best, (st["best_chain"], best)
+    assert np.array_equal(pos, ox), "best chain positions differ from the oracle's chain"
+    assert st["best_chain_moves"] == ost["moves"]
+    assert st["best_chain_accepted"] == ost["accepted"]
+    assert st["energy"] == pytest.approx(ost["energy"], rel=1e-6)
+    assert st["moves"] == sum(r[1]["moves"] for r in res)
+
+
+@pytest.mark.parametrize("preset", [0, 1])
+def test_trajectory_heatmap(ctx, preset):
+    prob, _ = synth.heatmap_problem(n=32, seed=5)
+    s = settings(max_temp=5.0, dt_temp=0.999, improvement=0.99, steps=512, min_successes=10,
+                 milestone_fails=3)
+    sched = schedule_preset(1, preset, s, 1.0)
+    sched.max_moves = 4096
+    _assert_traj(*_run_pair(ctx, prob, sched))
+
+
+@pytest.mark.parametrize("preset", [0, 1])
+def test_trajectory_arcs(ctx, preset):
+    prob = synth.arcs_problem(n=120, n_arcs=360, seed=5)
+    s = settings(max_temp=5.0, dt_temp=0.9999, improvement=0.975, steps=2048, min_successes=100,
+                 milestone_fails=3)
+    sched = schedule_preset(2, preset, s, 0.3)
+    sched.max_moves = 4096
+    _assert_traj(*_run_pair(ctx, prob, sched))
+
+
+@pytest.mark.parametrize("preset", [0, 1])
+def test_trajectory_smooth(ctx, preset):
+    prob = synth.smooth_problem(n_anchors=20, add=9, seed=5)
+    s = settings(max_temp=5.0, dt_temp=0.9995, improvement=0.99, steps=2048, min_successes=50,
+                 milestone_fails=3)
+    sched = schedule_preset(3, preset, s, 0.5)
+    sched.max_moves = 4096
+    pos, st, res, best = _run_pair(ctx, prob, sched)
+    ox, ost = res[st["best_chain"]]
+    assert st["best_chain"] == best
+    free = prob.fixed == 0
+    assert np.array_equal(pos[free], ox[free])
+    assert np.array_equal(pos[~free], prob.pos[~free])
+    assert st["energy"] == pytest.approx(ost["energy"], rel=1e-6)
+
+
+def test_resummed_energy_matches_oracle(ctx):
+    prob, _ = synth.heatmap_problem(n=48, seed=9)
+    s = settings(steps=512)
+    sched = schedule_preset(1, 1, s, 1.0)
+    sched.max_moves = 2048
+    pos, st = ctx.anneal(prob, sched, seed=1, n_chains=64)
+    P = O.Problem(prob)
+    # heatmap return value: the full score of the returned positions
+    assert st["energy_resummed"] == pytest.approx(P.total(pos), rel=1e-12)
+
+
+# ---------------- arc marking ----------------
+def _chrom(rng, A, R, sort=True):
+    starts = rng.integers(1, 300_000, size=A)
+    if sort:
+        starts = np.sort(starts)
+    lens = rng.integers(0, 5000, size=A)
+    anc = np.stack([starts, starts + lens], 1)
+    anc[rng.integers(0, A, size=3)] = 0
+    pick = rng.integers(0, A, size=(R, 2))
+    gs = np.abs(anc[pick[:, 0], 0] + rng.integers(-20, 80, R))
+    ge = np.abs(anc[pick[:, 1], 0] + rng.integers(-20, 80, R))
+    raw = np.stack([gs, ge, rng.integers(1, 30, R), rng.integers(0, 3, R)], 1)
+    return anc.astype(np.int32), raw.astype(np.int32)
+
+
+@pytest.mark.parametrize("sort", [True, False])
+def test_mark_arcs_bit_exact(ctx, sort):
+    rng = np.random.default_rng(31 + sort)
+    anc, raw = _chrom(rng, 3000, 20000, sort)
+    s, e = ctx.mark_arcs(anc[:, 0], anc[:, 1], raw[:, 0], raw[:, 1])
+    os_, oe = O.mark_arcs(anc[:, 0], anc[:, 1], raw[:, 0], raw[:, 1])
+    assert np.array_equal(s, os_) and np.array_equal(e, oe)
+
+
+def test_mark_arcs_edges(ctx):
+    s, e = ctx.mark_arcs([], [], [1, 2], [3, 4])
+    assert s.tolist() == [-1, -1] and e.tolist() == [-1, -1]
+    s, e = ctx.mark_arcs([5], [9], [], [])
+    assert s.size == 0
+
+
+def test_group_arcs_bit_exact(ctx):
+    rng = np.random.default_rng(41)
+    chrs = [_chrom(rng, 200, 1500) for _ in range(4)]
+    got, nm = ctx.group_arcs(chrs)
+    want, wnm = O.group_arcs(chrs, use_unordered=True)
+    assert np.array_equal(nm, wnm)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+
+
+def test_group_arcs_golden_fixture(ctx):
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "markarcs_ref.json")))
+    chrs = [(np.array(c["anchors"], np.int32), np.array(c["raw"], np.int32)) for c in fx["chromosomes"]]
+    got, _ = ctx.group_arcs(chrs)
+    for a, c in zip(got, fx["chromosomes"]):
+        ref = np.array(c["arcs"], np.int32).reshape(-1, 7)
+        assert sorted(map(tuple, a.tolist())) == sorted(map(tuple, ref.tolist()))
+
+
+# ---------------- batched == plain (alone vs in the mix) ----------------
+def test_batch_equals_plain(ctx):
+    s = settings(max_temp=5.0, dt_temp=0.9999, improvement=0.975, steps=2048, min_successes=100)
+    sched = schedule_preset(2, 1, s, 0.3)
+    sched.max_moves = 3072
+    probs = [synth.arcs_problem(n=n, n_arcs=3 * n, seed=50 + n) for n in (30, 90, 200)]
+    hs, _ = synth.heatmap_problem(n=40, seed=3)
+    hsched = schedule_preset(1, 1, settings(steps=512), 1.0)
+    hsched.max_moves = 2048
+    alone = [ctx.anneal(p, sched, seed=11, segment=i, n_chains=128) for i, p in enumerate(probs)]
+    alone.append(ctx.anneal(hs, hsched, seed=11, segment=3, n_chains=64))
+    mixed = ctx.anneal_batch(probs + [hs], [sched] * 3 + [hsched], seeds=[11] * 4,
+                             n_chains=[128, 128, 128, 64], blocks_per_step=3, use_graph=True)
+    for (pa, sa), (pb, sb) in zip(alone, mixed):
+        assert np.array_equal(pa, pb)
+        assert sa["energy"] == sb["energy"] and sa["moves"] == sb["moves"]
+        assert sa["best_chain"] == sb["best_chain"]
