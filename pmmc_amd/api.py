@@ -233,6 +233,10 @@ class Context:
         res = [flat[out_off[c]:out_off[c + 1]].copy() for c in range(len(chromosomes))]
         return res, nonm[:len(chromosomes)].copy()
 
+    # ---- served ----
+    def server(self, hbm_budget=8 << 30, max_jobs=256, blocks_per_step=4):
+        return Server(self, hbm_budget, max_jobs, blocks_per_step)
+
     # ---- batched ----
     def anneal_batch(self, problems, schedules, seeds, segments=None, n_chains=0,
                      blocks_per_step=0, steps_per_graph=0, use_graph=True):
@@ -256,3 +260,50 @@ class Context:
         opts = A.BatchOpts(blocks_per_step, steps_per_graph, 1 if use_graph else 0)
         self._check(self.lib.pmmc_anneal_batch(self.h, jobs, k, C.byref(opts)))
         return [(outs[i], stats[i].as_dict()) for i in range(k)]
+
+
+class Server:
+    """Continuous-batching server (pmmc_server_*): submit jobs, step, collect."""
+
+    def __init__(self, ctx: Context, hbm_budget: int, max_jobs: int, blocks_per_step: int):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = C.c_void_p()
+        opts = A.ServerOpts(hbm_budget, max_jobs, blocks_per_step)
+        ctx._check(self.lib.pmmc_server_create(ctx.h, C.byref(opts), C.byref(h)))
+        self.h = h
+        self.jobs = {}
+
+    def submit(self, prob, sched: A.Schedule, seed=1, segment=0, n_chains=0):
+        cp = prob.c()
+        out = _f32(prob.pos).reshape(-1, 3).copy()
+        st = A.Stats()
+        job = A.Job(prob.level, C.cast(C.pointer(cp), C.c_void_p), sched,
+                    A.Run(seed, segment, 0, n_chains), out.ctypes.data_as(A.f32p), C.pointer(st))
+        jid = C.c_uint64()
+        self.ctx._check(self.lib.pmmc_server_submit(self.h, C.byref(job), C.byref(jid)))
+        self.jobs[jid.value] = (prob, cp, out, st)
+        return jid.value
+
+    def step(self):
+        d, r, q = C.c_int32(), C.c_int32(), C.c_int32()
+        self.ctx._check(self.lib.pmmc_server_step(self.h, C.byref(d), C.byref(r), C.byref(q)))
+        return d.value, r.value, q.value
+
+    def done(self, jid) -> bool:
+        return bool(self.lib.pmmc_server_done(self.h, jid))
+
+    def result(self, jid):
+        _, _, out, st = self.jobs[jid]
+        return out, st.as_dict()
+
+    def close(self):
+        if self.h:
+            self.lib.pmmc_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -152,10 +152,15 @@ pmmc_status check_schedule(pmmc_ctx *ctx, const pmmc_schedule &s) {
   return PMMC_OK;
 }
 
-int default_chains(const pmmc_ctx *ctx, int /*level*/) {
-  // one 64-chain ensemble per compute unit
+int default_chains(const pmmc_ctx *ctx, int /*level*/, int n) {
+  // eight 64-chain ensembles (wavefronts) per compute unit, capped so the
+  // chains' positions take at most 8 GiB of HBM
   const int cus = ctx && ctx->cu_count > 0 ? ctx->cu_count : 256;
-  return cus * kLanes;
+  const size_t per_ens = static_cast<size_t>(n > 0 ? n : 1) * 3 * kLanes * sizeof(float);
+  size_t ens = static_cast<size_t>(8) * cus;
+  const size_t cap = (size_t(8) << 30) / per_ens;
+  if (ens > cap) ens = cap > 0 ? cap : 1;
+  return static_cast<int>(ens) * kLanes;
 }
 
 // heatmap_chromosome_boundaries lookup, LooperSolver.hpp:6034-6050
@@ -293,7 +298,7 @@ pmmc_status prepare_problem(pmmc_ctx *ctx, int level, const void *problem, const
   h.segment = run.segment;
   if (run.chain0 % kLanes != 0) return fail(ctx, PMMC_ERR_INVALID, "run.chain0 must be a multiple of 64");
   h.chain0 = run.chain0;
-  h.n_chains = run.n_chains > 0 ? run.n_chains : default_chains(ctx, level);
+  h.n_chains = run.n_chains > 0 ? run.n_chains : default_chains(ctx, level, h.n);
   if (static_cast<uint64_t>(h.chain0) + h.n_chains > 0x7FFFFFFFull)
     return fail(ctx, PMMC_ERR_INVALID, "chain ids exceed 2^31");
   return PMMC_OK;

@@ -61,8 +61,18 @@ __host__ __device__ __forceinline__ uint32_t pick(uint32_t x, uint32_t m) {
 }
 
 // displacement component (2u - 1) * step; 2u - 1 is exact, one rounding.
-__host__ __device__ __forceinline__ float disp(uint32_t x, float step) {
-  return (2.0f * u01(x) - 1.0f) * step;
+// The product and the position update are kept as separate IEEE operations
+// (no fused multiply-add), exactly as the reference's vector3 += does.
+// (__fmul_rn/__fadd_rn alone do not stop the backend from fusing them.)
+__device__ __forceinline__ float disp(uint32_t x, float step) {
+#pragma clang fp contract(off)
+  const float t = 2.0f * u01(x) - 1.0f;
+  return t * step;
+}
+
+__device__ __forceinline__ float move_coord(float x, float d) {
+#pragma clang fp contract(off)
+  return x + d;
 }
 
 __device__ __forceinline__ float sphere_penalty(float x, float y, float z, float r) {

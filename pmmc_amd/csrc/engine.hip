@@ -255,11 +255,11 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
       const uint32_t sel = m & 3u;
       const uint32_t word = sel == 0 ? rb.x : (sel == 1 ? rb.y : (sel == 2 ? rb.z : rb.w));
       const int p = P.free_list[pick(word, static_cast<uint32_t>(nfree))];
+      const float px = X.get(p, 0), py = X.get(p, 1), pz = X.get(p, 2);
       const U4 rc = draw(seed, m, chain, segment, LEVEL, kSubChain);
       const float dx = disp(rc.x, step), dy = disp(rc.y, step);
       const float dz = S.use2d ? 0.f : disp(rc.z, step);
-      const float px = X.get(p, 0), py = X.get(p, 1), pz = X.get(p, 2);
-      const float nx = px + dx, ny = py + dy, nz = pz + dz;
+      const float nx = move_coord(px, dx), ny = move_coord(py, dy), nz = move_coord(pz, dz);
       float dE;
       if constexpr (LEVEL == PMMC_LEVEL_HEATMAP) {
         dE = heatmap_delta(P, X, p, px, py, pz, nx, ny, nz);
@@ -848,7 +848,8 @@ __global__ __launch_bounds__(64) void k_delta(const DevProblem *__restrict__ pro
   const int n = P.n;
   const int p = beads[i];
   const float px = X.get(p, 0), py = X.get(p, 1), pz = X.get(p, 2);
-  const float nx = px + dsp[3 * i], ny = py + dsp[3 * i + 1], nz = pz + dsp[3 * i + 2];
+  const float nx = move_coord(px, dsp[3 * i]), ny = move_coord(py, dsp[3 * i + 1]);
+  const float nz = move_coord(pz, dsp[3 * i + 2]);
   float dE;
   if constexpr (LEVEL == PMMC_LEVEL_HEATMAP) {
     dE = heatmap_delta(P, X, p, px, py, pz, nx, ny, nz);

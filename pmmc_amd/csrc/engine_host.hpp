@@ -37,7 +37,7 @@ pmmc_status prepare_problem(pmmc_ctx *ctx, int level, const void *problem, const
                             const pmmc_run &run, HostProblem &out);
 pmmc_status check_schedule(pmmc_ctx *ctx, const pmmc_schedule &s);
 int autotune_blocks_per_step(const std::vector<HostProblem> &hp);
-int default_chains(const pmmc_ctx *ctx, int level);
+int default_chains(const pmmc_ctx *ctx, int level, int n);
 
 // contiguous range of ensembles launched together: one (level, LDS staging, n)
 struct LaunchGroup {

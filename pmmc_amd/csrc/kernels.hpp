@@ -92,15 +92,30 @@ __device__ __forceinline__ float arcs_delta(const DevProblem &P, const PosView &
                                             float py, float pz, float nx, float ny, float nz) {
   const int e0 = P.off[p], e1 = P.off[p + 1];
   float acc = 0.f;
-  for (int e = e0; e < e1; ++e) {
-    const ArcEntry en = P.ent[e];
-    const int j = en.j & 0x7fffffff;
-    const float xj = X.get(j, 0), yj = X.get(j, 1), zj = X.get(j, 2);
-    const float t0 = dist3(px, py, pz, xj, yj, zj) - en.c;
-    const float t1 = dist3(nx, ny, nz, xj, yj, zj) - en.c;
-    const float k0 = t0 >= 0.f ? en.ka : en.kb;
-    const float k1 = t1 >= 0.f ? en.ka : en.kb;
-    acc += t1 * t1 * k1 - t0 * t0 * k0;
+  // Four entries per round trip: the (wave-uniform) entry loads and then the
+  // twelve per-lane position loads are all issued before any is consumed.
+  for (int e = e0; e < e1; e += 4) {
+    ArcEntry en[4];
+    float xj[4], yj[4], zj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) en[u] = P.ent[min(e + u, e1 - 1)];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = en[u].j & 0x7fffffff;
+      xj[u] = X.get(j, 0);
+      yj[u] = X.get(j, 1);
+      zj[u] = X.get(j, 2);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (e + u < e1) {
+        const float t0 = dist3(px, py, pz, xj[u], yj[u], zj[u]) - en[u].c;
+        const float t1 = dist3(nx, ny, nz, xj[u], yj[u], zj[u]) - en[u].c;
+        const float k0 = t0 >= 0.f ? en[u].ka : en[u].kb;
+        const float k1 = t1 >= 0.f ? en[u].ka : en[u].kb;
+        acc += t1 * t1 * k1 - t0 * t0 * k0;
+      }
+    }
   }
   return acc;
 }

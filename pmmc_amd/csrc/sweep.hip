@@ -60,8 +60,8 @@ __global__ __launch_bounds__(64) void k_sweep_propose(SweepArgs a) {
   for (int k = 0; k < a.inner; ++k) {
     const U4 rc = draw(a.seed, a.iter * static_cast<uint32_t>(a.inner) + k, chain, a.segment,
                        kLevelSweep, kSubChain);
-    const float nx = x + disp(rc.x, a.step), ny = y + disp(rc.y, a.step);
-    const float nz = a.use2d ? z : z + disp(rc.z, a.step);
+    const float nx = move_coord(x, disp(rc.x, a.step)), ny = move_coord(y, disp(rc.y, a.step));
+    const float nz = a.use2d ? z : move_coord(z, disp(rc.z, a.step));
     float s0 = 0.f, s1 = 0.f;
     int j = 0;
     for (; j + 1 < a.n; j += 2) {

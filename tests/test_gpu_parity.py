@@ -223,3 +223,68 @@ def test_batch_equals_plain(ctx):
         assert np.array_equal(pa, pb)
         assert sa["energy"] == sb["energy"] and sa["moves"] == sb["moves"]
         assert sa["best_chain"] == sb["best_chain"]
+
+
+# ---------------- heatmap sweep (reference GPU heatmap algorithm) ----------------
+def test_sweep_matches_oracle(ctx):
+    from pmmc_amd import _abi as A
+    prob, _ = synth.heatmap_problem(n=24, seed=6, skip_below=1e-3)
+    ss = A.SweepSchedule(5.0, 0.9999, 0.99, 1e-4, 50.0, 20.0, 0.75, 0.95, 0.0, 64, 64, 3, 3, 0)
+    pos, st = ctx.heatmap_sweep(prob, ss, seed=9, segment=1)
+    P = O.Problem(prob)
+    osw = O.OrcSweep(5.0, 0.9999, 0.99, 1e-4, 50.0, 20.0, 0.75, 0.95, 0.0, 64, 64, 3, 3, 0)
+    x = P.pos0.copy()
+    ost = O.OrcStats()
+    O.load().orc_heatmap_sweep(O.C.byref(P.c), O.C.byref(osw), 9, 1, x.ctypes.data_as(O.f32p),
+                               O.C.byref(ost))
+    x = x.reshape(-1, 3)
+    same = np.mean(np.all(pos == x, axis=1))
+    assert same >= 0.9, same
+    assert st["energy"] == pytest.approx(ost.energy, rel=1e-2)
+    assert st["milestones"] == ost.milestones
+    assert st["energy"] < P.total()          # the sweep lowers the full score
+
+
+# ---------------- served form ----------------
+def test_server_alone_equals_mixed(ctx):
+    s = settings(max_temp=5.0, dt_temp=0.9999, improvement=0.975, steps=2048, min_successes=100)
+    asched = schedule_preset(2, 1, s, 0.3)
+    asched.max_moves = 2048
+    hsched = schedule_preset(1, 1, settings(steps=512), 1.0)
+    hsched.max_moves = 1024
+    ssched = schedule_preset(3, 1, settings(steps=1024, dt_temp=0.9995), 0.5)
+    jobs = [(synth.arcs_problem(n=n, n_arcs=3 * n, seed=70 + n), asched, 128) for n in (40, 150)]
+    jobs.append((synth.heatmap_problem(n=30, seed=4)[0], hsched, 64))
+    jobs.append((synth.smooth_problem(n_anchors=12, add=9, seed=2), ssched, 64))
+    alone = [ctx.anneal(p, sc, seed=5, segment=i, n_chains=nc) for i, (p, sc, nc) in enumerate(jobs)]
+    srv = ctx.server(hbm_budget=1 << 30, max_jobs=3, blocks_per_step=2)
+    ids = [srv.submit(p, sc, seed=5, segment=i, n_chains=nc) for i, (p, sc, nc) in enumerate(jobs)]
+    max_res, steps = 0, 0
+    while not all(srv.done(j) for j in ids):
+        _, r, _ = srv.step()
+        max_res = max(max_res, r)
+        steps += 1
+        assert steps < 10000
+    assert max_res <= 3
+    for (pa, sa), j in zip(alone, ids):
+        pb, sb = srv.result(j)
+        assert np.array_equal(pa, pb)
+        assert sa["energy"] == sb["energy"] and sa["best_chain"] == sb["best_chain"]
+    srv.close()
+
+
+def test_server_budget_admission(ctx):
+    sched = schedule_preset(2, 1, settings(steps=1024), 0.3)
+    sched.max_moves = 1024
+    probs = [synth.arcs_problem(n=60, n_arcs=180, seed=80 + i) for i in range(4)]
+    srv = ctx.server(hbm_budget=1 << 30, max_jobs=1, blocks_per_step=1)
+    ids = [srv.submit(p, sched, seed=3, segment=i, n_chains=64) for i, p in enumerate(probs)]
+    seen = []
+    while not all(srv.done(j) for j in ids):
+        _, r, q = srv.step()
+        seen.append((r, q))
+    assert max(r for r, _ in seen) <= 1          # one slot: jobs run one after another
+    with pytest.raises(Exception):
+        srv2 = ctx.server(hbm_budget=1024, max_jobs=4)
+        srv2.submit(probs[0], sched, n_chains=64)   # larger than the budget
+    srv.close()
