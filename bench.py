@@ -51,18 +51,15 @@ def cpu_baseline(prob, s, step, budget_s=15.0):
     sched = O.schedule_from(schedule_preset(2, 0, s, step))
     P = O.Problem(prob)
     threads = max(1, min(16, os.cpu_count() or 1))
-    # calibrate on one short chain
-    sched.max_moves = 20000
-    sched.block = 20000
-    sched.milestone_period = 20000
-    _, st, secs = P.anneal_many(sched, 1, 0, 0, 1, 1)
-    per_move = secs / max(1, st[0]["moves"])
+    # calibrate on one thread per core, then size the sample to the budget
     moves_per_chain = 50000
     sched.max_moves = moves_per_chain
     sched.block = 50000
     sched.milestone_period = 50000
-    chains = max(threads, int(budget_s * threads / (per_move * moves_per_chain)))
-    chains = min(chains, 64 * threads)
+    _, st, secs = P.anneal_many(sched, 1, 0, 0, threads, threads)
+    per_chain = secs  # one chain per thread ran concurrently
+    rounds = max(1, int(budget_s / max(per_chain, 1e-3)))
+    chains = threads * rounds
     _, stats, secs = P.anneal_many(sched, 1, 0, 0, chains, threads)
     moves = sum(x["moves"] for x in stats)
     return {"value": moves / secs, "unit": "moves/s", "cores": threads, "kind": "port",

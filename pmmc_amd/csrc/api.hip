@@ -153,12 +153,13 @@ pmmc_status check_schedule(pmmc_ctx *ctx, const pmmc_schedule &s) {
 }
 
 int default_chains(const pmmc_ctx *ctx, int /*level*/, int n) {
-  // eight 64-chain ensembles (wavefronts) per compute unit, capped so the
-  // chains' positions take at most 8 GiB of HBM
+  // sixteen 64-chain ensembles (wavefronts) per compute unit -- enough
+  // resident wavefronts to hide the HBM latency of the partner loads --
+  // capped so the chains' positions take at most 32 GiB of HBM
   const int cus = ctx && ctx->cu_count > 0 ? ctx->cu_count : 256;
   const size_t per_ens = static_cast<size_t>(n > 0 ? n : 1) * 3 * kLanes * sizeof(float);
-  size_t ens = static_cast<size_t>(8) * cus;
-  const size_t cap = (size_t(8) << 30) / per_ens;
+  size_t ens = static_cast<size_t>(16) * cus;
+  const size_t cap = (size_t(32) << 30) / per_ens;
   if (ens > cap) ens = cap > 0 ? cap : 1;
   return static_cast<int>(ens) * kLanes;
 }
@@ -202,19 +203,22 @@ static pmmc_status prep_heatmap(pmmc_ctx *ctx, const pmmc_heatmap_problem *p, Ho
         h.symmetric = false;
         break;
       }
-  // block partition must be symmetric for the double-counting identity
-  h.W.assign(nn, 0.f);
-  h.Hc.assign(nn, 0.f);
-  if (!h.symmetric) h.W2.assign(nn, 0.f);
+  // rows padded with zeros to 32 / 64 entries for small heatmaps, so the
+  // register-resident kernel reads whole rows with wide scalar loads
+  h.ldw = n <= 32 ? 32 : (n <= 64 ? 64 : n);
+  const size_t nl = static_cast<size_t>(n) * h.ldw;
+  h.W.assign(nl, 0.f);
+  h.Hc.assign(nl, 0.f);
+  if (!h.symmetric) h.W2.assign(nl, 0.f);
   for (int q = 0; q < n; ++q)
     for (int j = 0; j < n; ++j) {
       if (valid(j, q)) {
         const float hv = p->dist[static_cast<size_t>(j) * n + q];
-        h.W[static_cast<size_t>(q) * n + j] = 1.0f / hv;
-        h.Hc[static_cast<size_t>(q) * n + j] = hv;
+        h.W[static_cast<size_t>(q) * h.ldw + j] = 1.0f / hv;
+        h.Hc[static_cast<size_t>(q) * h.ldw + j] = hv;
       }
       if (!h.symmetric && valid(q, j))
-        h.W2[static_cast<size_t>(q) * n + j] = 1.0f / p->dist[static_cast<size_t>(q) * n + j];
+        h.W2[static_cast<size_t>(q) * h.ldw + j] = 1.0f / p->dist[static_cast<size_t>(q) * n + j];
     }
   h.pos0.assign(p->pos, p->pos + 3 * static_cast<size_t>(n));
   h.fixed.assign(n, 0);

@@ -45,7 +45,7 @@ __device__ double exact_heatmap(const DevProblem &P, const float *x, double *red
   for (int m = threadIdx.x; m < n; m += blockDim.x) {
     const int j0 = P.symmetric ? m + 1 : 0;
     for (int j = j0; j < n; ++j) {
-      const float h = P.Hc[static_cast<size_t>(m) * n + j];
+      const float h = P.Hc[static_cast<size_t>(m) * P.ldw + j];
       if (h == 0.f) continue;
       const double d = dist3d(x[3 * m], x[3 * m + 1], x[3 * m + 2], x[3 * j], x[3 * j + 1],
                               x[3 * j + 2]);
@@ -295,45 +295,28 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
       }
       if (active && !S.cool_per_block) T *= S.dt;
     }
+    bool due = false;
     if (active) {
       iters += static_cast<uint32_t>(S.block);
       if (S.cool_per_block) T *= S.dt;
       step *= S.step_decay;
-      if ((iters % static_cast<uint32_t>(S.milestone_period)) < static_cast<uint32_t>(S.block)) {
+      due = (iters % static_cast<uint32_t>(S.milestone_period)) < static_cast<uint32_t>(S.block);
+    }
+    // The active chains of an ensemble share `iters`, so they reach a
+    // milestone together.  The re-summation runs with the whole wavefront
+    // (it broadcasts table entries across lanes), stopped chains included;
+    // only the chains at the milestone use its result.
+    if (__ballot(due) != 0) {
+      double Er = E;
+      if (S.resum) {
+        if constexpr (LEVEL == PMMC_LEVEL_HEATMAP) Er = resum_heatmap(P, X, r);
+        else if constexpr (LEVEL == PMMC_LEVEL_ARCS) Er = resum_arcs(P, X, r, offset);
+        else Er = resum_smooth(P, X, r, offset);
+      }
+      if (due) {
         ++miles;
-        if (S.resum) {
-          if constexpr (LEVEL == PMMC_LEVEL_HEATMAP) E = resum_heatmap(P, X, r);
-          else if constexpr (LEVEL == PMMC_LEVEL_ARCS) E = resum_arcs(P, X, r, offset);
-          else E = resum_smooth(P, X, r, offset);
-        }
-        if (!isfinite(E)) {
-          status = PMMC_STOP_NONFINITE;
-          active = false;
-        } else {
-          const bool no_imp = (E > S.improvement * ms && msucc < S.min_successes) ||
-                              E < S.small_energy ||
-                              (S.ratio_stop > 0.0 && ms > 1e-10 && E / ms > S.ratio_stop);
-          if (S.stop_rule == PMMC_STOP_FIRST_FAIL) {
-            if (no_imp) {
-              status = E < S.small_energy ? PMMC_STOP_SMALL : PMMC_STOP_RULE;
-              active = false;
-            }
-          } else {
-            if (no_imp) ++misses;
-            else if (S.stop_rule == PMMC_STOP_MISSES_RESET) misses = 0;
-            if (E < S.small_energy) {
-              status = PMMC_STOP_SMALL;
-              active = false;
-            } else if (misses >= S.fails_threshold) {
-              status = PMMC_STOP_RULE;
-              active = false;
-            }
-          }
-          if (active) {
-            ms = E;
-            msucc = 0;
-          }
-        }
+        E = Er;
+        milestone(S, E, ms, msucc, misses, status, active);
       }
     }
   }
@@ -476,7 +459,15 @@ __global__ __launch_bounds__(256) void k_finalize(DevProblem *probs, const DevEn
 // ------------------------------------------------------------------ //
 // host side
 // ------------------------------------------------------------------ //
-static constexpr int kStageMaxBytes = 64 * 1024;
+// Ensembles whose positions fit this many bytes of LDS are staged there for
+// the whole launch (one workgroup per ensemble; the LDS use bounds residency).
+// Off by default: on MI355X sixteen resident wavefronts per CU reading
+// positions through L2 beat three LDS-staged ones (N=64 heatmap: 5.4 vs 3.0
+// G moves/s; N=24: equal).
+#ifndef PMMC_STAGE_MAX_BYTES
+#define PMMC_STAGE_MAX_BYTES 0
+#endif
+static constexpr int kStageMaxBytes = PMMC_STAGE_MAX_BYTES;
 
 bool stage_ok(int n) { return static_cast<size_t>(n) * 3 * kLanes * sizeof(float) <= kStageMaxBytes; }
 
@@ -567,6 +558,7 @@ void fill_problem(Carver &cv, hipStream_t s, const HostProblem &h, int index, De
   P.n = h.n;
   P.nfree = static_cast<int32_t>(h.free_list.size());
   P.symmetric = h.symmetric ? 1 : 0;
+  P.ldw = h.ldw > 0 ? h.ldw : h.n;
   P.free_list = put(cv, s, h.free_list, err, 1);
   P.pos0 = put(cv, s, h.pos0, err);
   P.W = put(cv, s, h.W, err);
@@ -839,14 +831,16 @@ __global__ __launch_bounds__(64) void k_delta(const DevProblem *__restrict__ pro
                                               const DevEns *__restrict__ ens, int n_prop,
                                               const int32_t *__restrict__ beads,
                                               const float *__restrict__ dsp, float *__restrict__ out) {
+  // one proposal per wavefront: the bead is wave-uniform, exactly as on the
+  // annealing path (all 64 lanes hold the same replicated state)
   const DevProblem &P = probs[0];
   const int lane = threadIdx.x;
-  const int i = blockIdx.x * kLanes + lane;
+  const int i = blockIdx.x;
   if (i >= n_prop) return;
   PosView X;
   X.p = ens[0].pos + lane;
   const int n = P.n;
-  const int p = beads[i];
+  const int p = __builtin_amdgcn_readfirstlane(beads[i]);
   const float px = X.get(p, 0), py = X.get(p, 1), pz = X.get(p, 2);
   const float nx = move_coord(px, dsp[3 * i]), ny = move_coord(py, dsp[3 * i + 1]);
   const float nz = move_coord(pz, dsp[3 * i + 2]);
@@ -871,7 +865,7 @@ __global__ __launch_bounds__(64) void k_delta(const DevProblem *__restrict__ pro
   }
   const float r = P.sched.sphere_radius;
   if (r > 0.f) dE += sphere_penalty(nx, ny, nz, r) - sphere_penalty(px, py, pz, r);
-  out[i] = dE;
+  if (lane == 0) out[i] = dE;
 }
 
 }  // namespace pmmc
@@ -910,7 +904,7 @@ extern "C" pmmc_status pmmc_delta_energy(pmmc_ctx *ctx, int level, const void *p
   d_o = d_d + ((3 * static_cast<size_t>(n_prop) + 63) & ~size_t(63));
   PMMC_HIP_TRY(ctx, hipMemcpyAsync(d_b, beads, sizeof(int32_t) * n_prop, hipMemcpyHostToDevice, st));
   PMMC_HIP_TRY(ctx, hipMemcpyAsync(d_d, disp, sizeof(float) * 3 * n_prop, hipMemcpyHostToDevice, st));
-  const int grid = (n_prop + kLanes - 1) / kLanes;
+  const int grid = n_prop;
   switch (level) {
     case PMMC_LEVEL_HEATMAP:
       hipLaunchKernelGGL(k_delta<PMMC_LEVEL_HEATMAP>, dim3(grid), dim3(kLanes), 0, st, bd.probs, bd.ens, n_prop, d_b, d_d, d_o);

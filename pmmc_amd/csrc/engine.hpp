@@ -44,6 +44,7 @@ struct ArcEntry {
 
 struct DevProblem {
   int32_t level, n, nfree, symmetric;
+  int32_t ldw, pad_ldw;      // row stride of W, W2, Hc (n, or 32/64 for small heatmaps)
   const int32_t *free_list;  // [nfree]
   const float *pos0;         // [n*3] initial positions (AoS)
   // heatmap: W[p*n+j] = valid(j,p) ? 1/h[j][p] : 0 ; W2 likewise with h[p][j]

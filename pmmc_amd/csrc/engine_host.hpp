@@ -19,6 +19,7 @@ struct HostProblem {
   // heatmap tables (see DevProblem)
   std::vector<float> W, W2, Hc;
   bool symmetric = true;
+  int ldw = 0;  // row stride of W, W2, Hc
   // arcs CSR
   std::vector<int32_t> off;
   std::vector<ArcEntry> ent;

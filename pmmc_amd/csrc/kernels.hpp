@@ -41,6 +41,43 @@ __device__ __forceinline__ bool metropolis(const pmmc_schedule &S, double E, dou
   return ok;
 }
 
+// Milestone stop test on the (re-summed) running total E: the
+// no-improvement predicate and the three stop rules of the reference
+// annealers (LooperSolver.hpp:1543-1553, :4765-4783, :5015-5034;
+// ParallelMonteCarloHeatmap.cu:334-352; ParallelMonteCarloArcs.cu:636-655;
+// ParallelMonteCarloSmooth.cu:320-335).  Updates ms/msucc when the chain
+// continues.
+__device__ __forceinline__ void milestone(const pmmc_schedule &S, double E, double &ms, int &msucc,
+                                          int &misses, int &status, bool &active) {
+  if (!isfinite(E)) {
+    status = PMMC_STOP_NONFINITE;
+    active = false;
+    return;
+  }
+  const bool no_imp = (E > S.improvement * ms && msucc < S.min_successes) || E < S.small_energy ||
+                      (S.ratio_stop > 0.0 && ms > 1e-10 && E / ms > S.ratio_stop);
+  if (S.stop_rule == PMMC_STOP_FIRST_FAIL) {
+    if (no_imp) {
+      status = E < S.small_energy ? PMMC_STOP_SMALL : PMMC_STOP_RULE;
+      active = false;
+    }
+  } else {
+    if (no_imp) ++misses;
+    else if (S.stop_rule == PMMC_STOP_MISSES_RESET) misses = 0;
+    if (E < S.small_energy) {
+      status = PMMC_STOP_SMALL;
+      active = false;
+    } else if (misses >= S.fails_threshold) {
+      status = PMMC_STOP_RULE;
+      active = false;
+    }
+  }
+  if (active) {
+    ms = E;
+    msucc = 0;
+  }
+}
+
 // ------------------------------------------------------------------ //
 // level energies, FP32 move deltas
 // ------------------------------------------------------------------ //
@@ -51,37 +88,55 @@ __device__ __forceinline__ float heatmap_delta(const DevProblem &P, const PosVie
                                                float px, float py, float pz, float nx, float ny,
                                                float nz) {
   const int n = P.n;
-  const float *__restrict__ w1 = P.W + static_cast<size_t>(p) * n;
+  const int lane = threadIdx.x & (kLanes - 1);
+  // The weight row of the (wave-uniform) bead p is read 64 entries at a time
+  // with one coalesced vector load; partner j's weight is then broadcast from
+  // lane j%64 with v_readlane.  Partner terms alternate between two
+  // accumulators by the parity of j.
+  const float *__restrict__ w1 = P.W + static_cast<size_t>(p) * P.ldw;
   float acc0 = 0.f, acc1 = 0.f;
-  int j = 0;
-  for (; j + 1 < n; j += 2) {
-    const float wa = w1[j], wb = w1[j + 1];
-    const float xa = X.get(j, 0), ya = X.get(j, 1), za = X.get(j, 2);
-    const float xb = X.get(j + 1, 0), yb = X.get(j + 1, 1), zb = X.get(j + 1, 2);
-    const float a0 = fmaf(dist3(px, py, pz, xa, ya, za), wa, -1.f);
-    const float a1 = fmaf(dist3(nx, ny, nz, xa, ya, za), wa, -1.f);
-    const float b0 = fmaf(dist3(px, py, pz, xb, yb, zb), wb, -1.f);
-    const float b1 = fmaf(dist3(nx, ny, nz, xb, yb, zb), wb, -1.f);
-    acc0 = fmaf(a1 - a0, a1 + a0, acc0);
-    acc1 = fmaf(b1 - b0, b1 + b0, acc1);
-  }
-  if (j < n) {
-    const float wa = w1[j];
-    const float xa = X.get(j, 0), ya = X.get(j, 1), za = X.get(j, 2);
-    const float a0 = fmaf(dist3(px, py, pz, xa, ya, za), wa, -1.f);
-    const float a1 = fmaf(dist3(nx, ny, nz, xa, ya, za), wa, -1.f);
-    acc0 = fmaf(a1 - a0, a1 + a0, acc0);
+  for (int c = 0; c < n; c += kLanes) {
+    const int cnt = min(kLanes, n - c);
+    const float wv = lane < cnt ? w1[c + lane] : 0.f;
+    int jj = 0;
+#pragma unroll 4
+    for (; jj + 1 < cnt; jj += 2) {
+      const float wa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), jj));
+      const float wb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), jj + 1));
+      const int j = c + jj;
+      const float xa = X.get(j, 0), ya = X.get(j, 1), za = X.get(j, 2);
+      const float xb = X.get(j + 1, 0), yb = X.get(j + 1, 1), zb = X.get(j + 1, 2);
+      const float a0 = fmaf(dist3(px, py, pz, xa, ya, za), wa, -1.f);
+      const float a1 = fmaf(dist3(nx, ny, nz, xa, ya, za), wa, -1.f);
+      const float b0 = fmaf(dist3(px, py, pz, xb, yb, zb), wb, -1.f);
+      const float b1 = fmaf(dist3(nx, ny, nz, xb, yb, zb), wb, -1.f);
+      acc0 = fmaf(a1 - a0, a1 + a0, acc0);
+      acc1 = fmaf(b1 - b0, b1 + b0, acc1);
+    }
+    if (jj < cnt) {  // cnt odd: only in the last chunk
+      const float wa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), jj));
+      const int j = c + jj;
+      const float xa = X.get(j, 0), ya = X.get(j, 1), za = X.get(j, 2);
+      const float a0 = fmaf(dist3(px, py, pz, xa, ya, za), wa, -1.f);
+      const float a1 = fmaf(dist3(nx, ny, nz, xa, ya, za), wa, -1.f);
+      acc0 = fmaf(a1 - a0, a1 + a0, acc0);
+    }
   }
   float d = acc0 + acc1;
   if (P.symmetric) return 2.f * d;
-  const float *__restrict__ w2 = P.W2 + static_cast<size_t>(p) * n;
+  const float *__restrict__ w2 = P.W2 + static_cast<size_t>(p) * P.ldw;
   float acc2 = 0.f;
-  for (int k = 0; k < n; ++k) {
-    const float w = w2[k];
-    const float xa = X.get(k, 0), ya = X.get(k, 1), za = X.get(k, 2);
-    const float a0 = fmaf(dist3(px, py, pz, xa, ya, za), w, -1.f);
-    const float a1 = fmaf(dist3(nx, ny, nz, xa, ya, za), w, -1.f);
-    acc2 = fmaf(a1 - a0, a1 + a0, acc2);
+  for (int c = 0; c < n; c += kLanes) {
+    const int cnt = min(kLanes, n - c);
+    const float wv = lane < cnt ? w2[c + lane] : 0.f;
+    for (int jj = 0; jj < cnt; ++jj) {
+      const float w = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), jj));
+      const int j = c + jj;
+      const float xa = X.get(j, 0), ya = X.get(j, 1), za = X.get(j, 2);
+      const float a0 = fmaf(dist3(px, py, pz, xa, ya, za), w, -1.f);
+      const float a1 = fmaf(dist3(nx, ny, nz, xa, ya, za), w, -1.f);
+      acc2 = fmaf(a1 - a0, a1 + a0, acc2);
+    }
   }
   return d + acc2;
 }
@@ -177,16 +232,22 @@ __device__ __forceinline__ float smooth_local(const DevProblem &P, int p, const 
 __device__ double resum_heatmap(const DevProblem &P, const PosView &X, float r) {
   const int n = P.n;
   double tot = 0.0;
+  const int lane = threadIdx.x & (kLanes - 1);
   for (int m = 0; m < n; ++m) {
     const float xm = X.get(m, 0), ym = X.get(m, 1), zm = X.get(m, 2);
-    const float *__restrict__ wr = P.W + static_cast<size_t>(m) * n;
+    const float *__restrict__ wr = P.W + static_cast<size_t>(m) * P.ldw;
     float part = 0.f;
     const int j0 = P.symmetric ? m + 1 : 0;
-    for (int j = j0; j < n; ++j) {
-      const float w = wr[j];
-      if (w == 0.f) continue;
-      const float t = fmaf(dist3(xm, ym, zm, X.get(j, 0), X.get(j, 1), X.get(j, 2)), w, -1.f);
-      part = fmaf(t, t, part);
+    for (int c = j0; c < n; c += kLanes) {
+      const int cnt = min(kLanes, n - c);
+      const float wv = lane < cnt ? wr[c + lane] : 0.f;
+      for (int jj = 0; jj < cnt; ++jj) {
+        const float w = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), jj));
+        if (w == 0.f) continue;
+        const int j = c + jj;
+        const float t = fmaf(dist3(xm, ym, zm, X.get(j, 0), X.get(j, 1), X.get(j, 2)), w, -1.f);
+        part = fmaf(t, t, part);
+      }
     }
     tot += static_cast<double>(part);
   }

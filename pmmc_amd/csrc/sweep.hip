@@ -24,7 +24,7 @@
 namespace pmmc {
 
 struct SweepArgs {
-  int n, replicas, inner, proposers;
+  int n, ldw, replicas, inner, proposers;
   uint64_t seed;
   uint32_t segment, iter;
   double T;
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(64) void k_sweep_propose(SweepArgs a) {
     sx[j] = a.snap[3 * j];
     sy[j] = a.snap[3 * j + 1];
     sz[j] = a.snap[3 * j + 2];
-    w[j] = a.W[static_cast<size_t>(b) * a.n + j];
+    w[j] = a.W[static_cast<size_t>(b) * a.ldw + j];
   }
   __syncthreads();
   float x = sx[b], y = sy[b], z = sz[b];
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(64) void k_sweep_propose(SweepArgs a) {
 __device__ double sweep_local64(const SweepArgs &a, int b, float x, float y, float z) {
   double e = 0.0;
   for (int j = 0; j < a.n; ++j) {
-    const float h = a.Hc[static_cast<size_t>(b) * a.n + j];
+    const float h = a.Hc[static_cast<size_t>(b) * a.ldw + j];
     if (h == 0.f) continue;
     const double c = dist3d(x, y, z, a.snap[3 * j], a.snap[3 * j + 1], a.snap[3 * j + 2]) / h - 1.0;
     e += c * c;
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void k_sweep_energy(SweepArgs a) {
   double s = 0.0;
   for (int m = threadIdx.x; m < a.n; m += 256)
     for (int j = 0; j < a.n; ++j) {
-      const float h = a.Hc[static_cast<size_t>(m) * a.n + j];
+      const float h = a.Hc[static_cast<size_t>(m) * a.ldw + j];
       if (h == 0.f) continue;
       const double c =
           dist3d(a.pos[3 * m], a.pos[3 * m + 1], a.pos[3 * m + 2], a.pos[3 * j], a.pos[3 * j + 1],
@@ -206,6 +206,7 @@ extern "C" pmmc_status pmmc_heatmap_sweep(pmmc_ctx *ctx, const pmmc_heatmap_prob
   hipStream_t st = ctx->stream;
   SweepArgs a;
   a.n = n;
+  a.ldw = h.ldw;
   a.replicas = R;
   a.inner = s->inner;
   a.proposers = s->proposers;
