@@ -1,11 +1,22 @@
-"""Benchmark: plain arc-level simulated annealing of one 2 Mb segment
-(400 anchors, 1,200 PET-cluster arcs) on one MI355X.
+"""Benchmark of the pMMC annealing hot path on one MI355X.
 
-One step = one plain solve (pmmc_arcs_anneal: all chains of the problem run to
-their stop rule, the best chain is returned).  Inputs are uploaded inside the
-step, as the drop-in does; `value` is counted moves per second over the timed
-steps (whole job).  The roofline figure uses the annealing kernel's own
-duration, measured with HIP events on the library's stream.
+Headline (`value`): plain arc-level simulated annealing of one 2 Mb segment
+(400 anchors, 1,200 PET-cluster arcs).  One step = one plain solve
+(pmmc_arcs_anneal: every chain runs to its stop rule, the best chain is
+returned; inputs are uploaded inside the step, as the drop-in does).  `value`
+is counted MC moves per second over the timed steps (whole job).  The
+roofline figure uses the annealing kernel's own duration, measured with HIP
+events on the library's stream.
+
+Beside it, in the same line:
+  batched -- 512 arc problems (32 segments x 16 ensemble seeds, 20-2,000
+             anchors) in one device-resident pool with a captured step loop,
+             against the same problems solved one plain call at a time
+             (timed on a subset, per-problem results checked identical);
+  served  -- Poisson arrivals of mixed heatmap / arc / smooth jobs through the
+             continuous-batching server, p50/p99 request latency and solves/s;
+  cpu_baseline -- the reference's CPU arc annealer restated in the oracle,
+             timed on the host's cores.
 """
 from __future__ import annotations
 
@@ -20,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MC moves/s + batched solves/s vs plain GPU; p50/p99 latency at matched energy"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_anneal_arcs_n400_hbm_traffic.txt")
 
 
 def genomic_step(anchors, noise=0.01, power=0.75, scale=0.5, base=1.0):
@@ -32,15 +44,31 @@ def genomic_step(anchors, noise=0.01, power=0.75, scale=0.5, base=1.0):
     return float(d.sum() / len(anchors) * noise)
 
 
-def workload():
-    from pmmc_amd import schedule_preset, settings, synth
-    anchors, arcs, ed = synth.arcs_inputs(n=400, n_arcs=1200, seed=0)
-    prob = synth.arcs_problem(n=400, n_arcs=1200, seed=0)
-    step = genomic_step(anchors)
+def arcs_settings():
+    from pmmc_amd import settings
     # [simulation_arcs] and [cuda] of auxiliary/configs/default.ini
-    s = settings(max_temp=5.0, dt_temp=0.9999, improvement=0.975, tjs=50.0, tjc=20.0,
-                 min_successes=100, steps=50000, milestone_fails=3)
-    return prob, s, step, arcs
+    return settings(max_temp=5.0, dt_temp=0.9999, improvement=0.975, tjs=50.0, tjc=20.0,
+                    min_successes=100, steps=50000, milestone_fails=3)
+
+
+def workload():
+    from pmmc_amd import synth
+    anchors, arcs, _ = synth.arcs_inputs(n=400, n_arcs=1200, seed=0)
+    prob = synth.arcs_problem(n=400, n_arcs=1200, seed=0)
+    return prob, arcs_settings(), genomic_step(anchors), arcs
+
+
+def measured_traffic_per_move():
+    """HBM bytes per move of the arc kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by its copy-kernel
+    calibration factor), or None when the summary is absent."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("hbm_bytes_per_move:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(prob, s, step, budget_s=15.0):
@@ -51,14 +79,12 @@ def cpu_baseline(prob, s, step, budget_s=15.0):
     sched = O.schedule_from(schedule_preset(2, 0, s, step))
     P = O.Problem(prob)
     threads = max(1, min(16, os.cpu_count() or 1))
-    # calibrate on one thread per core, then size the sample to the budget
     moves_per_chain = 50000
     sched.max_moves = moves_per_chain
     sched.block = 50000
     sched.milestone_period = 50000
-    _, st, secs = P.anneal_many(sched, 1, 0, 0, threads, threads)
-    per_chain = secs  # one chain per thread ran concurrently
-    rounds = max(1, int(budget_s / max(per_chain, 1e-3)))
+    _, _, per_round = P.anneal_many(sched, 1, 0, 0, threads, threads)
+    rounds = max(1, int(budget_s / max(per_round, 1e-3)))
     chains = threads * rounds
     _, stats, secs = P.anneal_many(sched, 1, 0, 0, chains, threads)
     moves = sum(x["moves"] for x in stats)
@@ -68,13 +94,119 @@ def cpu_baseline(prob, s, step, budget_s=15.0):
                       f"{secs:.1f} s on {threads} host threads"}
 
 
+def batched(ctx, chains_per_problem=256, plain_subset=16):
+    """Config 3: 512 problems in one pool vs one plain call per problem."""
+    import numpy as np
+    from pmmc_amd import schedule_preset, synth
+    s = arcs_settings()
+    probs, meta = synth.batch_arcs_problems(n_segments=32, n_seeds=16, seed=0)
+    sched = schedule_preset(2, 1, s, 0.03)
+    n = len(probs)
+    # batched
+    t0 = time.perf_counter()
+    res = ctx.anneal_batch(probs, sched, seeds=[1 + i for i in range(n)], n_chains=chains_per_problem,
+                           use_graph=True)
+    tb = time.perf_counter() - t0
+    mv_b = sum(r[1]["moves"] for r in res)
+    ac_b = sum(r[1]["accepted"] for r in res)
+    # plain, one problem at a time, on an evenly spread subset
+    idx = list(range(0, n, max(1, n // plain_subset)))[:plain_subset]
+    t0 = time.perf_counter()
+    same = True
+    mv_p = ac_p = 0
+    for i in idx:
+        pos, st = ctx.anneal(probs[i], sched, seed=1 + i, segment=i, n_chains=chains_per_problem)
+        mv_p += st["moves"]
+        ac_p += st["accepted"]
+        same &= bool(np.array_equal(pos, res[i][0])) and st["energy"] == res[i][1]["energy"]
+    tp = time.perf_counter() - t0
+    sub_moves_b = sum(res[i][1]["moves"] for i in idx)
+    return {
+        "problems": n, "chains_per_problem": chains_per_problem,
+        "anchors_min_max": [int(min(m[2] for m in meta)), int(max(m[2] for m in meta))],
+        "batched_solves_per_s": n / tb, "batched_moves_per_s": mv_b / tb,
+        "batched_accepted_moves_per_s": ac_b / tb,
+        "plain_solves_per_s": len(idx) / tp, "plain_moves_per_s": mv_p / tp,
+        "plain_accepted_moves_per_s": ac_p / tp,
+        "solves_speedup": (n / tb) / (len(idx) / tp),
+        "accepted_moves_speedup": (ac_b / tb) / (ac_p / tp),
+        "plain_subset": len(idx), "subset_moves_equal": sub_moves_b == mv_p,
+        "results_identical_alone_and_batched": same,
+        "final_energy_median": float(np.median([r[1]["energy"] for r in res])),
+    }
+
+
+def served(ctx, seconds=3.0, rate=200.0, seed=0):
+    """Config 4: Poisson arrivals of mixed jobs through the server."""
+    import numpy as np
+    from pmmc_amd import schedule_preset, settings, synth
+    rng = np.random.default_rng(seed)
+    hs = schedule_preset(1, 1, settings(steps=50000), 1.0)
+    hs.max_moves = 8192
+    asched = schedule_preset(2, 1, settings(steps=8192, improvement=0.975, min_successes=100), 0.03)
+    ss = schedule_preset(3, 1, settings(steps=8192, dt_temp=0.9995, improvement=0.99,
+                                        min_successes=50), 0.3)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 1.5)))
+    arrivals = arrivals[arrivals < seconds]
+    reqs = []
+    for k, t in enumerate(arrivals):
+        u = rng.random()
+        ens = int(rng.integers(1, 17))
+        if u < 0.4:
+            n = int(np.exp(rng.uniform(np.log(64), np.log(512))))
+            p = synth.heatmap_problem(n=n, seed=k)[0]
+            sc = hs
+        elif u < 0.8:
+            n = int(np.exp(rng.uniform(np.log(20), np.log(2000))))
+            p = synth.arcs_problem(n=n, n_arcs=3 * n, seed=k)
+            sc = asched
+        else:
+            a = int(np.exp(rng.uniform(np.log(21), np.log(401))))
+            p = synth.smooth_problem(n_anchors=a, add=49, seed=k)
+            sc = ss
+        reqs.append((float(t), p, sc, ens))
+    srv = ctx.server(hbm_budget=64 << 30, max_jobs=4096, blocks_per_step=4)
+    t0 = time.perf_counter()
+    jobs = {}
+    done_at = {}
+    nxt = 0
+    while len(done_at) < len(reqs):
+        now = time.perf_counter() - t0
+        while nxt < len(reqs) and reqs[nxt][0] <= now:
+            t, p, sc, ens = reqs[nxt]
+            jobs[nxt] = [srv.submit(p, sc, seed=1 + e, segment=nxt, n_chains=64) for e in range(ens)]
+            nxt += 1
+        if not jobs or all(k in done_at for k in jobs):
+            if nxt < len(reqs):
+                time.sleep(max(0.0, min(0.001, reqs[nxt][0] - now)))
+            continue
+        n_done, _, _ = srv.step()
+        if n_done:
+            now = time.perf_counter() - t0
+            for k, ids in jobs.items():
+                if k not in done_at and all(srv.done(j) for j in ids):
+                    done_at[k] = now
+    wall = time.perf_counter() - t0
+    lat = np.array([done_at[k] - reqs[k][0] for k in range(len(reqs))]) * 1e3
+    solves = sum(len(v) for v in jobs.values())
+    srv.close()
+    return {"requests": len(reqs), "arrival_rate_per_s": rate, "solves": solves,
+            "solves_per_s": solves / wall, "requests_per_s": len(reqs) / wall,
+            "p50_latency_ms": float(np.percentile(lat, 50)),
+            "p99_latency_ms": float(np.percentile(lat, 99)),
+            "mix": "40% heatmap 64-512 beads, 40% arcs 20-2000 anchors, 20% smooth 1k-20k beads; "
+                   "ensemble 1-16 solves of 64 chains per request"}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--chains", type=int, default=0, help="chains per solve (0 = 64 per CU)")
+    ap.add_argument("--chains", type=int, default=0, help="chains per solve (0 = library default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -103,8 +235,14 @@ def main(argv=None):
             energies.append(st["energy"])
             chains = st["n_chains"]
         wall = time.perf_counter() - t0
+        extra = {}
+        if not a.no_batched:
+            extra["batched"] = batched(ctx)
+        if not a.no_served:
+            extra["served"] = served(ctx)
     value = moves / wall
     achieved = bytes_per_move * moves / (kern_ms * 1e-3) / 1e9
+    tpm = measured_traffic_per_move()
     line = {
         "metric": METRIC, "value": value, "unit": "moves/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True,
@@ -119,11 +257,15 @@ def main(argv=None):
         "kernel_moves_per_s": moves / (kern_ms * 1e-3),
         "final_energy_median": float(np.median(energies)),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None if tpm is None else tpm * moves / a.steps,
                      "bytes_per_move": bytes_per_move,
                      "accounting": "CSR variant: 2 evaluations x deg x (12 B position + 4 B "
-                                   "target + 4 B index) per proposal"},
+                                   "target + 4 B index) per proposal; traffic = measured HBM "
+                                   "bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE "
+                                   "x1.0, copy-kernel calibrated)"},
     }
+    line.update(extra)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(prob, s, step)
     print(json.dumps(line))
