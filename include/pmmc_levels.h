@@ -1,0 +1,200 @@
+/*
+ * pmmc_levels.h -- C ABI of the host-side rows around the annealing kernels:
+ * the cluster tree and bead indexing, sub-anchor densification, the arc and
+ * heatmap expected-distance inputs, and the level drivers that run the
+ * annealers over a tree.
+ *
+ * Pure C (stdint.h only), caller-owned host memory, pmmc_status returns as in
+ * pmmc_hip.h.  Host code only: everything here runs on the CPU except the
+ * level drivers, which launch the annealing kernels through pmmc_hip.h.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * reference's src/ directory):
+ *   pmmc_tree_create        LooperSolver::createTreeGenome / createTreeChromosome
+ *                           LooperSolver.hpp:2272-2438, findGaps :2119-2180,
+ *                           findSplit :2185-2269 (arcs shifted to global
+ *                           cluster indices :2319-2329)
+ *   pmmc_tree_level         LooperSolver::setLevel / levelDown :2441-2452,
+ *                           :3087-3104 (children in stored order, no sort)
+ *   pmmc_tree_densify       LooperSolver::densifyActiveRegion :3836-3898
+ *   pmmc_tree_set_dist_to_next  dist_to_next / noise_size of
+ *                           reconstructClusterArcsDistances :4339-4354 with
+ *                           genomicLengthToDistance :3900-3904
+ *   pmmc_tree_expected_distances
+ *                           LooperSolver::calcAnchorExpectedDistancesHeatmap
+ *                           :5494-5627
+ *   pmmc_freq_to_distance   LooperSolver::freqToDistance :3949-3967
+ *   pmmc_distance_heatmap   LooperSolver::createDistanceHeatmap :3042-3085,
+ *                           freqToDistanceHeatmap(Inter) :3934-3942
+ *
+ * Errors: the message of the last failure on the calling thread is returned
+ * by pmmc_host_last_error() (the reference prints and exits instead).
+ */
+#ifndef PMMC_LEVELS_H
+#define PMMC_LEVELS_H
+
+#include <stdint.h>
+
+#include "pmmc_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Settings of the host-side rows; names follow Settings.hpp.  Defaults
+ * (pmmc_level_params_default) are the reference's (Settings.hpp:395-461). */
+typedef struct {
+  float count_a;            /* countToDistA: < 0 selects the power-law branch  */
+  float count_scale;        /* countToDistScale                                */
+  float count_shift;        /* countToDistShift                                */
+  float count_base;         /* countToDistBaseLevel                            */
+  float heat_scale;         /* freqToDistHeatmapScale                          */
+  float heat_power;         /* freqToDistHeatmapPower                          */
+  float heat_scale_inter;   /* freqToDistHeatmapScaleInter                     */
+  float heat_power_inter;   /* freqToDistHeatmapPowerInter                     */
+  float heat_stretching;    /* heatmapDistanceHeatmapStretching                */
+  float glen_base;          /* genomicLengthToDistBase                         */
+  float glen_scale;         /* genomicLengthToDistScale                        */
+  float glen_power;         /* genomicLengthToDistPower                        */
+  int32_t loop_density;     /* loopDensity: beads inserted between anchors     */
+  float noise_chr;          /* noiseCoefficientLevelChr                        */
+  float noise_segment;      /* noiseCoefficientLevelSegment                    */
+  float noise_anchor;       /* noiseCoefficientLevelAnchor                     */
+  float noise_subanchor;    /* noiseCoefficientLevelSubanchor                  */
+  float noise_anchor_small; /* noiseAnchorSmall                                */
+  int32_t restarts_chr;     /* simulationStepsLevelChr                         */
+  int32_t restarts_segment; /* simulationStepsLevelSegment                     */
+  int32_t restarts_anchor;  /* simulationStepsLevelAnchor                      */
+  int32_t restarts_subanchor; /* simulationStepsLevelSubanchor                 */
+  int32_t use_anchor_heatmap;      /* useAnchorHeatmap                         */
+  float anchor_heatmap_influence;  /* anchorHeatmapInfluence                   */
+  int32_t use2d;            /* use2D                                           */
+} pmmc_level_params;
+
+void pmmc_level_params_default(pmmc_level_params *p);
+
+/* Message of the last failed call of this header on the calling thread. */
+const char *pmmc_host_last_error(void);
+
+/* ---- expected-distance rules ---------------------------------------------- */
+
+/* PET count -> expected distance (float, as the reference returns it).  The
+ * reference's memo leaves count 1 uninitialised (LooperSolver.hpp:950-951);
+ * here every count is computed. */
+float pmmc_freq_to_distance(int32_t count, const pmmc_level_params *p);
+/* genomic length (bp) -> distance, double as the reference computes it */
+double pmmc_genomic_length_to_distance(int32_t length, const pmmc_level_params *p);
+
+/* Distance heatmap of an n x n frequency heatmap: -1 inside the diagonal
+ * band (|i-j| < diagonal_size), 0 where the frequency is below 1e-6, else
+ * scale * f^power (the inter-chromosome rule when inter != 0, which the
+ * reference uses for heatmap index 0); then values above
+ * avg * heat_stretching are clamped.  Only the upper triangle of freq is
+ * read (the output is symmetric).  out: n*n row-major.  *avg_out (nullable)
+ * = the average the clamp used (Heatmap::getAvg, Heatmap.hpp:440-451). */
+pmmc_status pmmc_distance_heatmap(const float *freq, int32_t n, int32_t diagonal_size,
+                                  int32_t inter, const pmmc_level_params *p, float *out,
+                                  float *avg_out);
+
+/* ---- cluster tree ---------------------------------------------------------- */
+
+typedef struct pmmc_tree pmmc_tree;
+
+/* One chromosome after arc marking (InteractionArcs::anchors / arcs /
+ * long_arcs, InteractionArcs.hpp:151-170). */
+typedef struct {
+  const char *name;
+  int32_t n_anchors;
+  const int32_t *anchor_start;  /* [n_anchors] bp, sorted by start          */
+  const int32_t *anchor_end;    /* [n_anchors] bp                           */
+  const char *orientation;      /* [n_anchors] 'L' 'R' 'N', nullable        */
+  int32_t n_arcs;
+  const int32_t *arc_start;     /* [n_arcs] anchor index (chromosome-local) */
+  const int32_t *arc_end;       /* [n_arcs]                                  */
+  const int32_t *arc_score;     /* [n_arcs] PET count                        */
+  int32_t n_long_arcs;          /* long-range arcs, genomic positions       */
+  const int32_t *long_start;
+  const int32_t *long_end;
+  const int32_t *long_score;
+} pmmc_chromosome;
+
+/* One predefined segment (the reference's segment-split BED file, in file
+ * order). */
+typedef struct {
+  const char *chr;
+  int32_t start, end;
+} pmmc_segment_region;
+
+/* One cluster of the tree (Cluster.hpp:86-102).  Levels: 1 chromosome root,
+ * 2 segment, 3 interaction block, 4 anchor, 5 sub-anchor bead. */
+typedef struct {
+  int32_t start, end, genomic_pos;
+  int32_t base_start, base_end;
+  int32_t level;
+  int32_t parent;              /* -1 for roots and sub-anchor beads         */
+  int32_t n_children;
+  int32_t is_fixed;
+  int32_t orientation;         /* 'L' 'R' 'N'                               */
+  float pos[3];
+  double dist_to_next;
+} pmmc_cluster;
+
+/* Builds the tree of every chromosome, in order: per chromosome its anchors,
+ * then interleaved interaction blocks and segments, then its root.  With
+ * n_regions > 0 the segment boundaries follow the predefined regions (the
+ * path every pMMC run takes); with none, every interaction block is a
+ * segment.  Fails where the reference calls error() or reads past a list. */
+pmmc_status pmmc_tree_create(int32_t n_chr, const pmmc_chromosome *chrs, int32_t n_regions,
+                             const pmmc_segment_region *regions, pmmc_tree **out);
+void pmmc_tree_destroy(pmmc_tree *t);
+
+int32_t pmmc_tree_size(const pmmc_tree *t);
+int32_t pmmc_tree_n_chromosomes(const pmmc_tree *t);
+pmmc_status pmmc_tree_get(const pmmc_tree *t, int32_t first, int32_t count, pmmc_cluster *out);
+/* children of a cluster; *n = count even when it exceeds cap */
+pmmc_status pmmc_tree_children(const pmmc_tree *t, int32_t cluster, int32_t *out, int32_t cap,
+                               int32_t *n);
+pmmc_status pmmc_tree_chromosome(const pmmc_tree *t, int32_t chr, int32_t *first_cluster,
+                                 int32_t *root);
+/* arcs of a chromosome with global cluster indices: out[2*i], out[2*i+1] */
+pmmc_status pmmc_tree_arcs(const pmmc_tree *t, int32_t chr, int32_t *out, int32_t cap,
+                           int32_t *n);
+/* current_level[chr] after setLevel(level), level 0 = the root */
+pmmc_status pmmc_tree_level(const pmmc_tree *t, int32_t chr, int32_t level, int32_t *out,
+                            int32_t cap, int32_t *n);
+pmmc_status pmmc_tree_set_positions(pmmc_tree *t, int32_t n, const int32_t *clusters,
+                                    const float *xyz);
+pmmc_status pmmc_tree_get_positions(const pmmc_tree *t, int32_t n, const int32_t *clusters,
+                                    float *xyz);
+pmmc_status pmmc_tree_set_fixed(pmmc_tree *t, int32_t n, const int32_t *clusters,
+                                const int32_t *fixed);
+
+/* densifyActiveRegion over the children of `cluster`: inserts loop_density
+ * beads between consecutive children (genomic spacing range/(add+1), linear
+ * interpolation of positions), appends them to the tree, fixes the children
+ * when fix != 0 and makes the densified list the cluster's children.
+ * *n_active (nullable) = the new number of children. */
+pmmc_status pmmc_tree_densify(pmmc_tree *t, int32_t cluster, int32_t loop_density, int32_t fix,
+                              int32_t *n_active);
+
+/* Sets dist_to_next of the children of `cluster` from their genomic spacing
+ * and returns the mean spacing times noise_anchor / noise_subanchor (the
+ * reference's noise_size; either pointer nullable). */
+pmmc_status pmmc_tree_set_dist_to_next(pmmc_tree *t, int32_t cluster, const pmmc_level_params *p,
+                                       float *noise_anchor, float *noise_subanchor);
+
+/* Expected distances between the children of interaction block `cluster` of
+ * chromosome `chr`: -1 = no arc, 0 on the diagonal, freq_to_distance of the
+ * arc's count for arcs (and long arcs resolved to anchors) between children;
+ * an arc whose other end lies outside the block is written against child 0,
+ * as the reference does.  anchor_heatmap (nullable, n*n) applies the
+ * useAnchorHeatmap shrink when p->use_anchor_heatmap.  out: n*n row-major,
+ * n = number of children. */
+pmmc_status pmmc_tree_expected_distances(const pmmc_tree *t, int32_t chr, int32_t cluster,
+                                         const pmmc_level_params *p,
+                                         const float *anchor_heatmap, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PMMC_LEVELS_H */

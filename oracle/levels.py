@@ -1,0 +1,416 @@
+"""CPU restatement of the host-side rows around the annealing kernels --
+TEST INFRASTRUCTURE ONLY (imported by tests/, never by pmmc_amd/).
+
+Restates, in plain Python with float32 arithmetic where the reference computes
+in float (numpy float32 scalars round every operation to float32, as C does;
+powf/pow/exp are libm's, through ctypes):
+  build_tree        LooperSolver::createTreeGenome / createTreeChromosome
+                    (LooperSolver.hpp:2272-2438), findGaps (:2119-2180),
+                    findSplit (:2185-2269)
+  set_level         LooperSolver::setTopLevel / setLevel / levelDown
+                    (:2431-2452, :3087-3104)
+  densify           LooperSolver::densifyActiveRegion (:3836-3898)
+  dist_to_next      dist_to_next / noise_size of reconstructClusterArcsDistances
+                    (:4339-4354) with genomicLengthToDistance (:3900-3904)
+  freq_to_distance  LooperSolver::freqToDistance (:3949-3967)
+  expected_distances LooperSolver::calcAnchorExpectedDistancesHeatmap
+                    (:5494-5627)
+  distance_heatmap  LooperSolver::createDistanceHeatmap (:3042-3085) with
+                    freqToDistanceHeatmap(Inter) (:3934-3942) and
+                    Heatmap::getDiagonalSize / getAvg / getRange
+                    (Heatmap.hpp:356-367, :427-451)
+
+PINNED BY tests/golden/levels_ref.json, written by the reference's own code
+(oracle/_ref/levels_ref), and by the invariants in tests/test_levels.py
+(children partition parents, genomic monotonicity, base ranges, counts).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+_libm = ctypes.CDLL("libm.so.6")
+_libm.powf.restype = ctypes.c_float
+_libm.powf.argtypes = [ctypes.c_float, ctypes.c_float]
+F = np.float32
+INT_MAX = 2**31 - 1
+
+
+class ReferenceError_(RuntimeError):
+    """The reference calls error() (prints and exits) for this input."""
+
+
+def c_div(a, b):
+    q = abs(a) // abs(b)
+    return q if (a >= 0) == (b >= 0) else -q
+
+
+def new_cluster(start, end):
+    """Cluster::Cluster(int, int) (Cluster.hpp:115-136)."""
+    return dict(start=start, end=end, genomic_pos=c_div(start + end, 2), orientation="N",
+                base_start=0, base_end=0, parent=-1, is_fixed=False, level=0,
+                children=[], arcs=[], pos=[F(0), F(0), F(0)])
+
+
+# ------------------------------------------------------------------ T1
+def _find_gaps(t, chr_name):
+    """LooperSolver::findGaps (LooperSolver.hpp:2119-2180)."""
+    cl, arcs = t["clusters"], t["arcs"][chr_name]
+    start = t["chr_first"][chr_name]
+    gaps = [start]
+    cnt = 0
+    for i in range(start, len(cl)):
+        for a in cl[i]["arcs"]:
+            s, e = arcs[a][0], arcs[a][1]
+            other = e if s == i else (s if e == i else -1)
+            if other == i:
+                continue
+            cnt += 1 if other > i else -1
+        if cnt == 0:
+            gaps.append(i)
+    if len(cl) - 1 not in gaps:
+        gaps.append(len(cl) - 1)
+    regions = t["splits"]
+    if regions:
+        for rc, _, rend in regions:
+            if rc != chr_name:
+                continue
+            for i in range(start, len(cl)):
+                if cl[i]["end"] >= rend:
+                    if i not in gaps:
+                        gaps.append(i)
+                    break
+        gaps = sorted(set(gaps))
+    return gaps
+
+
+def _find_split(t, gaps, chr_name):
+    """LooperSolver::findSplit (LooperSolver.hpp:2185-2269)."""
+    cl, regions = t["clusters"], t["splits"]
+    if len(cl) < len(gaps):
+        raise ReferenceError_(f"find split: size mismatch, gaps={len(gaps)}, clusters={len(cl)}")
+    if not regions:
+        return gaps
+    splits = [gaps[0]]
+    first, last = -1, len(regions) - 1
+    for i, (rc, _, _) in enumerate(regions):
+        if rc == chr_name:
+            if first == -1:
+                first = i
+        elif first != -1:
+            last = i - 1
+            break
+    if first == -1:
+        raise ReferenceError_(f"no segment split found for {chr_name}")
+    for s in range(first, last):
+        boundary = regions[s][2]
+        best, best_d = -1, INT_MAX
+        for i in range(1, len(gaps) - 1):
+            d = abs(cl[gaps[i]]["end"] - boundary)
+            if d < best_d:
+                best, best_d = i, d
+        if best >= 0 and gaps[best] not in splits:
+            splits.append(gaps[best])
+    splits.sort()
+    if gaps[-1] not in splits:
+        splits.append(gaps[-1])
+    return splits
+
+
+def build_tree(chrs, splits):
+    """createTreeGenome (LooperSolver.hpp:2272-2293) over createTreeChromosome
+    (:2296-2438).  chrs: [{name, anchors [(start, end)], orient, arcs
+    [(a, b, score)] with chromosome-local anchor indices}]; splits: predefined
+    segments [(chr, start, end)] in file order."""
+    t = dict(clusters=[], arcs={}, scores={}, chr_first={}, chr_root={}, chrs=[c["name"] for c in chrs],
+             splits=[tuple(s) for s in splits], anchors={})
+    cl = t["clusters"]
+    for c in chrs:
+        name = c["name"]
+        cs = len(cl)
+        t["chr_first"][name] = cs
+        t["anchors"][name] = [tuple(a) for a in c["anchors"]]
+        for i, (s, e) in enumerate(c["anchors"]):
+            k = new_cluster(s, e)
+            k["orientation"] = c.get("orient", ["N"] * len(c["anchors"]))[i]
+            k["base_start"] = k["base_end"] = cs + i
+            k["level"] = 4
+            cl.append(k)
+        t["arcs"][name] = [[a + cs, b + cs] for a, b, _ in c["arcs"]]
+        t["scores"][name] = [sc for _, _, sc in c["arcs"]]
+        for i, (a, b) in enumerate(t["arcs"][name]):
+            cl[a]["arcs"].append(i)
+            cl[b]["arcs"].append(i)
+        root = new_cluster(0, 0)
+        gaps = _find_gaps(t, name)
+        spl = _find_split(t, gaps, name)
+        next_split = 1
+        start_gap = len(cl)
+        for i in range(1, len(gaps)):
+            prev_gap = gaps[i - 1] if i == 1 else gaps[i - 1] + 1
+            cur_gap = gaps[i]
+            k = new_cluster(cl[prev_gap]["start"], cl[cur_gap]["end"])
+            k["base_start"], k["base_end"], k["level"] = prev_gap, cur_gap, 3
+            for j in range(prev_gap, cur_gap + 1):
+                k["children"].append(j)
+                cl[j]["parent"] = len(cl)
+            cl.append(k)
+            if gaps[i] == spl[next_split]:
+                end_gap = len(cl) - 1
+                s = new_cluster(cl[start_gap]["start"], cl[end_gap]["end"])
+                s["base_start"] = cl[start_gap]["base_start"]
+                s["base_end"] = cl[end_gap]["base_end"]
+                s["level"] = 2
+                seg = len(cl)
+                for j in range(start_gap, end_gap + 1):
+                    s["children"].append(j)
+                    cl[j]["parent"] = seg
+                root["children"].append(seg)
+                cl.append(s)
+                start_gap = len(cl)
+                next_split += 1
+        if not root["children"]:
+            raise ReferenceError_(f"chromosome {name}: no segment (reads children[0] of an empty list)")
+        root["level"] = 1
+        root["start"] = cl[root["children"][0]]["start"]
+        root["end"] = cl[root["children"][-1]]["end"]
+        root["genomic_pos"] = c_div(root["start"] + root["end"], 2)
+        root["base_start"] = cl[root["children"][0]]["base_start"]
+        root["base_end"] = cl[root["children"][-1]]["base_end"]
+        for ch in root["children"]:
+            cl[ch]["parent"] = len(cl)
+        t["chr_root"][name] = len(cl)
+        cl.append(root)
+    return t
+
+
+def set_level(t, level):
+    """setLevel = setTopLevel + level x levelDown (LooperSolver.hpp:2431-2452,
+    :3087-3104); children keep their stored order."""
+    cur = {c: [t["chr_root"][c]] for c in t["chrs"]}
+    for _ in range(level):
+        for c in t["chrs"]:
+            nxt = []
+            for k in cur[c]:
+                ch = t["clusters"][k]["children"]
+                nxt.extend(ch if ch else [k])
+            cur[c] = nxt
+    return cur
+
+
+# ------------------------------------------------------------------ S5
+def densify(t, ib, loop_density, fix):
+    """densifyActiveRegion(ib, fix) (LooperSolver.hpp:3836-3898) with
+    active_region = children of ib.  Appends the new beads to t's clusters and
+    returns the new active region (also stored as ib's children)."""
+    cl = t["clusters"]
+    active = list(cl[ib]["children"])
+    if not active:
+        return active
+    add = loop_density
+    level = cl[active[0]]["level"] + 1
+    out = []
+    for i in range(len(active) - 1):
+        a, b = cl[active[i]], cl[active[i + 1]]
+        if fix:
+            a["is_fixed"] = True
+        rng = b["start"] - a["end"]
+        d = c_div(rng, add + 1)
+        p = a["end"]
+        out.append(active[i])
+        dst = F(F(1.0) / F(add + 1))
+        st = dst
+        for _ in range(add):
+            p += d
+            k = new_cluster(p, p)
+            k["pos"] = [F(pa + F(F(pb - pa) * st)) for pa, pb in zip(a["pos"], b["pos"])]
+            k["level"] = level
+            cl.append(k)
+            out.append(len(cl) - 1)
+            st = F(st + dst)
+    out.append(active[-1])
+    if fix:
+        cl[active[-1]]["is_fixed"] = True
+    cl[ib]["children"] = list(out)
+    return out
+
+
+def genomic_length_to_distance(length, p):
+    """genomicLengthToDistance (LooperSolver.hpp:3900-3904): double math with
+    float settings."""
+    return float(F(p["glen_base"])) + float(F(p["glen_scale"])) * math.pow(length / 1000.0,
+                                                                        float(F(p["glen_power"])))
+
+
+def dist_to_next(genomic_pos, p, coef_subanchor=0.5, coef_anchor=0.5):
+    """dist_to_next and noise_size of reconstructClusterArcsDistances
+    (LooperSolver.hpp:4339-4354).  Returns (float32 dists, noise for the
+    subanchor coefficient, noise for the anchor coefficient)."""
+    out = []
+    noise = F(0.0)
+    for i in range(len(genomic_pos) - 1):
+        d = abs(genomic_pos[i + 1] - genomic_pos[i])
+        dist = F(genomic_length_to_distance(d, p))
+        out.append(dist)
+        noise = F(noise + dist)
+    noise = F(noise / F(len(genomic_pos)))
+    return np.array(out, np.float32), F(noise * F(coef_subanchor)), F(noise * F(coef_anchor))
+
+
+# ------------------------------------------------------------------ A6
+def freq_to_distance(freq, p):
+    """LooperSolver::freqToDistance (LooperSolver.hpp:3949-3967); the memo of
+    counts 2..100 holds the same values (:950-951)."""
+    a, scale, shift, base = F(p["count_a"]), F(p["count_scale"]), F(p["count_shift"]), F(p["count_base"])
+    if a < 0.0:
+        fi = int(F(F(freq) + shift))        # float -> int truncates toward zero
+        fi = max(fi, 1)
+        return F(float(base) + float(scale) * math.pow(float(fi), float(a)))
+    x = float(F(F(freq) + shift))
+    try:
+        e = math.exp(float(a) * x)
+    except OverflowError:      # C's exp returns +inf
+        e = math.inf
+    return F(float(base) + float(scale) / e)
+
+
+def diagonal_size(h):
+    """Heatmap::getDiagonalSize (Heatmap.hpp:356-367)."""
+    n = h.shape[0]
+    for w in range(n):
+        for i in range(n - w):
+            if float(h[i, i + w]) > 1e-6:
+                return w
+    return 0
+
+
+def heat_range(h):
+    """Heatmap::getRange (Heatmap.hpp:427-438) -> (min, max)."""
+    mx, mn = F(0.0), F(1e9)
+    w = diagonal_size(h)
+    n = h.shape[0]
+    for i in range(n):
+        for j in range(i + w, n):
+            v = h[i, j]
+            if v < mn:
+                mn = v
+            if v > mx:
+                mx = v
+    return mn, mx
+
+
+def heat_avg(h):
+    """Heatmap::getAvg (Heatmap.hpp:440-451): sequential float sum."""
+    n = h.shape[0]
+    w = diagonal_size(h)
+    s = F(0.0)
+    cnt = 0
+    for i in range(n):
+        for j in range(i + w, n):
+            s = F(s + h[i, j])
+            cnt += 1
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return F(s / F(cnt))
+
+
+def expected_distances(t, chr_name, ib, long_arcs, p, anchor_heatmap=None, influence=0.5):
+    """calcAnchorExpectedDistancesHeatmap (LooperSolver.hpp:5494-5627) with
+    active_region = children of ib and current_chr = chr_name.  An arc whose
+    other end lies outside the active region lands in slot 0, as the
+    reference's std::map::operator[] default does (:5538-5539)."""
+    cl = t["clusters"]
+    active = cl[ib]["children"]
+    n = len(active)
+    m = np.full((n, n), -1.0, np.float32)
+    for i in range(n):
+        m[i, i] = 0.0
+    c2a = {}
+    for i, c in enumerate(active):
+        c2a[c] = i
+    arcs, scores = t["arcs"][chr_name], t["scores"][chr_name]
+    for i, ai in enumerate(active):
+        for a in cl[ai]["arcs"]:
+            s, e = arcs[a]
+            other = e if s == ai else (s if e == ai else -1)
+            if other == -1:
+                raise ReferenceError_("no other end")
+            if other < ai:
+                continue
+            ed = freq_to_distance(scores[a], p)
+            x = c2a[ai]
+            y = c2a.setdefault(other, 0)
+            m[x, y] = ed
+            m[y, x] = ed
+    anc = t["anchors"].get(chr_name)
+    if long_arcs and anc is not None:
+        n_anc = len(anc)
+        cs = t["chr_first"][chr_name]
+        a2a = [-1] * n_anc
+        for ci in sorted(c2a):
+            if cs <= ci < cs + n_anc:
+                a2a[ci - cs] = c2a[ci]
+
+        def pos_to_anc(x):
+            for k, (s, e) in enumerate(anc):
+                if s <= x <= e:
+                    return k
+            return -1
+
+        for s, e, sc in long_arcs:
+            ia, ib2 = pos_to_anc(s), pos_to_anc(e)
+            if ia < 0 or ib2 < 0 or ia == ib2:
+                continue
+            x, y = a2a[ia], a2a[ib2]
+            if x < 0 or y < 0:
+                continue
+            if m[x, y] >= 0.0:
+                continue
+            ed = freq_to_distance(sc, p)
+            m[x, y] = ed
+            m[y, x] = ed
+    if anchor_heatmap is not None:
+        h = np.asarray(anchor_heatmap, np.float32)
+        if h.shape != (n, n):
+            raise ReferenceError_("anchor heatmaps size mismatch!")
+        _, mx = heat_range(h)
+        if mx > 1e-6:
+            infl = F(influence)
+            for i in range(n):
+                for j in range(i + 1, n):
+                    if m[i, j] < 0.0:
+                        continue
+                    s = F(F(h[i, j] / mx) * infl)
+                    if s > 1.0:
+                        s = F(1.0)
+                    m[i, j] = F(m[i, j] * F(F(1.0) - s))
+                    m[j, i] = m[i, j]
+    return m
+
+
+# ------------------------------------------------------------------ H6
+def distance_heatmap(freq, diag, inter, p):
+    """createDistanceHeatmap (LooperSolver.hpp:3042-3085); inter selects
+    freqToDistanceHeatmapInter (heatmap index 0).  Returns (h, avg used for
+    the clamp)."""
+    f = np.asarray(freq, np.float32)
+    n = f.shape[0]
+    scale = F(p["heat_scale_inter"] if inter else p["heat_scale"])
+    power = F(p["heat_power_inter"] if inter else p["heat_power"])
+    h = np.zeros((n, n), np.float32)
+    for i in range(n):
+        for j in range(i, n):
+            v = f[i, j]
+            if float(v) < 1e-6:
+                x = F(0.0)
+            elif abs(i - j) < diag:
+                x = F(-1.0)
+            else:
+                x = F(scale * F(_libm.powf(float(v), float(power))))
+            h[i, j] = x
+            h[j, i] = x
+    avg = heat_avg(h)
+    mx = F(avg * F(p["stretching"]))
+    h[h > mx] = mx
+    return h, avg

@@ -1,0 +1,250 @@
+// Test infrastructure only: runs the reference's own host code, compiled in
+// place from its sources, on inputs read from stdin and prints the results as
+// text (floats as C99 hex literals, so nothing is lost).  Used to write the
+// golden fixtures of tests/golden/ for the host-side rows of the annealing
+// path and to cross-check the oracle's energies.  Nothing here is shipped.
+//
+// Modes (first token on stdin):
+//   tree      LooperSolver::createTreeGenome (LooperSolver.hpp:2272-2438)
+//             with findGaps / findSplit (:2119-2269), then setLevel(0..4)
+//             (:2441-2452, levelDown :3087-3104)
+//   densify   tree, then LooperSolver::densifyActiveRegion (:3836-3898) on one
+//             interaction block and genomicLengthToDistance (:3900-3904)
+//   expdist   tree, then LooperSolver::calcAnchorExpectedDistancesHeatmap
+//             (:5494-5627) on one interaction block
+//   distheat  LooperSolver::createDistanceHeatmap (:3042-3085)
+//   score     calcScoreHeatmapActiveRegion (:3583-3612),
+//             calcScoreDistancesActiveRegion (:3215-3247),
+//             calcScoreStructureSmooth (:3414-3451)
+//
+// The reference keeps these members private; this driver is a white-box test,
+// so it lifts the access specifiers before including the reference header.
+#include <bits/stdc++.h>
+#define private public
+#define protected public
+#include <LooperSolver.hpp>
+#undef private
+#undef protected
+#include <Static.hpp>
+
+static std::string tok() {
+  std::string s;
+  if (!(std::cin >> s)) {
+    fprintf(stderr, "unexpected end of input\n");
+    exit(2);
+  }
+  return s;
+}
+static int geti() { return std::stoi(tok()); }
+static float getf() { return std::strtof(tok().c_str(), nullptr); }
+
+// Input: n_chr, per chromosome "name n_anchors n_arcs", anchors "start end
+// orient", arcs "a b score" (anchor indices local to the chromosome), then
+// n_seg and "chr start end" predefined segments.
+static void read_tree(LooperSolver &s) {
+  int n_chr = geti();
+  std::vector<std::string> names;
+  for (int c = 0; c < n_chr; ++c) {
+    std::string chr = tok();
+    int na = geti(), nr = geti();
+    names.push_back(chr);
+    std::vector<Anchor> anc;
+    for (int i = 0; i < na; ++i) {
+      int st = geti(), en = geti();
+      char o = tok()[0];
+      anc.push_back(Anchor(chr, st, en, o));
+    }
+    std::vector<InteractionArc> arcs;
+    for (int i = 0; i < nr; ++i) {
+      int a = geti(), b = geti(), sc = geti();
+      arcs.push_back(InteractionArc(a, b, sc));
+    }
+    s.arcs.anchors[chr] = anc;
+    s.arcs.anchors_cnt[chr] = na;
+    s.arcs.arcs[chr] = arcs;
+    s.arcs.arcs_cnt[chr] = nr;
+  }
+  int n_seg = geti();
+  for (int i = 0; i < n_seg; ++i) {
+    std::string chr = tok();
+    int st = geti(), en = geti();
+    s.segments_predefined.regions.push_back(BedRegion(chr, st, en));
+  }
+  s.arcs.chrs = names;
+  s.chrs = names;
+  s.createTreeGenome();
+}
+
+static void print_tree(LooperSolver &s) {
+  printf("\nclusters %zu\n", s.clusters.size());
+  for (size_t i = 0; i < s.clusters.size(); ++i) {
+    Cluster &c = s.clusters[i];
+    printf("%d %d %d %d %d %d %d %zu", c.start, c.end, c.genomic_pos, c.base_start, c.base_end,
+           c.level, c.parent, c.children.size());
+    for (int ch : c.children) printf(" %d", ch);
+    printf("\n");
+  }
+  for (const std::string &chr : s.chrs) {
+    printf("chr %s %d %d\n", chr.c_str(), s.chr_first_cluster[chr], s.chr_root[chr]);
+    printf("arcs %zu", s.arcs.arcs[chr].size());
+    for (auto &a : s.arcs.arcs[chr]) printf(" %d %d", a.start, a.end);
+    printf("\n");
+  }
+  for (int lvl = 0; lvl <= 4; ++lvl) {
+    s.setLevel(lvl);
+    for (const std::string &chr : s.chrs) {
+      printf("level %d %s %zu", lvl, chr.c_str(), s.current_level[chr].size());
+      for (int c : s.current_level[chr]) printf(" %d", c);
+      printf("\n");
+    }
+  }
+}
+
+static void mode_densify(LooperSolver &s) {
+  int ib = geti();
+  Settings::loopDensity = geti();
+  bool fix = geti() != 0;
+  s.active_region = s.clusters[ib].children;
+  for (int c : s.active_region) {
+    float x = getf(), y = getf(), z = getf();
+    s.clusters[c].pos.set(x, y, z);
+  }
+  s.densifyActiveRegion(ib, fix);
+  printf("\nactive %zu\n", s.active_region.size());
+  for (int c : s.active_region) {
+    Cluster &k = s.clusters[c];
+    printf("%d %d %d %d %d %d %a %a %a\n", c, k.start, k.end, k.genomic_pos, k.level,
+           (int)k.is_fixed, k.pos.x, k.pos.y, k.pos.z);
+  }
+  // distances to the next bead as the arc/smooth driver sets them
+  // (LooperSolver.hpp:4339-4354)
+  float noise = 0.0f;
+  printf("dist_to_next");
+  for (size_t i = 0; i + 1 < s.active_region.size(); ++i) {
+    int d = abs(s.clusters[s.active_region[i + 1]].genomic_pos -
+                s.clusters[s.active_region[i]].genomic_pos);
+    float dist = static_cast<float>(s.genomicLengthToDistance(d));
+    noise += dist;
+    printf(" %a", dist);
+  }
+  printf("\n");
+  noise /= s.active_region.size();
+  printf("noise %a %a\n", noise * Settings::noiseCoefficientLevelSubanchor,
+         noise * Settings::noiseCoefficientLevelAnchor);
+  printf("children %zu", s.clusters[ib].children.size());
+  for (int c : s.clusters[ib].children) printf(" %d", c);
+  printf("\n");
+}
+
+static void mode_expdist(LooperSolver &s) {
+  std::string chr = tok();
+  int ib = geti();
+  int n_long = geti();
+  for (int i = 0; i < n_long; ++i) {
+    int st = geti(), en = geti(), sc = geti();
+    s.arcs.long_arcs[chr].push_back(InteractionArc(st, en, sc));
+  }
+  s.current_chr = chr;
+  s.active_region = s.clusters[ib].children;
+  int n = (int)s.active_region.size();
+  Settings::useAnchorHeatmap = geti() != 0;
+  Settings::anchorHeatmapInfluence = getf();
+  if (Settings::useAnchorHeatmap) {
+    s.heatmap_anchor.init(n);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) s.heatmap_anchor.v[i][j] = getf();
+  }
+  fflush(stdout);
+  // the reference prints progress; keep the payload after a marker
+  s.calcAnchorExpectedDistancesHeatmap();
+  printf("\nexpdist %d\n", n);
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) printf("%a ", s.heatmap_exp_dist_anchor.v[i][j]);
+    printf("\n");
+  }
+}
+
+static void mode_distheat(LooperSolver &s) {
+  int n = geti(), diag = geti(), ind = geti();
+  s.heatmap.resize(std::max(ind + 1, 2));
+  s.heatmap[ind].init(n);
+  s.heatmap[ind].diagonal_size = diag;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) s.heatmap[ind].v[i][j] = getf();
+  s.createDistanceHeatmap(ind);
+  printf("\ndistheat %d %d %a\n", n, s.heatmap_dist.diagonal_size, s.heatmap_dist.getAvg());
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) printf("%a ", s.heatmap_dist.v[i][j]);
+    printf("\n");
+  }
+}
+
+// score: n, positions, then heatmap (diag + n*n), then arc expected
+// distances (n*n), then dist_to_next (n).
+static void mode_score(LooperSolver &s) {
+  int n = geti();
+  s.clusters.assign(n, Cluster());
+  s.active_region.clear();
+  for (int i = 0; i < n; ++i) {
+    float x = getf(), y = getf(), z = getf();
+    s.clusters[i].pos.set(x, y, z);
+    s.active_region.push_back(i);
+  }
+  s.heatmap_dist.init(n);
+  s.heatmap_dist.diagonal_size = geti();
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) s.heatmap_dist.v[i][j] = getf();
+  s.heatmap_exp_dist_anchor.init(n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) s.heatmap_exp_dist_anchor.v[i][j] = getf();
+  for (int i = 0; i < n; ++i) s.clusters[i].dist_to_next = getf();
+  printf("\nscore %.17g %.17g %.17g\n", s.calcScoreHeatmapActiveRegion(-1),
+         s.calcScoreDistancesActiveRegion(), s.calcScoreStructureSmooth(true, true));
+  printf("constants %a %a %a %a %a %a %a\n", Settings::springConstantStretchArcs,
+         Settings::springConstantSqueezeArcs, Settings::springConstantStretch,
+         Settings::springConstantSqueeze, Settings::springAngularConstant,
+         Settings::weightDistSmooth, Settings::weightAngleSmooth);
+}
+
+int main() {
+  Settings settings;
+  settings.init();
+  Settings::outputLevel = 0;
+  std::string mode = tok();
+  // the memo of freqToDistance is filled in the constructor, so the
+  // count-to-distance parameters come first (LooperSolver.hpp:950-951)
+  Settings::countToDistA = getf();
+  Settings::countToDistScale = getf();
+  Settings::countToDistShift = getf();
+  Settings::countToDistBaseLevel = getf();
+  Settings::freqToDistHeatmapScale = getf();
+  Settings::freqToDistHeatmapPower = getf();
+  Settings::freqToDistHeatmapScaleInter = getf();
+  Settings::freqToDistHeatmapPowerInter = getf();
+  Settings::heatmapDistanceHeatmapStretching = getf();
+  Settings::genomicLengthToDistBase = getf();
+  Settings::genomicLengthToDistScale = getf();
+  Settings::genomicLengthToDistPower = getf();
+  // the constructor insists on a segment-split file (LooperSolver.hpp:977-982);
+  // an empty one loads no region, and the regions come from stdin instead
+  Settings::dataSegmentsSplit = "/dev/null";
+  LooperSolver s("ref", "./");
+  if (mode == "tree") {
+    read_tree(s);
+    print_tree(s);
+  } else if (mode == "densify") {
+    read_tree(s);
+    mode_densify(s);
+  } else if (mode == "expdist") {
+    read_tree(s);
+    mode_expdist(s);
+  } else if (mode == "distheat") {
+    mode_distheat(s);
+  } else if (mode == "score") {
+    mode_score(s);
+  } else {
+    fprintf(stderr, "unknown mode %s\n", mode.c_str());
+    return 2;
+  }
+  return 0;
+}

@@ -1,4 +1,4 @@
-"""ctypes mirror of include/pmmc_hip.h (field order and types match the C
+"""ctypes mirror of include/pmmc_hip.h and include/pmmc_levels.h (field order and types match the C
 structs exactly) and the loader of libpmmc_hip.so.
 
 The library is the product: if it is missing this module raises, there is no
@@ -116,6 +116,36 @@ class ServerOpts(C.Structure):
 
 
 # name -> (restype, argtypes); every symbol declared in include/pmmc_hip.h
+class LevelParams(C.Structure):
+    _fields_ = [(n, C.c_float) for n in (
+        "count_a", "count_scale", "count_shift", "count_base", "heat_scale", "heat_power",
+        "heat_scale_inter", "heat_power_inter", "heat_stretching", "glen_base", "glen_scale",
+        "glen_power")] + [("loop_density", C.c_int32)] + [(n, C.c_float) for n in (
+        "noise_chr", "noise_segment", "noise_anchor", "noise_subanchor", "noise_anchor_small")] + [
+        (n, C.c_int32) for n in ("restarts_chr", "restarts_segment", "restarts_anchor",
+                                 "restarts_subanchor", "use_anchor_heatmap")] + [
+        ("anchor_heatmap_influence", C.c_float), ("use2d", C.c_int32)]
+
+
+class Chromosome(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("n_anchors", C.c_int32), ("anchor_start", i32p),
+                ("anchor_end", i32p), ("orientation", C.c_char_p), ("n_arcs", C.c_int32),
+                ("arc_start", i32p), ("arc_end", i32p), ("arc_score", i32p),
+                ("n_long_arcs", C.c_int32), ("long_start", i32p), ("long_end", i32p),
+                ("long_score", i32p)]
+
+
+class SegmentRegion(C.Structure):
+    _fields_ = [("chr", C.c_char_p), ("start", C.c_int32), ("end", C.c_int32)]
+
+
+class Cluster(C.Structure):
+    _fields_ = [("start", C.c_int32), ("end", C.c_int32), ("genomic_pos", C.c_int32),
+                ("base_start", C.c_int32), ("base_end", C.c_int32), ("level", C.c_int32),
+                ("parent", C.c_int32), ("n_children", C.c_int32), ("is_fixed", C.c_int32),
+                ("orientation", C.c_int32), ("pos", C.c_float * 3), ("dist_to_next", C.c_double)]
+
+
 SIGNATURES = {
     "pmmc_schedule_preset": (C.c_int, [C.c_int, C.c_int, C.POINTER(LevelSettings), C.c_float,
                                        C.POINTER(Schedule)]),
@@ -146,6 +176,31 @@ SIGNATURES = {
     "pmmc_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_uint64)]),
     "pmmc_server_step": (C.c_int, [C.c_void_p, i32p, i32p, i32p]),
     "pmmc_server_done": (C.c_int, [C.c_void_p, C.c_uint64]),
+    # include/pmmc_levels.h
+    "pmmc_level_params_default": (None, [C.POINTER(LevelParams)]),
+    "pmmc_host_last_error": (C.c_char_p, []),
+    "pmmc_freq_to_distance": (C.c_float, [C.c_int32, C.POINTER(LevelParams)]),
+    "pmmc_genomic_length_to_distance": (C.c_double, [C.c_int32, C.POINTER(LevelParams)]),
+    "pmmc_distance_heatmap": (C.c_int, [f32p, C.c_int32, C.c_int32, C.c_int32,
+                                        C.POINTER(LevelParams), f32p, f32p]),
+    "pmmc_tree_create": (C.c_int, [C.c_int32, C.POINTER(Chromosome), C.c_int32,
+                                   C.POINTER(SegmentRegion), C.POINTER(C.c_void_p)]),
+    "pmmc_tree_destroy": (None, [C.c_void_p]),
+    "pmmc_tree_size": (C.c_int32, [C.c_void_p]),
+    "pmmc_tree_n_chromosomes": (C.c_int32, [C.c_void_p]),
+    "pmmc_tree_get": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(Cluster)]),
+    "pmmc_tree_children": (C.c_int, [C.c_void_p, C.c_int32, i32p, C.c_int32, i32p]),
+    "pmmc_tree_chromosome": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p]),
+    "pmmc_tree_arcs": (C.c_int, [C.c_void_p, C.c_int32, i32p, C.c_int32, i32p]),
+    "pmmc_tree_level": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, i32p, C.c_int32, i32p]),
+    "pmmc_tree_set_positions": (C.c_int, [C.c_void_p, C.c_int32, i32p, f32p]),
+    "pmmc_tree_get_positions": (C.c_int, [C.c_void_p, C.c_int32, i32p, f32p]),
+    "pmmc_tree_set_fixed": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p]),
+    "pmmc_tree_densify": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, i32p]),
+    "pmmc_tree_set_dist_to_next": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(LevelParams),
+                                             f32p, f32p]),
+    "pmmc_tree_expected_distances": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32,
+                                               C.POINTER(LevelParams), f32p, f32p]),
 }
 
 _lib = None

@@ -5,7 +5,10 @@
 
 The library is built in place (pmmc_amd/libpmmc_hip.so) so that it travels
 with the source tree.  Every HIP translation unit is compiled with packed
-FP32 instructions disabled (-packed-fp32-ops).
+FP32 instructions disabled (-packed-fp32-ops).  The host-only rows
+(tree, densification, expected distances, level drivers) are plain C++
+built with g++ and -ffp-contract=off, so their float arithmetic rounds where
+the reference's host code does.
 """
 from __future__ import annotations
 
@@ -21,6 +24,10 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpmmc_hip.so")
 SOURCES = ["engine.hip", "api.hip", "markarcs.hip", "sweep.hip", "server.hip"]
+HOST_SOURCES = ["levels.cpp", "drivers.cpp"]
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wextra",
+              f"-I{os.path.join(ROOT, 'include')}"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_FLAGS = [
@@ -40,10 +47,12 @@ def _run(cmd):
 
 def build_lib(jobs: int = 8) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    objs = [os.path.join(BUILD, s.replace(".hip", ".o")) for s in srcs]
+    srcs = [s for s in SOURCES + HOST_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    objs = [os.path.join(BUILD, os.path.splitext(s)[0] + ".o") for s in srcs]
 
     def one(i):
+        if srcs[i].endswith(".cpp"):
+            return _run([CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, srcs[i]), "-o", objs[i]])
         return _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, srcs[i]), "-o", objs[i]])
 
     with cf.ThreadPoolExecutor(max_workers=min(jobs, len(srcs))) as ex:

@@ -1,0 +1,192 @@
+"""numpy front end of include/pmmc_levels.h: the cluster tree, densification,
+expected distances and the distance heatmap (host code of libpmmc_hip.so).
+
+    from pmmc_amd.tree import Tree, level_params
+    t = Tree([{"name": "chr1", "anchors": [(s, e), ...], "arcs": [(a, b, count), ...]}],
+             splits=[("chr1", 0, 2_000_000), ...])
+    ibs = t.level(0, 2)                  # interaction blocks of chromosome 0
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import _abi as A
+
+_i32p = A.i32p
+_f32p = A.f32p
+
+
+def _lib():
+    return A.load()
+
+
+def _check(st, what):
+    if st != 0:
+        msg = _lib().pmmc_host_last_error().decode()
+        raise RuntimeError(f"{what}: {A.STATUS.get(st, st)}: {msg}")
+
+
+def level_params(**overrides) -> A.LevelParams:
+    """Reference defaults (Settings.hpp), with keyword overrides."""
+    p = A.LevelParams()
+    _lib().pmmc_level_params_default(C.byref(p))
+    for k, v in overrides.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def freq_to_distance(count: int, p: A.LevelParams) -> np.float32:
+    return np.float32(_lib().pmmc_freq_to_distance(int(count), C.byref(p)))
+
+
+def genomic_length_to_distance(length: int, p: A.LevelParams) -> float:
+    return _lib().pmmc_genomic_length_to_distance(int(length), C.byref(p))
+
+
+def distance_heatmap(freq, diagonal_size: int, inter: bool, p: A.LevelParams):
+    """Returns (distance heatmap, average used for the clamp)."""
+    f = np.ascontiguousarray(freq, np.float32)
+    n = f.shape[0]
+    out = np.empty((n, n), np.float32)
+    avg = C.c_float()
+    _check(_lib().pmmc_distance_heatmap(f.ctypes.data_as(_f32p), n, int(diagonal_size), int(inter),
+                                        C.byref(p), out.ctypes.data_as(_f32p), C.byref(avg)),
+           "pmmc_distance_heatmap")
+    return out, np.float32(avg.value)
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a, np.int64).astype(np.int32))
+
+
+class Tree:
+    """Owns a pmmc_tree.  chrs: [{name, anchors [(start, end)], arcs
+    [(a, b, count)] (chromosome-local anchor indices), optional orient,
+    long_arcs [(start_bp, end_bp, count)]}]; splits: [(chr, start, end)]."""
+
+    def __init__(self, chrs: Sequence[dict], splits: Iterable = ()):
+        self._keep = []
+        cs = (A.Chromosome * max(1, len(chrs)))()
+        for k, c in enumerate(chrs):
+            anc = np.asarray(c["anchors"], np.int64).reshape(-1, 2)
+            arcs = np.asarray(c.get("arcs", []), np.int64).reshape(-1, 3)
+            longs = np.asarray(c.get("long_arcs", []), np.int64).reshape(-1, 3)
+            arrs = [_i32(anc[:, 0]), _i32(anc[:, 1]), _i32(arcs[:, 0]), _i32(arcs[:, 1]),
+                    _i32(arcs[:, 2]), _i32(longs[:, 0]), _i32(longs[:, 1]), _i32(longs[:, 2])]
+            name = c["name"].encode()
+            orient = "".join(c["orient"]).encode() if c.get("orient") else None
+            self._keep += arrs + [name, orient]
+            p = [a.ctypes.data_as(_i32p) for a in arrs]
+            cs[k] = A.Chromosome(name, len(anc), p[0], p[1], orient, len(arcs), p[2], p[3], p[4],
+                                 len(longs), p[5], p[6], p[7])
+        splits = list(splits)
+        rs = (A.SegmentRegion * max(1, len(splits)))()
+        for k, (chr_name, s, e) in enumerate(splits):
+            b = chr_name.encode()
+            self._keep.append(b)
+            rs[k] = A.SegmentRegion(b, int(s), int(e))
+        h = C.c_void_p()
+        _check(_lib().pmmc_tree_create(len(chrs), cs, len(splits), rs, C.byref(h)),
+               "pmmc_tree_create")
+        self.handle = h
+        self._keep = []
+
+    def close(self):
+        if self.handle:
+            _lib().pmmc_tree_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return _lib().pmmc_tree_size(self.handle)
+
+    @property
+    def n_chromosomes(self):
+        return _lib().pmmc_tree_n_chromosomes(self.handle)
+
+    def clusters(self, first=0, count=None):
+        count = len(self) - first if count is None else count
+        out = (A.Cluster * max(1, count))()
+        _check(_lib().pmmc_tree_get(self.handle, first, count, out), "pmmc_tree_get")
+        return [out[i] for i in range(count)]
+
+    def _list(self, fn, *args):
+        n = C.c_int32()
+        _check(fn(self.handle, *args, None, 0, C.byref(n)), fn.__name__)
+        out = np.empty(max(1, n.value), np.int32)
+        _check(fn(self.handle, *args, out.ctypes.data_as(_i32p), n.value, C.byref(n)), fn.__name__)
+        return out[:n.value].tolist()
+
+    def children(self, cluster):
+        return self._list(_lib().pmmc_tree_children, int(cluster))
+
+    def level(self, chr_index, level):
+        return self._list(_lib().pmmc_tree_level, int(chr_index), int(level))
+
+    def chromosome(self, chr_index):
+        f, r = C.c_int32(), C.c_int32()
+        _check(_lib().pmmc_tree_chromosome(self.handle, int(chr_index), C.byref(f), C.byref(r)),
+               "pmmc_tree_chromosome")
+        return f.value, r.value
+
+    def arcs(self, chr_index):
+        n = C.c_int32()
+        _check(_lib().pmmc_tree_arcs(self.handle, int(chr_index), None, 0, C.byref(n)), "pmmc_tree_arcs")
+        out = np.empty(max(1, 2 * n.value), np.int32)
+        _check(_lib().pmmc_tree_arcs(self.handle, int(chr_index), out.ctypes.data_as(_i32p), n.value,
+                                     C.byref(n)), "pmmc_tree_arcs")
+        return out[:2 * n.value].reshape(-1, 2).tolist()
+
+    def set_positions(self, ids, xyz):
+        ids = _i32(ids)
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        _check(_lib().pmmc_tree_set_positions(self.handle, len(ids), ids.ctypes.data_as(_i32p),
+                                              xyz.ctypes.data_as(_f32p)), "pmmc_tree_set_positions")
+
+    def positions(self, ids):
+        ids = _i32(ids)
+        out = np.empty((len(ids), 3), np.float32)
+        _check(_lib().pmmc_tree_get_positions(self.handle, len(ids), ids.ctypes.data_as(_i32p),
+                                              out.ctypes.data_as(_f32p)), "pmmc_tree_get_positions")
+        return out
+
+    def set_fixed(self, ids, fixed):
+        ids, fx = _i32(ids), _i32(fixed)
+        _check(_lib().pmmc_tree_set_fixed(self.handle, len(ids), ids.ctypes.data_as(_i32p),
+                                          fx.ctypes.data_as(_i32p)), "pmmc_tree_set_fixed")
+
+    def densify(self, cluster, loop_density, fix=True):
+        n = C.c_int32()
+        _check(_lib().pmmc_tree_densify(self.handle, int(cluster), int(loop_density), int(bool(fix)),
+                                        C.byref(n)), "pmmc_tree_densify")
+        return n.value
+
+    def set_dist_to_next(self, cluster, p):
+        a, s = C.c_float(), C.c_float()
+        _check(_lib().pmmc_tree_set_dist_to_next(self.handle, int(cluster), C.byref(p), C.byref(a),
+                                                 C.byref(s)), "pmmc_tree_set_dist_to_next")
+        return np.float32(a.value), np.float32(s.value)
+
+    def expected_distances(self, chr_index, cluster, p, anchor_heatmap=None):
+        n = len(self.children(cluster))
+        out = np.empty((n, n), np.float32)
+        h = None
+        if anchor_heatmap is not None:
+            h = np.ascontiguousarray(anchor_heatmap, np.float32)
+            if h.shape != (n, n):
+                raise ValueError(f"anchor heatmap must be {n}x{n}")
+        _check(_lib().pmmc_tree_expected_distances(
+            self.handle, int(chr_index), int(cluster), C.byref(p),
+            h.ctypes.data_as(_f32p) if h is not None else None, out.ctypes.data_as(_f32p)),
+            "pmmc_tree_expected_distances")
+        return out

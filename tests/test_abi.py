@@ -1,7 +1,8 @@
 """C ABI checks that need no GPU: the library loads, exports every function
-declared in include/pmmc_hip.h, the ctypes mirrors have the C layouts, and
+declared in include/*.h, the ctypes mirrors have the C layouts, and
 the schedule presets reproduce the reference annealers' constants."""
 import ctypes as C
+import glob
 import os
 import re
 import subprocess
@@ -13,11 +14,11 @@ from pmmc_amd import _abi as A
 from pmmc_amd.api import schedule_preset, settings
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "pmmc_hip.h")
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 
 
 def declared_functions():
-    txt = open(HEADER).read()
+    txt = "".join(open(h).read() for h in HEADERS)
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(pmmc_[a-z0-9_]+)\s*\(", txt)))
 
@@ -37,11 +38,14 @@ STRUCTS = {
     "pmmc_smooth_problem": A.SmoothProblem, "pmmc_run": A.Run, "pmmc_anneal_stats": A.Stats,
     "pmmc_job": A.Job, "pmmc_batch_opts": A.BatchOpts, "pmmc_sweep_schedule": A.SweepSchedule,
     "pmmc_arc": A.Arc, "pmmc_server_opts": A.ServerOpts,
+    "pmmc_level_params": A.LevelParams, "pmmc_chromosome": A.Chromosome,
+    "pmmc_segment_region": A.SegmentRegion, "pmmc_cluster": A.Cluster,
 }
 
 
 def test_ctypes_layouts_match_header():
-    prog = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
+    prog = ['#include <stdio.h>', '#include <stddef.h>'] + [f'#include "{h}"' for h in HEADERS]
+    prog.append("int main(void){")
     for cname, py in STRUCTS.items():
         prog.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
         for f, _ in py._fields_:
@@ -50,7 +54,8 @@ def test_ctypes_layouts_match_header():
     with tempfile.TemporaryDirectory() as d:
         src, exe = os.path.join(d, "t.c"), os.path.join(d, "t")
         open(src, "w").write("\n".join(prog))
-        subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", src, "-o", exe], check=True)
+        subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", f"-I{os.path.join(ROOT, 'include')}",
+                        src, "-o", exe], check=True)
         out = subprocess.run([exe], stdout=subprocess.PIPE, text=True, check=True).stdout
     got = dict(line.rsplit(" ", 1) for line in out.strip().splitlines())
     for cname, py in STRUCTS.items():
