@@ -194,6 +194,85 @@ pmmc_status pmmc_tree_expected_distances(const pmmc_tree *t, int32_t chr, int32_
                                          const pmmc_level_params *p,
                                          const float *anchor_heatmap, float *out);
 
+/* ---- level drivers ------------------------------------------------------- */
+/*
+ * The reference runs every restart of every interaction block as its own
+ * host loop iteration with its own GPU launch (LooperSolver.hpp:1334-1398,
+ * :4027-4138, :4408-4463).  Here all restarts of all blocks of a stage are
+ * one batched launch (pmmc_anneal_batch): restart k of block b is job
+ * (b, k), whose chains start from the block's initial structure plus that
+ * restart's noise; the best restart of each block (lowest score, first on
+ * ties, as `score < best_score || best_score < 0`) is written back into the
+ * tree.  Random streams: restart noise for bead i of job j is
+ * (2u-1)*amplitude per axis from Philox(seed, {i, 0, segment + j, 5<<28})
+ * (z = 0 under use2d); the annealing chains of job j run with segment
+ * `segment + j`, chain ids from 0.
+ */
+
+typedef struct {
+  int32_t n_blocks;          /* blocks (or regions) annealed                   */
+  int32_t n_jobs;            /* jobs launched over all stages                  */
+  int64_t moves;             /* counted moves over all chains                  */
+  int64_t accepted;
+  double score_anchor;       /* sum over blocks of the best restart's score:   */
+  double score_smooth;       /*   anchor / sub-anchor stage (heatmap: score_anchor) */
+  double ms_anchor;          /* host wall time of each batched stage          */
+  double ms_smooth;
+} pmmc_level_report;
+
+typedef struct {
+  pmmc_level_settings arcs;   /* [simulation_arcs] + [cuda] settings           */
+  pmmc_level_settings smooth; /* [simulation_arcs_smooth] settings             */
+  int32_t arcs_preset;        /* pmmc_preset of the anchor stage               */
+  int32_t smooth_preset;      /* of the sub-anchor stage; blocks of fewer than
+                                 4 beads always take the CPU annealer, as the
+                                 reference dispatches (LooperSolver.hpp:4431-4443) */
+  float k_stretch_arcs, k_squeeze_arcs;   /* springConstantStretchArcs / SqueezeArcs */
+  float k_stretch, k_squeeze, k_angular;  /* [springs]                       */
+  float w_dist, w_angle;                  /* weightDistSmooth / weightAngleSmooth */
+  int32_t chains_per_restart; /* chains of each job; 0 = share the library
+                                 default chain pool among the jobs             */
+  int32_t densify;            /* 0: stop after the anchor stage                */
+} pmmc_arcs_level_opts;
+
+void pmmc_arcs_level_opts_default(pmmc_arcs_level_opts *o);
+
+/* Arc and sub-anchor levels over the interaction blocks of chromosome `chr`
+ * (-1: every chromosome): reconstructClustersArcsDistances' block loop
+ * (LooperSolver.hpp:3978-4140) over reconstructClusterArcsDistances
+ * (:4303-4487).  Per block: children start at the block's position, expected
+ * distances as pmmc_tree_expected_distances, restarts of the arc annealer
+ * with step = mean dist_to_next * noise_anchor and restart noise
+ * noise_anchor_small on every bead; then densification (fixing the anchors)
+ * and restarts of the smooth annealer with step = restart noise = mean
+ * dist_to_next * noise_subanchor on the non-fixed beads.  The interaction
+ * blocks' own positions are the caller's (positionInteractionBlocks). */
+pmmc_status pmmc_arcs_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t chr,
+                            const pmmc_level_params *p, const pmmc_arcs_level_opts *o,
+                            uint64_t seed, uint32_t segment, pmmc_level_report *report);
+
+typedef struct {
+  pmmc_level_settings heatmap;  /* [simulation_heatmap] + [cuda] settings       */
+  int32_t preset;               /* pmmc_preset                                  */
+  int32_t chains_per_restart;   /* 0 = share the library default chain pool     */
+} pmmc_heatmap_level_opts;
+
+void pmmc_heatmap_level_opts_default(pmmc_heatmap_level_opts *o);
+
+/* Heatmap level over tree level `level` (0: chromosome roots, 1: segments):
+ * reconstructClustersHeatmapSingleLevel (LooperSolver.hpp:1258-1422).  The
+ * active region is the concatenation of every chromosome's level list; dist
+ * is its n x n distance heatmap (pmmc_distance_heatmap) with band
+ * `diagonal`.  Noise amplitude = annealing step = getAvg(dist) * noise_chr
+ * (level 0) or * noise_segment (level 1, where every segment first moves to
+ * its chromosome's position and chromosome boundaries split the heatmap when
+ * there are several chromosomes, :1221-1229).  Restarts restarts_chr /
+ * restarts_segment; the best restart's structure is written back. */
+pmmc_status pmmc_heatmap_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t level, const float *dist,
+                               int32_t diagonal, const pmmc_level_params *p,
+                               const pmmc_heatmap_level_opts *o, uint64_t seed, uint32_t segment,
+                               pmmc_level_report *report);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,7 +51,7 @@ def new_cluster(start, end):
     """Cluster::Cluster(int, int) (Cluster.hpp:115-136)."""
     return dict(start=start, end=end, genomic_pos=c_div(start + end, 2), orientation="N",
                 base_start=0, base_end=0, parent=-1, is_fixed=False, level=0,
-                children=[], arcs=[], pos=[F(0), F(0), F(0)])
+                children=[], arcs=[], pos=[F(0), F(0), F(0)], dist_to_next=0.0)
 
 
 # ------------------------------------------------------------------ T1
@@ -125,7 +125,8 @@ def build_tree(chrs, splits):
     [(a, b, score)] with chromosome-local anchor indices}]; splits: predefined
     segments [(chr, start, end)] in file order."""
     t = dict(clusters=[], arcs={}, scores={}, chr_first={}, chr_root={}, chrs=[c["name"] for c in chrs],
-             splits=[tuple(s) for s in splits], anchors={})
+             splits=[tuple(s) for s in splits], anchors={},
+             long_arcs={c["name"]: [tuple(x) for x in c.get("long_arcs", [])] for c in chrs})
     cl = t["clusters"]
     for c in chrs:
         name = c["name"]
@@ -320,6 +321,8 @@ def expected_distances(t, chr_name, ib, long_arcs, p, anchor_heatmap=None, influ
     active_region = children of ib and current_chr = chr_name.  An arc whose
     other end lies outside the active region lands in slot 0, as the
     reference's std::map::operator[] default does (:5538-5539)."""
+    if long_arcs is None:        # the chromosome's long arcs stored in the tree
+        long_arcs = t["long_arcs"].get(chr_name, [])
     cl = t["clusters"]
     active = cl[ib]["children"]
     n = len(active)
@@ -414,3 +417,157 @@ def distance_heatmap(freq, diag, inter, p):
     mx = F(avg * F(p["stretching"]))
     h[h > mx] = mx
     return h, avg
+
+
+# ------------------------------------------------------------------ H7 / A7 drivers
+LEVEL_INIT = 5
+
+
+def restart_noise(seed, segment, bead, amp, use2d):
+    """random_vector(amp, use2d) (common.cpp:14-25) on the Philox stream the
+    library uses for restart noise: (2u-1)*amp per axis from
+    Philox(seed, {bead, 0, segment, 5<<28})."""
+    from oracle import binding as O
+    o = O.philox([bead, 0, segment, LEVEL_INIT << 28], [seed & 0xFFFFFFFF, seed >> 32])
+    out = [F(F(F(2.0) * F(F(int(x) >> 8) * F(2.0 ** -24)) - F(1.0)) * F(amp)) for x in o[:3]]
+    if use2d:
+        out[2] = F(0.0)
+    return out
+
+
+def _offer(best, b, score, payload):
+    """best-of rule score < best || best < 0 (LooperSolver.hpp:1392-1398, :4457-4463)"""
+    cur = best.get(b)
+    if cur is None or score < cur[0] or cur[0] < 0.0:
+        best[b] = (score, payload)
+
+
+def arcs_level(t, chr_name, p, o, seed, segment, run_job):
+    """reconstructClustersArcsDistances' block loop (LooperSolver.hpp:3978-4140)
+    over reconstructClusterArcsDistances (:4303-4487), in the library's
+    batched order: every block's anchor restarts, then densification, then
+    every block's sub-anchor restarts.  p: level parameters (dict, the keys of
+    pmmc_level_params), o: dict with arcs_sched(step) / smooth_sched(step,
+    n_beads) schedule factories, springs, chains and densify.  run_job(level,
+    problem dict, schedule, seed, segment, n_chains) returns
+    dict(pos, energy, resummed) of the job's best chain.  Returns the report
+    dict and records candidates per block in t["_candidates"]."""
+    cl = t["clusters"]
+    blocks = []
+    for name in t["chrs"]:
+        if chr_name is not None and name != chr_name:
+            continue
+        for ib in set_level(t, 2)[name]:
+            if cl[ib]["level"] == 3:
+                blocks.append((name, ib))
+    rep = dict(n_blocks=len(blocks), n_jobs=0, score_anchor=0.0, score_smooth=0.0)
+    next_job = 0
+    best = {}
+    for bi, (name, ib) in enumerate(blocks):
+        active = list(cl[ib]["children"])
+        for c in active:
+            cl[c]["pos"] = list(cl[ib]["pos"])
+        if len(active) <= 1:
+            continue
+        ed = expected_distances(t, name, ib, None, p)
+        dtn, _, noise = dist_to_next([cl[c]["genomic_pos"] for c in active], p,
+                                     p["noise_subanchor"], p["noise_anchor"])
+        for i, c in enumerate(active[:-1]):
+            cl[c]["dist_to_next"] = float(dtn[i])
+        sched = o["arcs_sched"](noise)
+        for _ in range(p["restarts_anchor"]):
+            seg = segment + next_job
+            next_job += 1
+            pos = np.array([[F(cl[c]["pos"][a] + restart_noise(seed, seg, i, p["noise_anchor_small"],
+                                                               p["use2d"])[a]) for a in range(3)]
+                            for i, c in enumerate(active)], np.float32)
+            prob = dict(pos=pos, fixed=np.array([cl[c]["is_fixed"] for c in active], np.uint8),
+                        exp_dist=ed, k_stretch=o["k_stretch_arcs"], k_squeeze=o["k_squeeze_arcs"])
+            r = run_job(2, prob, sched, seed, seg, o["chains"])
+            rep["n_jobs"] += 1
+            _offer(best, bi, r["energy"], (active, r))
+    cand = {}
+    for bi, (score, (active, r)) in best.items():
+        for i, c in enumerate(active):
+            cl[c]["pos"] = [F(x) for x in r["pos"][i]]
+        rep["score_anchor"] += score
+    if not o.get("densify", True):
+        return rep
+    for name, ib in blocks:
+        densify(t, ib, p["loop_density"], True)
+    best = {}
+    for bi, (name, ib) in enumerate(blocks):
+        active = list(cl[ib]["children"])
+        if len(active) <= 1:
+            continue
+        dtn, noise, _ = dist_to_next([cl[c]["genomic_pos"] for c in active], p,
+                                     p["noise_subanchor"], p["noise_anchor"])
+        for i, c in enumerate(active[:-1]):
+            cl[c]["dist_to_next"] = float(dtn[i])
+        sched = o["smooth_sched"](noise, len(active))
+        for _ in range(p["restarts_subanchor"]):
+            seg = segment + next_job
+            next_job += 1
+            pos = []
+            for i, c in enumerate(active):
+                d = [F(0), F(0), F(0)] if cl[c]["is_fixed"] else restart_noise(seed, seg, i, noise, p["use2d"])
+                pos.append([F(cl[c]["pos"][a] + d[a]) for a in range(3)])
+            prob = dict(pos=np.array(pos, np.float32),
+                        fixed=np.array([cl[c]["is_fixed"] for c in active], np.uint8),
+                        dist_to_next=np.array([cl[c]["dist_to_next"] for c in active], np.float32),
+                        k_stretch=o["k_stretch"], k_squeeze=o["k_squeeze"], k_angular=o["k_angular"],
+                        w_dist=o["w_dist"], w_angle=o["w_angle"])
+            r = run_job(3, prob, sched, seed, seg, o["chains"])
+            rep["n_jobs"] += 1
+            cand.setdefault(bi, []).append((r["resummed"], active, r))
+            _offer(best, bi, r["resummed"], (active, r))
+    for bi, (score, (active, r)) in best.items():
+        for i, c in enumerate(active):
+            if not cl[c]["is_fixed"]:
+                cl[c]["pos"] = [F(x) for x in r["pos"][i]]
+        rep["score_smooth"] += score
+    t["_candidates"] = cand
+    return rep
+
+
+def heatmap_level(t, level, dist, diag, p, o, seed, segment, run_job):
+    """reconstructClustersHeatmapSingleLevel (LooperSolver.hpp:1258-1422):
+    restarts of the heatmap annealer over the concatenated level lists, with
+    the library's restart noise; o: dict with sched(step), chains,
+    skip_below."""
+    cl = t["clusters"]
+    cur = set_level(t, level)
+    active = [c for name in t["chrs"] for c in cur[name]]
+    n = len(active)
+    avg = heat_avg(np.asarray(dist, np.float32))
+    avg_dist = F(avg * F(p["noise_chr"] if level == 0 else p["noise_segment"]))
+    bnd = None
+    if level == 1:
+        for name in t["chrs"]:
+            par = cl[cur[name][0]]["parent"]
+            for s in cur[name]:
+                cl[s]["pos"] = list(cl[par]["pos"])
+        if len(t["chrs"]) > 1:
+            bnd = [0]
+            for name in t["chrs"]:
+                bnd.append(bnd[-1] + len(cur[name]))
+    restarts = p["restarts_chr"] if level == 0 else p["restarts_segment"]
+    sched = o["sched"](avg_dist)
+    best = {}
+    results = []
+    for k in range(restarts):
+        seg = segment + k
+        pos = np.array([[F(cl[c]["pos"][a] + restart_noise(seed, seg, i, avg_dist, p["use2d"])[a])
+                         for a in range(3)] for i, c in enumerate(active)], np.float32)
+        prob = dict(pos=pos, fixed=np.array([cl[c]["is_fixed"] for c in active], np.uint8),
+                    dist=np.asarray(dist, np.float32), diagonal=diag,
+                    boundaries=None if bnd is None else np.array(bnd, np.int32),
+                    skip_below=o["skip_below"])
+        r = run_job(1, prob, sched, seed, seg, o["chains"])
+        results.append((r["resummed"], r))
+        _offer(best, 0, r["resummed"], r)
+    score, r = best[0]
+    for i, c in enumerate(active):
+        cl[c]["pos"] = [F(x) for x in r["pos"][i]]
+    t["_candidates"] = {0: [(s, active, rr) for s, rr in results]}
+    return dict(n_blocks=1, n_jobs=restarts, score_anchor=score, avg_dist=avg_dist)

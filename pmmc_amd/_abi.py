@@ -146,6 +146,24 @@ class Cluster(C.Structure):
                 ("orientation", C.c_int32), ("pos", C.c_float * 3), ("dist_to_next", C.c_double)]
 
 
+class LevelReport(C.Structure):
+    _fields_ = [("n_blocks", C.c_int32), ("n_jobs", C.c_int32), ("moves", C.c_int64),
+                ("accepted", C.c_int64), ("score_anchor", C.c_double), ("score_smooth", C.c_double),
+                ("ms_anchor", C.c_double), ("ms_smooth", C.c_double)]
+
+
+class ArcsLevelOpts(C.Structure):
+    _fields_ = [("arcs", LevelSettings), ("smooth", LevelSettings), ("arcs_preset", C.c_int32),
+                ("smooth_preset", C.c_int32), ("k_stretch_arcs", C.c_float),
+                ("k_squeeze_arcs", C.c_float), ("k_stretch", C.c_float), ("k_squeeze", C.c_float),
+                ("k_angular", C.c_float), ("w_dist", C.c_float), ("w_angle", C.c_float),
+                ("chains_per_restart", C.c_int32), ("densify", C.c_int32)]
+
+
+class HeatmapLevelOpts(C.Structure):
+    _fields_ = [("heatmap", LevelSettings), ("preset", C.c_int32), ("chains_per_restart", C.c_int32)]
+
+
 SIGNATURES = {
     "pmmc_schedule_preset": (C.c_int, [C.c_int, C.c_int, C.POINTER(LevelSettings), C.c_float,
                                        C.POINTER(Schedule)]),
@@ -201,6 +219,14 @@ SIGNATURES = {
                                              f32p, f32p]),
     "pmmc_tree_expected_distances": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32,
                                                C.POINTER(LevelParams), f32p, f32p]),
+    "pmmc_arcs_level_opts_default": (None, [C.POINTER(ArcsLevelOpts)]),
+    "pmmc_heatmap_level_opts_default": (None, [C.POINTER(HeatmapLevelOpts)]),
+    "pmmc_arcs_level": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(LevelParams),
+                                  C.POINTER(ArcsLevelOpts), C.c_uint64, C.c_uint32,
+                                  C.POINTER(LevelReport)]),
+    "pmmc_heatmap_level": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, f32p, C.c_int32,
+                                     C.POINTER(LevelParams), C.POINTER(HeatmapLevelOpts), C.c_uint64,
+                                     C.c_uint32, C.POINTER(LevelReport)]),
 }
 
 _lib = None

@@ -152,7 +152,8 @@ class Context:
 
     def _check(self, st):
         if st != 0:
-            raise PmmcError(st, self.lib.pmmc_last_error(self.h).decode())
+            msg = self.lib.pmmc_last_error(self.h).decode() or self.lib.pmmc_host_last_error().decode()
+            raise PmmcError(st, msg)
 
     @property
     def cu_count(self) -> int:

@@ -190,3 +190,49 @@ class Tree:
             h.ctypes.data_as(_f32p) if h is not None else None, out.ctypes.data_as(_f32p)),
             "pmmc_tree_expected_distances")
         return out
+
+
+# ------------------------------------------------------------------ level drivers
+def arcs_level_opts(**overrides) -> A.ArcsLevelOpts:
+    """Reference defaults (Settings.hpp), with keyword overrides; `arcs` and
+    `smooth` take pmmc_amd.api.settings(...) objects."""
+    o = A.ArcsLevelOpts()
+    _lib().pmmc_arcs_level_opts_default(C.byref(o))
+    for k, v in overrides.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+def heatmap_level_opts(**overrides) -> A.HeatmapLevelOpts:
+    o = A.HeatmapLevelOpts()
+    _lib().pmmc_heatmap_level_opts_default(C.byref(o))
+    for k, v in overrides.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+def _report(r: A.LevelReport) -> dict:
+    return {k: getattr(r, k) for k, _ in A.LevelReport._fields_}
+
+
+def arcs_level(ctx, tree: Tree, chr_index: int, p: A.LevelParams, o: A.ArcsLevelOpts,
+               seed: int = 1, segment: int = 0) -> dict:
+    """pmmc_arcs_level on a pmmc_amd.api.Context; positions land in the tree."""
+    rep = A.LevelReport()
+    ctx._check(_lib().pmmc_arcs_level(ctx.h, tree.handle, int(chr_index), C.byref(p), C.byref(o),
+                                      int(seed), int(segment), C.byref(rep)))
+    return _report(rep)
+
+
+def heatmap_level(ctx, tree: Tree, level: int, dist, diagonal: int, p: A.LevelParams,
+                  o: A.HeatmapLevelOpts, seed: int = 1, segment: int = 0) -> dict:
+    d = np.ascontiguousarray(dist, np.float32)
+    rep = A.LevelReport()
+    ctx._check(_lib().pmmc_heatmap_level(ctx.h, tree.handle, int(level), d.ctypes.data_as(_f32p),
+                                         int(diagonal), C.byref(p), C.byref(o), int(seed),
+                                         int(segment), C.byref(rep)))
+    return _report(rep)

@@ -40,6 +40,8 @@ STRUCTS = {
     "pmmc_arc": A.Arc, "pmmc_server_opts": A.ServerOpts,
     "pmmc_level_params": A.LevelParams, "pmmc_chromosome": A.Chromosome,
     "pmmc_segment_region": A.SegmentRegion, "pmmc_cluster": A.Cluster,
+    "pmmc_level_report": A.LevelReport, "pmmc_arcs_level_opts": A.ArcsLevelOpts,
+    "pmmc_heatmap_level_opts": A.HeatmapLevelOpts,
 }
 
 

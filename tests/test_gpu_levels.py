@@ -1,0 +1,185 @@
+"""Level drivers on the GPU (pmmc_arcs_level, pmmc_heatmap_level) against the
+oracle's restatement of the reference's driver loops (oracle/levels.py:
+arcs_level, heatmap_level) whose annealing jobs run on the oracle's chains.
+
+Bar: the structure the library writes back into the tree is bit-identical to
+the oracle's.  Every chain's trajectory is bit-exact; the best chain of a job
+is chosen on its FP64 running total, which the GPU accumulates from FP32 move
+deltas, so where the oracle's ranking of two chains is closer than 1e-5 of
+the job's energy scale (|initial energy| + |best energy|) the library may
+pick either, and the test accepts any chain within that gap.
+"""
+import numpy as np
+import pytest
+
+from oracle import binding as O
+from oracle import levels as L
+from pmmc_amd import api
+from pmmc_amd import tree as T
+
+pytestmark = pytest.mark.gpu
+
+TIE = 1e-5
+
+
+def genome(seed, n_chr=2, n_anchors=(14, 18), span=4, splits_per_chr=2):
+    rng = np.random.default_rng(seed)
+    chrs, splits = [], []
+    for c in range(n_chr):
+        n = int(n_anchors[c % len(n_anchors)])
+        gaps = rng.integers(4000, 30000, size=n)
+        lens = rng.integers(500, 3000, size=n)
+        starts = 10_000 + np.cumsum(gaps + lens) - lens
+        anchors = [(int(s), int(s + l)) for s, l in zip(starts, lens)]
+        pairs = set()
+        for _ in range(n):
+            a = int(rng.integers(0, n - 1))
+            b = int(min(n - 1, a + rng.integers(1, span + 1)))
+            pairs.add((a, b))
+        arcs = [(a, b, int(rng.integers(2, 80))) for a, b in sorted(pairs)]
+        name = f"chr{c + 1}"
+        chrs.append(dict(name=name, anchors=anchors, arcs=arcs))
+        lo, hi = anchors[0][0] - 1000, anchors[-1][1] + 1000
+        cuts = sorted(int(x) for x in rng.integers(lo + 1, hi, size=splits_per_chr - 1))
+        edges = [lo] + cuts + [hi]
+        splits += [(name, edges[i], edges[i + 1]) for i in range(splits_per_chr)]
+    return dict(chrs=chrs, splits=splits), rng
+
+
+PARAMS = dict(count_a=0.5, count_scale=20.0, count_shift=1.0, count_base=0.01, heat_scale=100.0,
+              heat_power=-0.333, heat_scale_inter=100.0, heat_power_inter=-1.0, stretching=2.0,
+              glen_base=0.0, glen_scale=1.0, glen_power=0.5, loop_density=2, noise_chr=1.0,
+              noise_segment=0.1, noise_anchor=0.5, noise_subanchor=0.5, noise_anchor_small=0.05,
+              restarts_chr=2, restarts_segment=3, restarts_anchor=2, restarts_subanchor=2, use2d=0)
+RENAME = {"stretching": "heat_stretching"}
+
+
+def product_params(d):
+    return T.level_params(**{RENAME.get(k, k): v for k, v in d.items()})
+
+
+def run_job_oracle(level, prob, sched, seed, seg, n_chains):
+    ap = {1: api.Heatmap, 2: api.Arcs, 3: api.Smooth}[level](**prob)
+    P = O.Problem(ap)
+    out, stats, _ = P.anneal_many(O.schedule_from(sched), seed, seg, 0, n_chains, 8)
+    E = np.array([s["energy"] for s in stats])
+    b = int(np.lexsort((np.arange(n_chains), E))[0])
+    r = sched.sphere_radius
+    res = dict(pos=out[b], energy=float(E[b]), resummed=P.total(out[b], r), E0=P.total(None, r))
+    # every chain, for the near-tie check: (running total, job score, positions)
+    res["chains"] = [(float(E[c]), float(E[c]) if level == 2 else P.total(out[c], r), out[c])
+                     for c in range(n_chains)]
+    return res
+
+
+def ib_positions(pt, ot, rng):
+    """Stand-in for positionInteractionBlocks: the same random walk on both."""
+    for ci, name in enumerate(ot["chrs"]):
+        ibs = pt.level(ci, 2)
+        steps = rng.normal(0, 1.0, size=(len(ibs), 3)).astype(np.float32)
+        pos = np.cumsum(steps, axis=0).astype(np.float32)
+        pt.set_positions(ibs, pos)
+        for k, c in enumerate(ibs):
+            ot["clusters"][c]["pos"] = [np.float32(x) for x in pos[k]]
+
+
+def arcs_opts(steps, chains):
+    s = api.settings(max_temp=5.0, dt_temp=0.9995, improvement=0.99, tjs=50.0, tjc=20.0,
+                     min_successes=5, steps=steps, milestone_fails=3)
+    po = T.arcs_level_opts(arcs=s, smooth=s, chains_per_restart=chains)
+    oo = dict(arcs_sched=lambda step: api.schedule_preset(2, 1, s, step),
+              smooth_sched=lambda step, n: api.schedule_preset(3, 1 if n >= 4 else 0, s, step),
+              k_stretch_arcs=1.0, k_squeeze_arcs=1.0, k_stretch=0.1, k_squeeze=0.1,
+              k_angular=0.1, w_dist=1.0, w_angle=1.0, chains=chains)
+    return po, oo
+
+
+def assert_tree_positions_match(pt, ot):
+    n = len(pt)
+    got = pt.positions(list(range(n)))
+    want = np.array([[float(x) for x in ot["clusters"][i]["pos"]] for i in range(n)], np.float32)
+    same = np.all(got.view(np.uint32) == want.view(np.uint32), axis=1)
+    if same.all():
+        return 0.0
+    # Near ties: a chain whose running total is within TIE of its job's best
+    # (relative to the job's energy scale) may be the one returned, and a
+    # restart whose score is within TIE of the best restart may win.
+    bad = set(np.nonzero(~same)[0].tolist())
+    slack = 0.0
+    for bi, cands in ot.get("_candidates", {}).items():
+        active = cands[0][1]
+        if not set(active) & bad:
+            continue
+        tied = []
+        for _, _, r in cands:
+            tol = TIE * (abs(r["E0"]) + abs(r["energy"]))
+            tied += [(score, pos, tol) for e, score, pos in r["chains"] if e - r["energy"] <= tol]
+        best = min(t[0] for t in tied)
+        g = got[active].view(np.uint32)
+        if any(score - best <= tol and np.array_equal(g, np.asarray(pos, np.float32).view(np.uint32))
+               for score, pos, tol in tied):
+            bad -= set(active)
+            slack += max(t[2] for t in tied)
+    assert not bad, f"clusters differ from the oracle: {sorted(bad)[:10]}"
+    return slack     # how far a tied choice may move the summed score
+
+
+@pytest.mark.parametrize("seed,loop_density", [(3, 2), (11, 1)])
+def test_arcs_level_matches_oracle(ctx, seed, loop_density):
+    case, rng = genome(seed)
+    pt = T.Tree(case["chrs"], case["splits"])
+    ot = L.build_tree(case["chrs"], case["splits"])
+    ib_positions(pt, ot, rng)
+    pd = dict(PARAMS, loop_density=loop_density)
+    po, oo = arcs_opts(steps=1536, chains=64)
+    print(f"arcs_level seed={seed} loop_density={loop_density} clusters={len(pt)}", flush=True)
+    rep = T.arcs_level(ctx, pt, -1, product_params(pd), po, seed=77, segment=1000)
+    orep = L.arcs_level(ot, None, pd, oo, 77, 1000, run_job_oracle)
+    assert len(pt) == len(ot["clusters"])             # same densification
+    assert rep["n_blocks"] == orep["n_blocks"] and rep["n_jobs"] == orep["n_jobs"]
+    slack = assert_tree_positions_match(pt, ot)
+    assert rep["score_anchor"] == pytest.approx(orep["score_anchor"], rel=1e-5)
+    assert rep["score_smooth"] == pytest.approx(orep["score_smooth"], rel=1e-6, abs=slack)
+    assert rep["moves"] > 0
+
+
+def test_arcs_level_single_chromosome_anchor_stage_only(ctx):
+    case, rng = genome(5, n_chr=2)
+    pt = T.Tree(case["chrs"], case["splits"])
+    ot = L.build_tree(case["chrs"], case["splits"])
+    ib_positions(pt, ot, rng)
+    po, oo = arcs_opts(steps=1024, chains=128)
+    po.densify = 0
+    oo["densify"] = False
+    rep = T.arcs_level(ctx, pt, 1, product_params(PARAMS), po, seed=5, segment=0)
+    orep = L.arcs_level(ot, "chr2", PARAMS, oo, 5, 0, run_job_oracle)
+    assert rep["n_blocks"] == orep["n_blocks"] and len(pt) == len(ot["clusters"])
+    assert_tree_positions_match(pt, ot)
+
+
+@pytest.mark.parametrize("level", [0, 1])
+def test_heatmap_level_matches_oracle(ctx, level):
+    case, rng = genome(21, n_chr=2, n_anchors=(40, 30), span=3, splits_per_chr=4)
+    pt = T.Tree(case["chrs"], case["splits"])
+    ot = L.build_tree(case["chrs"], case["splits"])
+    n = sum(len(pt.level(c, level)) for c in range(2))
+    idx = np.arange(n)
+    f = (1.0 / (np.abs(idx[:, None] - idx[None, :]) ** 3 + 0.1)).astype(np.float32)
+    f *= rng.uniform(0.5, 1.5, size=f.shape).astype(np.float32)
+    f = ((f + f.T) / 2).astype(np.float32)
+    dist, _ = T.distance_heatmap(f, 1, level == 0, product_params(PARAMS))
+    if level == 1:   # chromosome positions the segments start from
+        roots = [pt.chromosome(c)[1] for c in range(2)]
+        rp = np.array([[0, 0, 0], [3, 1, 0]], np.float32)
+        pt.set_positions(roots, rp)
+        for k, r in enumerate(roots):
+            ot["clusters"][r]["pos"] = [np.float32(x) for x in rp[k]]
+    s = api.settings(max_temp=5.0, dt_temp=0.9999, improvement=0.99, steps=2048, milestone_fails=3)
+    po = T.heatmap_level_opts(heatmap=s, preset=1, chains_per_restart=64)
+    oo = dict(sched=lambda step: api.schedule_preset(1, 1, s, step), chains=64, skip_below=1e-3)
+    print(f"heatmap_level level={level} n={n}", flush=True)
+    rep = T.heatmap_level(ctx, pt, level, dist, 1, product_params(PARAMS), po, seed=9, segment=40)
+    orep = L.heatmap_level(ot, level, dist, 1, PARAMS, oo, 9, 40, run_job_oracle)
+    assert rep["n_jobs"] == orep["n_jobs"]
+    slack = assert_tree_positions_match(pt, ot)
+    assert rep["score_anchor"] == pytest.approx(orep["score_anchor"], rel=1e-9, abs=slack)
