@@ -108,7 +108,8 @@ typedef struct {
   int32_t positive_guard;
   int32_t cool_per_block;
   int32_t use2d;
-  int32_t resum;          /* 1: re-sum E at milestones (default)                */
+  int32_t resum;          /* K > 0: replace E by its FP64 re-sum at every K-th
+                             milestone (before the stop test); 0: never         */
   int64_t max_moves;      /* hard cap per chain                                 */
 } pmmc_schedule;
 

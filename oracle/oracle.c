@@ -434,7 +434,7 @@ int64_t orc_anneal_chain(int level, const void *problem, const orc_schedule *s,
     step *= s->step_decay;
     if ((iters % s->milestone_period) < s->block) {
       ++milestones;
-      if (s->resum) E = ctx_total(&c, pos);
+      if (s->resum > 0 && milestones % s->resum == 0) E = ctx_total(&c, pos);
       if (!isfinite(E)) { reason = 6; break; }
       int no_imp = (E > s->improvement * ms && msucc < s->min_successes) || E < s->small_energy ||
                    (s->ratio_stop > 0.0 && ms > 1e-10 && E / ms > s->ratio_stop);

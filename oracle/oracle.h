@@ -67,7 +67,7 @@ typedef struct {
   int32_t positive_guard;/* 1: uphill acceptance requires E > 0                  */
   int32_t cool_per_block;/* 1: T *= dt once per block; 0: once per move          */
   int32_t use2d;         /* 1: dz = 0                                            */
-  int32_t resum;         /* 1: re-sum total energy at milestones                 */
+  int32_t resum;         /* K > 0: re-sum total energy at every K-th milestone   */
   int64_t max_moves;     /* cap, checked before each block                       */
 } orc_schedule;
 

@@ -66,9 +66,25 @@ extern "C" void pmmc_set_gpu_seed(pmmc_ctx *c, uint64_t seed) {
 // ------------------------------------------------------------------ //
 // schedule presets: the reference annealers as pmmc_schedule
 // ------------------------------------------------------------------ //
+static pmmc_status fill_preset(int level, int preset, const pmmc_level_settings *s,
+                               float step_size, pmmc_schedule *o);
+
 extern "C" pmmc_status pmmc_schedule_preset(int level, int preset, const pmmc_level_settings *s,
                                             float step_size, pmmc_schedule *o) {
   if (!s || !o) return PMMC_ERR_INVALID;
+  const pmmc_status st = fill_preset(level, preset, s, step_size, o);
+  if (st != PMMC_OK) return st;
+  // FP64 re-sum of the running total at most once per kResumMoves (8192)
+  // moves: the GPU arc and heatmap kernels (256 / 512-move milestones)
+  // re-sum at every 32nd / 16th milestone, the others (milestones every
+  // `steps` moves) at every milestone.  The reference kernels keep a running
+  // score only.
+  o->resum = o->milestone_period >= kResumMoves ? 1 : (kResumMoves + o->milestone_period - 1) / o->milestone_period;
+  return PMMC_OK;
+}
+
+static pmmc_status fill_preset(int level, int preset, const pmmc_level_settings *s,
+                               float step_size, pmmc_schedule *o) {
   std::memset(o, 0, sizeof(*o));
   o->T0 = s->max_temp;
   o->dt = s->dt_temp;
@@ -258,6 +274,9 @@ static pmmc_status prep_arcs(pmmc_ctx *ctx, const pmmc_arcs_problem *p, HostProb
   for (int i = 0; i < n; ++i) h.off[i + 1] = h.off[i] + static_cast<int32_t>(rows[i].size());
   h.ent.reserve(h.off[n]);
   for (int i = 0; i < n; ++i) h.ent.insert(h.ent.end(), rows[i].begin(), rows[i].end());
+  for (int i = 0; i < n; ++i)
+    for (const ArcEntry &e : rows[i])
+      if ((e.j & INT32_MAX) > i) h.pairs.push_back(PairEntry{i, e});
   h.ks = p->k_stretch;
   h.kq = p->k_squeeze;
   h.pos0.assign(p->pos, p->pos + 3 * static_cast<size_t>(n));

@@ -308,7 +308,9 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
     // only the chains at the milestone use its result.
     if (__ballot(due) != 0) {
       double Er = E;
-      if (S.resum) {
+      // due lanes share `iters`, hence `miles`: the re-sum decision is uniform
+      const bool resum_now = S.resum > 0 && ((miles + 1) % S.resum) == 0;
+      if (__ballot(due && resum_now) != 0) {
         if constexpr (LEVEL == PMMC_LEVEL_HEATMAP) Er = resum_heatmap(P, X, r);
         else if constexpr (LEVEL == PMMC_LEVEL_ARCS) Er = resum_arcs(P, X, r, offset);
         else Er = resum_smooth(P, X, r, offset);
@@ -526,6 +528,7 @@ static void size_problem(Sizer &z, const HostProblem &h) {
   z.add<float>(h.Hc.size());
   z.add<int32_t>(h.off.size());
   z.add<ArcEntry>(std::max<size_t>(1, h.ent.size()));
+  z.add<PairEntry>(std::max<size_t>(1, h.pairs.size()));
   z.add<int32_t>(std::max<size_t>(1, h.zpairs.size()));
   z.add<float>(h.dtn.size());
   z.add<double>(2);
@@ -566,6 +569,8 @@ void fill_problem(Carver &cv, hipStream_t s, const HostProblem &h, int index, De
   P.Hc = put(cv, s, h.Hc, err);
   P.off = put(cv, s, h.off, err);
   P.ent = put(cv, s, h.ent, err, 1);
+  P.pairs = put(cv, s, h.pairs, err, 1);
+  P.npairs = static_cast<int32_t>(h.pairs.size());
   P.zpairs = put(cv, s, h.zpairs, err, 1);
   P.nnz = static_cast<int32_t>(h.ent.size());
   P.nz = static_cast<int32_t>(h.zpairs.size() / 2);

@@ -16,7 +16,8 @@
 namespace pmmc {
 
 constexpr int kLanes = 64;
-constexpr int kPickShift = 6;  // chains c and c' share bead picks iff c>>6 == c'>>6
+constexpr int kPickShift = 6;      // chains c and c' share bead picks iff c>>6 == c'>>6
+constexpr int kResumMoves = 8192;  // presets re-sum the running total at most this often
 
 // Philox level field of each annealer.
 constexpr uint32_t kLevelHeatmap = 1, kLevelArcs = 2, kLevelSmooth = 3, kLevelSweep = 4;
@@ -42,6 +43,12 @@ struct ArcEntry {
   float c, ka, kb;
 };
 
+// one arc or loop spring counted once (i < j), for the milestone re-sum
+struct PairEntry {
+  int32_t i;
+  ArcEntry e;
+};
+
 struct DevProblem {
   int32_t level, n, nfree, symmetric;
   int32_t ldw, pad_ldw;      // row stride of W, W2, Hc (n, or 32/64 for small heatmaps)
@@ -54,7 +61,8 @@ struct DevProblem {
   const int32_t *off;        // [n+1]
   const ArcEntry *ent;       // [nnz]
   const int32_t *zpairs;     // [2*nz] off-diagonal pairs with 0 <= e < 1e-6 (i<j)
-  int32_t nnz, nz;
+  const PairEntry *pairs;    // [npairs] the CSR entries with i < j, row order
+  int32_t nnz, nz, npairs, pad2;
   float ks, kq;
   // smooth
   const float *dtn;

@@ -23,6 +23,7 @@ struct HostProblem {
   // arcs CSR
   std::vector<int32_t> off;
   std::vector<ArcEntry> ent;
+  std::vector<PairEntry> pairs;
   std::vector<int32_t> zpairs;
   float ks = 0.f, kq = 0.f;
   // smooth

@@ -15,6 +15,12 @@
 
 namespace pmmc {
 
+#ifndef PMMC_ARCS_CHUNK
+#define PMMC_ARCS_CHUNK 8
+#endif
+// arc CSR entries evaluated per round trip of the move's partner loop
+constexpr int kArcsChunk = PMMC_ARCS_CHUNK;
+
 // ------------------------------------------------------------------ //
 // helpers
 // ------------------------------------------------------------------ //
@@ -147,22 +153,23 @@ __device__ __forceinline__ float arcs_delta(const DevProblem &P, const PosView &
                                             float py, float pz, float nx, float ny, float nz) {
   const int e0 = P.off[p], e1 = P.off[p + 1];
   float acc = 0.f;
-  // Four entries per round trip: the (wave-uniform) entry loads and then the
-  // twelve per-lane position loads are all issued before any is consumed.
-  for (int e = e0; e < e1; e += 4) {
-    ArcEntry en[4];
-    float xj[4], yj[4], zj[4];
+  // kArcsChunk entries per round trip: the (wave-uniform) entry loads and
+  // then the 3*kArcsChunk per-lane position loads are all issued before any
+  // is consumed.
+  for (int e = e0; e < e1; e += kArcsChunk) {
+    ArcEntry en[kArcsChunk];
+    float xj[kArcsChunk], yj[kArcsChunk], zj[kArcsChunk];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) en[u] = P.ent[min(e + u, e1 - 1)];
+    for (int u = 0; u < kArcsChunk; ++u) en[u] = P.ent[min(e + u, e1 - 1)];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kArcsChunk; ++u) {
       const int j = en[u].j & 0x7fffffff;
       xj[u] = X.get(j, 0);
       yj[u] = X.get(j, 1);
       zj[u] = X.get(j, 2);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kArcsChunk; ++u) {
       if (e + u < e1) {
         const float t0 = dist3(px, py, pz, xj[u], yj[u], zj[u]) - en[u].c;
         const float t1 = dist3(nx, ny, nz, xj[u], yj[u], zj[u]) - en[u].c;
@@ -257,23 +264,43 @@ __device__ double resum_heatmap(const DevProblem &P, const PosView &X, float r) 
   return tot;
 }
 
+// Arc + loop terms over the pair list (each pair once), eight pairs per
+// round trip: the (wave-uniform) pair entries and then the 48 per-lane
+// position loads are all issued before any is consumed.  FP32 partial sums of
+// at most 64 pairs are added into the FP64 total.
 __device__ double resum_arcs(const DevProblem &P, const PosView &X, float r, double offset) {
   const int n = P.n;
+  const int np = P.npairs;
   double tot = offset;
-  for (int p = 0; p < n; ++p) {
-    const int e0 = P.off[p], e1 = P.off[p + 1];
-    if (e0 == e1) continue;
-    const float xp = X.get(p, 0), yp = X.get(p, 1), zp = X.get(p, 2);
-    float part = 0.f;
-    for (int e = e0; e < e1; ++e) {
-      const ArcEntry en = P.ent[e];
-      const int j = en.j & 0x7fffffff;
-      if (j <= p) continue;
-      const float t = dist3(xp, yp, zp, X.get(j, 0), X.get(j, 1), X.get(j, 2)) - en.c;
-      part = fmaf(t * t, t >= 0.f ? en.ka : en.kb, part);
+  float part = 0.f;
+  for (int q = 0; q < np; q += 8) {
+    PairEntry pe[8];
+    float xi[8], yi[8], zi[8], xj[8], yj[8], zj[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) pe[u] = P.pairs[min(q + u, np - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = pe[u].i, j = pe[u].e.j & 0x7fffffff;
+      xi[u] = X.get(i, 0);
+      yi[u] = X.get(i, 1);
+      zi[u] = X.get(i, 2);
+      xj[u] = X.get(j, 0);
+      yj[u] = X.get(j, 1);
+      zj[u] = X.get(j, 2);
     }
-    tot += static_cast<double>(part);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (q + u < np) {
+        const float t = dist3(xi[u], yi[u], zi[u], xj[u], yj[u], zj[u]) - pe[u].e.c;
+        part = fmaf(t * t, t >= 0.f ? pe[u].e.ka : pe[u].e.kb, part);
+      }
+    }
+    if ((q & 63) == 56) {
+      tot += static_cast<double>(part);
+      part = 0.f;
+    }
   }
+  tot += static_cast<double>(part);
   if (r > 0.f)
     for (int m = 0; m < n; ++m) tot += sphere_penalty(X.get(m, 0), X.get(m, 1), X.get(m, 2), r);
   return tot;

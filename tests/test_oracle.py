@@ -238,6 +238,26 @@ def test_cap_is_checked_before_each_block():
     assert st["moves"] == 1024 and st["stop_reason"] == 3     # 4 blocks, like the GPU kernels
 
 
+@pytest.mark.parametrize("k", [1, 3, 4])
+def test_resum_interval(k):
+    """resum = K replaces the running total by the full score at every K-th
+    milestone only: after exactly K milestones E is the full score of the
+    final positions; after K - 1 it is still the running total."""
+    P = _small_heatmap()
+    for blocks, expect_exact in ((k, True), (k - 1, False)):
+        if blocks == 0:
+            continue
+        s = _sched(stop_rule=1, fails_threshold=10 ** 6, max_moves=100 * blocks, resum=k)
+        x, st = P.anneal(s, seed=8, segment=0, chain=2)
+        assert st["milestones"] == blocks
+        run = _sched(stop_rule=1, fails_threshold=10 ** 6, max_moves=100 * blocks, resum=0)
+        _, st0 = P.anneal(run, seed=8, segment=0, chain=2)
+        if expect_exact:
+            assert st["energy"] == P.total(x)
+        else:
+            assert st["energy"] == st0["energy"]
+
+
 def test_zero_temperature_never_goes_uphill():
     P = _small_heatmap()
     s = _sched(T0=0.0, max_moves=3000, stop_rule=1, fails_threshold=10 ** 6, resum=0)
