@@ -91,10 +91,14 @@ __device__ __forceinline__ double sphere_penalty_d(float x, float y, float z, fl
   return o * o;
 }
 
+// FP32 distance with the hardware square root (v_sqrt_f32, 1 ulp).  The
+// correctly rounded sqrtf expands to ~15 VALU instructions around it, which
+// made the square roots the largest VALU cost of the move loops; the move
+// deltas keep their 1e-5 relative bound against the FP64 energies.
 __device__ __forceinline__ float dist3(float ax, float ay, float az, float bx, float by,
                                        float bz) {
   const float dx = ax - bx, dy = ay - by, dz = az - bz;
-  return sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+  return __builtin_amdgcn_sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
 }
 
 __device__ __forceinline__ double dist3d(float ax, float ay, float az, float bx, float by,

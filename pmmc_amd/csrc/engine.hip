@@ -473,6 +473,20 @@ static constexpr int kStageMaxBytes = PMMC_STAGE_MAX_BYTES;
 
 bool stage_ok(int n) { return static_cast<size_t>(n) * 3 * kLanes * sizeof(float) <= kStageMaxBytes; }
 
+LaunchGroup group_of(const HostProblem &h) {
+  LaunchGroup g{};
+  g.level = h.level;
+  g.stage = stage_ok(h.n);
+  g.n = g.stage ? h.n : 0;
+  g.hreg = g.stage ? 0 : hreg_halves(h);
+  g.asym = g.hreg > 0 && !h.symmetric;
+  return g;
+}
+
+std::tuple<int, int, int, int, int> group_key(const LaunchGroup &g) {
+  return std::make_tuple(g.level, g.stage ? 1 : 0, g.n, g.hreg, g.asym ? 1 : 0);
+}
+
 template <int LEVEL>
 static hipError_t launch_level(hipStream_t s, const DevProblem *probs, const DevEns *ens, int n_ens,
                                int max_blocks, bool stage, int n_for_stage) {
@@ -496,7 +510,10 @@ hipError_t launch_groups(hipStream_t s, const DevProblem *probs, const DevEns *e
     const DevEns *e = ens + g.ens0;
     switch (g.level) {
       case PMMC_LEVEL_HEATMAP:
-        err = launch_level<PMMC_LEVEL_HEATMAP>(s, probs, e, g.n_ens, max_blocks, g.stage, g.n);
+        if (g.hreg > 0)
+          err = launch_heatmap_reg(s, probs, e, g.n_ens, max_blocks, g.hreg, g.asym);
+        else
+          err = launch_level<PMMC_LEVEL_HEATMAP>(s, probs, e, g.n_ens, max_blocks, g.stage, g.n);
         break;
       case PMMC_LEVEL_ARCS:
         err = launch_level<PMMC_LEVEL_ARCS>(s, probs, e, g.n_ens, max_blocks, g.stage, g.n);
@@ -630,21 +647,18 @@ pmmc_status batch_setup(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &b
   bd.hprobs.resize(hp.size());
   bd.hens.clear();
   bd.groups.clear();
-  // order ensembles by (level, stage, n) so each launch group is contiguous
+  // order ensembles by launch group so each group is contiguous
   std::vector<int> order(hp.size());
   for (size_t i = 0; i < hp.size(); ++i) order[i] = static_cast<int>(i);
-  auto key = [&](int i) {
-    const HostProblem &h = hp[i];
-    const bool stg = stage_ok(h.n);
-    return std::make_tuple(h.level, stg ? 1 : 0, stg ? h.n : 0);
-  };
+  auto key = [&](int i) { return group_key(group_of(hp[i])); };
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
   for (int oi : order) {
     HostProblem &h = hp[oi];
-    const bool stg = stage_ok(h.n);
-    if (bd.groups.empty() || bd.groups.back().level != h.level || bd.groups.back().stage != stg ||
-        (stg && bd.groups.back().n != h.n))
-      bd.groups.push_back({h.level, stg, stg ? h.n : 0, static_cast<int>(bd.hens.size()), 0});
+    const LaunchGroup g = group_of(h);
+    if (bd.groups.empty() || group_key(bd.groups.back()) != group_key(g)) {
+      bd.groups.push_back(g);
+      bd.groups.back().ens0 = static_cast<int>(bd.hens.size());
+    }
     const size_t before = bd.hens.size();
     fill_problem(cv, s, h, oi, bd.results + oi, bd.hprobs[oi], bd.hens, err);
     bd.groups.back().n_ens += static_cast<int>(bd.hens.size() - before);

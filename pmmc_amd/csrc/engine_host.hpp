@@ -3,6 +3,7 @@
 
 #include <stdint.h>
 
+#include <tuple>
 #include <vector>
 
 #include "ctx.hpp"
@@ -41,13 +42,18 @@ pmmc_status check_schedule(pmmc_ctx *ctx, const pmmc_schedule &s);
 int autotune_blocks_per_step(const std::vector<HostProblem> &hp);
 int default_chains(const pmmc_ctx *ctx, int level, int n);
 
-// contiguous range of ensembles launched together: one (level, LDS staging, n)
+// contiguous range of ensembles launched together with one kernel:
+// (level, LDS staging and its n, register-resident heatmap halves, asymmetry)
 struct LaunchGroup {
   int level;
   bool stage;
-  int n;
+  int n;      // staged groups: the problems' n (sets the LDS size); else 0
+  int hreg;   // heatmap kept in registers: 2 (33..64 beads, two 32-bead halves); 0 = no
+  bool asym;  // register heatmap group holds a non-symmetric heatmap
   int ens0, n_ens;
 };
+LaunchGroup group_of(const HostProblem &h);
+std::tuple<int, int, int, int, int> group_key(const LaunchGroup &g);
 
 // everything a batch needs on the device
 struct BatchDev {
@@ -68,6 +74,10 @@ void fill_problem(Carver &cv, hipStream_t s, const HostProblem &h, int index, De
 hipError_t launch_groups(hipStream_t s, const DevProblem *probs, const DevEns *ens,
                          const std::vector<LaunchGroup> &groups, int max_blocks);
 hipError_t launch_init(hipStream_t s, DevProblem *probs, int n_probs, const DevEns *ens, int n_ens);
+// register-resident heatmap kernel (heatmap_reg.hip) for heatmaps of <= 64 beads
+int hreg_halves(const HostProblem &h);
+hipError_t launch_heatmap_reg(hipStream_t s, const DevProblem *probs, const DevEns *ens, int n_ens,
+                              int max_blocks, int halves, bool asym);
 hipError_t launch_finalize(hipStream_t s, DevProblem *probs, int n_probs, const DevEns *ens);
 hipError_t launch_count_per_problem(hipStream_t s, const DevEns *ens, int n_ens, int *counts,
                                     int n_counts);

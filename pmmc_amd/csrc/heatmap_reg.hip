@@ -1,0 +1,259 @@
+// Register-resident heatmap annealer for heatmaps of at most 64 beads.
+//
+// Same Markov chains, random streams, schedule and stop rules as
+// k_anneal<PMMC_LEVEL_HEATMAP> (engine.hip); only where the positions live
+// differs.  Each lane keeps its chain's whole structure in VGPRs (NB = 64
+// beads per axis), so the O(N) partner loop of a move reads no memory: it
+// is fully unrolled over constant register indices.  The moved bead p is
+// wave-uniform (shared pick stream); its position is read and written
+// through a uniform switch.  The weight table (16 KB for 64 beads) is copied to LDS once per
+// launch; a move reads row p with uniform-address ds_read_b128.  Positions go
+// back to the SoA array in HBM only for the milestone re-sum and at the end
+// of the launch.
+#include <hip/hip_runtime.h>
+
+#include "ctx.hpp"
+#include "engine_host.hpp"
+#include "kernels.hpp"
+
+namespace pmmc {
+
+// Read / write bead p of the register arrays.  p is wave-uniform, so the
+// switch is a scalar branch tree and every case touches fixed registers
+// (dynamic indexing would force the arrays out of registers).
+#define PMMC_GET(J)  \
+  case (J):          \
+    px = X[J];       \
+    py = Y[J];       \
+    pz = Z[J];       \
+    break;
+#define PMMC_SET(J)          \
+  case (J):                  \
+    X[J] = ok ? nx : X[J];   \
+    Y[J] = ok ? ny : Y[J];   \
+    Z[J] = ok ? nz : Z[J];   \
+    break;
+#define PMMC_X4(M, B) M((B)) M((B) + 1) M((B) + 2) M((B) + 3)
+#define PMMC_X16(M, B) PMMC_X4(M, B) PMMC_X4(M, (B) + 4) PMMC_X4(M, (B) + 8) PMMC_X4(M, (B) + 12)
+#define PMMC_X32(M, B) PMMC_X16(M, B) PMMC_X16(M, (B) + 16)
+
+template <int NB>
+__device__ __forceinline__ void get_bead(const float (&X)[NB], const float (&Y)[NB],
+                                         const float (&Z)[NB], int p, float &px, float &py,
+                                         float &pz) {
+  static_assert(NB == 64, "64-bead form");
+  switch (p) { PMMC_X32(PMMC_GET, 0) PMMC_X32(PMMC_GET, 32) default: break; }
+}
+
+template <int NB>
+__device__ __forceinline__ void set_bead(float (&X)[NB], float (&Y)[NB], float (&Z)[NB], int p,
+                                         bool ok, float nx, float ny, float nz) {
+  static_assert(NB == 64, "64-bead form");
+  switch (p) { PMMC_X32(PMMC_SET, 0) PMMC_X32(PMMC_SET, 32) default: break; }
+}
+
+template <int NB>
+__device__ __forceinline__ void store_all(float *g, int n, const float (&X)[NB], const float (&Y)[NB],
+                                          const float (&Z)[NB]) {
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    if (j < n) {
+      g[(3 * j + 0) * kLanes] = X[j];
+      g[(3 * j + 1) * kLanes] = Y[j];
+      g[(3 * j + 2) * kLanes] = Z[j];
+    }
+  }
+}
+
+// ΔE of moving p from (px,py,pz) to (nx,ny,nz): exact change of the
+// double-counted full score, as heatmap_delta (kernels.hpp), over registers.
+// The weight rows come from LDS (row p of W at lw + p*NB; of W2 at
+// lw2 + p*NB), four partners per ds_read_b128, the same address in every
+// lane.
+template <int NB, bool ASYM>
+__device__ __forceinline__ float hreg_delta(const float *lw, const float *lw2, const float (&X)[NB],
+                                            const float (&Y)[NB], const float (&Z)[NB], int p,
+                                            float px, float py, float pz, float nx, float ny,
+                                            float nz) {
+  const float4 *w1 = reinterpret_cast<const float4 *>(lw + p * NB);
+  float acc0 = 0.f, acc1 = 0.f;
+#pragma unroll
+  for (int q = 0; q < NB; q += 4) {
+    const float4 w = w1[q / 4];
+    const float wq[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int u = 0; u < 4; u += 2) {
+      const int a = q + u, b = q + u + 1;
+      const float a0 = fmaf(dist3(px, py, pz, X[a], Y[a], Z[a]), wq[u], -1.f);
+      const float a1 = fmaf(dist3(nx, ny, nz, X[a], Y[a], Z[a]), wq[u], -1.f);
+      const float b0 = fmaf(dist3(px, py, pz, X[b], Y[b], Z[b]), wq[u + 1], -1.f);
+      const float b1 = fmaf(dist3(nx, ny, nz, X[b], Y[b], Z[b]), wq[u + 1], -1.f);
+      acc0 = fmaf(a1 - a0, a1 + a0, acc0);
+      acc1 = fmaf(b1 - b0, b1 + b0, acc1);
+    }
+  }
+  const float d = acc0 + acc1;
+  if constexpr (!ASYM) {
+    return 2.f * d;
+  } else {
+    const float4 *w2 = reinterpret_cast<const float4 *>(lw2 + p * NB);
+    float acc2 = 0.f;
+#pragma unroll
+    for (int q = 0; q < NB; q += 4) {
+      const float4 w = w2[q / 4];
+      const float wq[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int a = q + u;
+        const float a0 = fmaf(dist3(px, py, pz, X[a], Y[a], Z[a]), wq[u], -1.f);
+        const float a1 = fmaf(dist3(nx, ny, nz, X[a], Y[a], Z[a]), wq[u], -1.f);
+        acc2 = fmaf(a1 - a0, a1 + a0, acc2);
+      }
+    }
+    return d + acc2;
+  }
+}
+
+// Occupancy target: 2 waves per SIMD (192 position VGPRs of 256).
+template <int NB, bool ASYM>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_anneal_hreg(const DevProblem *__restrict__ probs,
+                                                    const DevEns *__restrict__ ens, int max_blocks) {
+  extern __shared__ float lw[];  // W (and W2), NB*NB floats each
+  const DevEns it = ens[blockIdx.x];
+  const DevProblem &P = probs[it.problem];
+  const int lane = threadIdx.x;
+  EnsState *__restrict__ st = it.st;
+  int status = st->status[lane];
+  if (__ballot(status == 0) == 0) return;
+
+  const int n = P.n;
+  constexpr int L = NB;  // == P.ldw (rows padded with zero weights)
+  float *lw2 = lw + L * L;
+  // W and W2 hold n rows of L floats; rows n..L-1 of the LDS tables are zero
+  for (int i = lane; i < L * L; i += kLanes) {
+    const bool in = i < n * L;
+    lw[i] = in ? P.W[i] : 0.f;
+    if constexpr (ASYM) lw2[i] = in ? P.W2[i] : 0.f;
+  }
+  __syncthreads();
+  float *g = it.pos + lane;
+  float X[NB], Y[NB], Z[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    X[j] = j < n ? g[(3 * j + 0) * kLanes] : 0.f;
+    Y[j] = j < n ? g[(3 * j + 1) * kLanes] : 0.f;
+    Z[j] = j < n ? g[(3 * j + 2) * kLanes] : 0.f;
+  }
+
+  const pmmc_schedule S = P.sched;
+  const uint64_t seed = P.seed;
+  const uint32_t segment = P.segment;
+  const int nfree = P.nfree;
+  const float r = S.sphere_radius;
+
+  double E = st->E[lane], T = st->T[lane], ms = st->ms[lane], best = st->best[lane];
+  float step = st->step[lane];
+  uint32_t iters = st->iters[lane], acc = st->acc[lane];
+  int msucc = st->msucc[lane], misses = st->misses[lane], miles = st->milestones[lane];
+  bool active = (status == 0);
+  const uint32_t chain = it.chain0 + static_cast<uint32_t>(lane);
+  const uint32_t group = it.chain0 >> kPickShift;
+
+  for (int b = 0; b < max_blocks; ++b) {
+    if (active && static_cast<int64_t>(iters) >= S.max_moves) {
+      status = PMMC_STOP_CAP;
+      active = false;
+    }
+    const uint64_t am = __ballot(active);
+    if (am == 0) break;
+    const uint32_t m0 = readlane_u32(iters, __builtin_ctzll(am));
+    U4 rb = {0, 0, 0, 0};
+    for (int k = 0; k < S.block; ++k) {
+      const uint32_t m = m0 + static_cast<uint32_t>(k);
+      if (k == 0 || (m & 3u) == 0) rb = draw(seed, m >> 2, group, segment, PMMC_LEVEL_HEATMAP, kSubBead);
+      const uint32_t sel = m & 3u;
+      const uint32_t word = sel == 0 ? rb.x : (sel == 1 ? rb.y : (sel == 2 ? rb.z : rb.w));
+      const int p = __builtin_amdgcn_readfirstlane(P.free_list[pick(word, static_cast<uint32_t>(nfree))]);
+      float px = 0.f, py = 0.f, pz = 0.f;
+      get_bead<NB>(X, Y, Z, p, px, py, pz);
+      const U4 rc = draw(seed, m, chain, segment, PMMC_LEVEL_HEATMAP, kSubChain);
+      const float dx = disp(rc.x, step), dy = disp(rc.y, step);
+      const float dz = S.use2d ? 0.f : disp(rc.z, step);
+      const float nx = move_coord(px, dx), ny = move_coord(py, dy), nz = move_coord(pz, dz);
+      float dE = hreg_delta<NB, ASYM>(lw, lw2, X, Y, Z, p, px, py, pz, nx, ny, nz);
+      if (r > 0.f) dE += sphere_penalty(nx, ny, nz, r) - sphere_penalty(px, py, pz, r);
+      const double En = E + static_cast<double>(dE);
+      const bool ok = active && metropolis(S, E, En, T, u01(rc.w));
+      set_bead<NB>(X, Y, Z, p, ok, nx, ny, nz);
+      if (ok) {
+        E = En;
+        ++msucc;
+        ++acc;
+        best = E < best ? E : best;
+      }
+      if (active && !S.cool_per_block) T *= S.dt;
+    }
+    bool due = false;
+    if (active) {
+      iters += static_cast<uint32_t>(S.block);
+      if (S.cool_per_block) T *= S.dt;
+      step *= S.step_decay;
+      due = (iters % static_cast<uint32_t>(S.milestone_period)) < static_cast<uint32_t>(S.block);
+    }
+    if (__ballot(due) != 0) {
+      double Er = E;
+      const bool resum_now = S.resum > 0 && ((miles + 1) % S.resum) == 0;
+      if (__ballot(due && resum_now) != 0) {
+        store_all<NB>(g, n, X, Y, Z);
+        PosView V;
+        V.p = g;
+        Er = resum_heatmap(P, V, r);
+      }
+      if (due) {
+        ++miles;
+        E = Er;
+        milestone(S, E, ms, msucc, misses, status, active);
+      }
+    }
+  }
+  store_all<NB>(g, n, X, Y, Z);
+  if (status == -1) return;  // padding lane of a partial ensemble
+  st->E[lane] = E;
+  st->T[lane] = T;
+  st->ms[lane] = ms;
+  st->best[lane] = best;
+  st->step[lane] = step;
+  st->iters[lane] = iters;
+  st->acc[lane] = acc;
+  st->msucc[lane] = msucc;
+  st->misses[lane] = misses;
+  st->milestones[lane] = miles;
+  st->status[lane] = status;
+}
+
+// heatmaps of 33..64 beads (weight rows padded to 64).  Up to 32 beads the
+// partner loop over L2-resident positions is faster (N=24: 16.3 vs 10.3
+// G moves/s; N=64: 6.2 vs 9.2).
+int hreg_halves(const HostProblem &h) {
+  if (h.level != PMMC_LEVEL_HEATMAP || h.n <= 32 || h.n > 64) return 0;
+  return h.ldw == 64 ? 2 : 0;
+}
+
+template <int NB, bool ASYM>
+static void launch_hreg(hipStream_t s, const DevProblem *probs, const DevEns *ens, int n_ens,
+                        int max_blocks) {
+  const size_t lds = static_cast<size_t>(NB) * NB * sizeof(float) * (ASYM ? 2 : 1);
+  hipLaunchKernelGGL((k_anneal_hreg<NB, ASYM>), dim3(n_ens), dim3(kLanes), lds, s, probs, ens,
+                     max_blocks);
+}
+
+hipError_t launch_heatmap_reg(hipStream_t s, const DevProblem *probs, const DevEns *ens, int n_ens,
+                              int max_blocks, int halves, bool asym) {
+  if (n_ens <= 0) return hipSuccess;
+  if (halves != 2) return hipErrorInvalidValue;
+  asym ? launch_hreg<64, true>(s, probs, ens, n_ens, max_blocks)
+       : launch_hreg<64, false>(s, probs, ens, n_ens, max_blocks);
+  return hipGetLastError();
+}
+
+}  // namespace pmmc

@@ -104,23 +104,21 @@ static pmmc_status admit(pmmc_server *s, SrvJob *j) {
 static pmmc_status rebuild(pmmc_server *s) {
   pmmc_ctx *ctx = s->ctx;
   std::vector<DevProblem> probs;
-  std::vector<std::tuple<int, int, int, int>> keys;  // (level, stage, n, job)
+  std::vector<std::pair<std::tuple<int, int, int, int, int>, int>> keys;  // (group key, job)
   for (size_t k = 0; k < s->resident.size(); ++k) {
-    const HostProblem &h = s->resident[k]->hp;
-    const bool st = stage_ok(h.n);
-    keys.emplace_back(h.level, st ? 1 : 0, st ? h.n : 0, static_cast<int>(k));
+    keys.emplace_back(group_key(group_of(s->resident[k]->hp)), static_cast<int>(k));
     probs.push_back(s->resident[k]->hprob);
   }
   std::stable_sort(keys.begin(), keys.end());
   std::vector<DevEns> ens;
   s->groups.clear();
   for (auto &t : keys) {
-    const int k = std::get<3>(t);
-    const HostProblem &h = s->resident[k]->hp;
-    const bool st = std::get<1>(t) != 0;
-    if (s->groups.empty() || s->groups.back().level != h.level || s->groups.back().stage != st ||
-        (st && s->groups.back().n != h.n))
-      s->groups.push_back({h.level, st, st ? h.n : 0, static_cast<int>(ens.size()), 0});
+    const int k = t.second;
+    const LaunchGroup g = group_of(s->resident[k]->hp);
+    if (s->groups.empty() || group_key(s->groups.back()) != t.first) {
+      s->groups.push_back(g);
+      s->groups.back().ens0 = static_cast<int>(ens.size());
+    }
     for (DevEns e : s->resident[k]->hens) {
       e.problem = k;
       ens.push_back(e);

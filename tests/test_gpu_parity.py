@@ -107,13 +107,23 @@ def _assert_traj(pos, st, res, best, total_moves_rel=0.0):
     assert st["moves"] == sum(r[1]["moves"] for r in res)
 
 
-@pytest.mark.parametrize("preset", [0, 1])
-def test_trajectory_heatmap(ctx, preset):
-    prob, _ = synth.heatmap_problem(n=32, seed=5)
+# 33 <= n <= 64 runs the register-resident kernel (symmetric and asymmetric
+# weight tables); n <= 32 and n = 100 the partner loop over memory.
+@pytest.mark.parametrize("n,asym,radius,use2d,preset", [
+    (32, False, 0.0, 0, 0), (32, False, 0.0, 0, 1), (64, False, 0.0, 0, 1), (50, True, 0.0, 0, 1),
+    (40, False, 6.0, 1, 1), (20, True, 5.0, 0, 0), (100, False, 0.0, 0, 1)])
+def test_trajectory_heatmap(ctx, n, asym, radius, use2d, preset):
+    prob, _ = synth.heatmap_problem(n=n, seed=5)
+    if asym:   # scale the upper triangle: h[i][j] != h[j][i] where both are targets
+        rng = np.random.default_rng(n)
+        up = np.triu(prob.dist > 0, 1)
+        prob.dist = np.where(up, prob.dist * rng.uniform(0.8, 1.2, prob.dist.shape),
+                             prob.dist).astype(np.float32)
     s = settings(max_temp=5.0, dt_temp=0.999, improvement=0.99, steps=512, min_successes=10,
-                 milestone_fails=3)
+                 milestone_fails=3, sphere_radius=radius, use2d=use2d)
     sched = schedule_preset(1, preset, s, 1.0)
     sched.max_moves = 4096
+    print(f"trajectory heatmap n={n} asym={asym} r={radius} 2d={use2d} preset={preset}", flush=True)
     _assert_traj(*_run_pair(ctx, prob, sched))
 
 
