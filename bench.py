@@ -71,6 +71,22 @@ def measured_traffic_per_move():
     return None
 
 
+def copy_calibration() -> int:
+    """Three device-to-device copies of 1 GiB (torch's copy kernel): reading
+    FETCH_SIZE / WRITE_SIZE for them gives the factor between each counter
+    and bytes actually moved."""
+    import torch
+    n = 1 << 28
+    src = torch.ones(n, device="cuda", dtype=torch.float32)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copied_bytes_per_launch": n * 4, "launches": 3}))
+    return 0
+
+
 def cpu_baseline(prob, s, step, budget_s=15.0):
     """The reference's CPU annealer (LooperSolver::MonteCarloArcs, dense O(n)
     local scans) restated in the oracle, one chain per host thread."""
@@ -207,7 +223,12 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB three times (device to device), the known-size "
+                         "transfer that scales the FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args(argv)
+    if a.copy_calibration:
+        return copy_calibration()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": a.gpus}))
@@ -219,7 +240,6 @@ def main(argv=None):
     prob, s, step, arcs = workload()
     sched = schedule_preset(2, 1, s, step)
     deg = 2.0 * len(arcs) / 400.0  # CSR entries per bead
-    bytes_per_move = 2.0 * deg * (12 + 4 + 4)
     with Context(0) as ctx:
         for _ in range(a.warmup):
             ctx.anneal(prob, sched, seed=1, n_chains=a.chains)
@@ -241,6 +261,11 @@ def main(argv=None):
         if not a.no_served:
             extra["served"] = served(ctx)
     value = moves / wall
+    # algorithmic HBM bytes per chain-move: the moved bead's position and its
+    # deg partners' positions (12 B each, read once for both evaluations),
+    # the CSR row (deg x 16 B, one read shared by the 64 chains of an
+    # ensemble) and the 12 B position write of an accepted move
+    bytes_per_move = 12.0 * (1.0 + deg) + 16.0 * deg / 64.0 + 12.0 * acc / moves
     achieved = bytes_per_move * moves / (kern_ms * 1e-3) / 1e9
     tpm = measured_traffic_per_move()
     line = {
@@ -260,10 +285,11 @@ def main(argv=None):
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None if tpm is None else tpm * moves / a.steps,
                      "bytes_per_move": bytes_per_move,
-                     "accounting": "CSR variant: 2 evaluations x deg x (12 B position + 4 B "
-                                   "target + 4 B index) per proposal; traffic = measured HBM "
-                                   "bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE "
-                                   "x1.0, copy-kernel calibrated)"},
+                     "accounting": "per chain-move: 12 B x (1 + deg) positions read + "
+                                   "16 B x deg / 64 CSR row (shared by the 64 chains of an "
+                                   "ensemble) + 12 B x acceptance rate written, deg = 2 x arcs / "
+                                   "anchors; traffic = measured HBM bytes per launch (rocprofv3 "
+                                   "FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-kernel calibrated)"},
     }
     line.update(extra)
     if not a.no_cpu:
