@@ -331,6 +331,25 @@ pmmc_status pmmc_server_step(pmmc_server *srv, int32_t *n_done, int32_t *n_resid
 /* 1 when job_id has finished (its outputs are written). */
 int pmmc_server_done(const pmmc_server *srv, uint64_t job_id);
 
+/* ---------------- equidistant model (spline) ---------------- */
+/* Number of outputs the reference allocates for [pos_start, pos_stop] at
+ * resolution_bp: (pos_stop - pos_start + 1) / resolution_bp + 1
+ * (HierarchicalChromosome.cu:1268). */
+int64_t pmmc_spline_count(int64_t pos_start, int64_t pos_stop, int32_t resolution_bp);
+
+/* Catmull-Rom positions of one chromosome's bead chain at genomic positions
+ * pos_start + k*resolution_bp, k < n_out <- HierarchicalChromosome::gpuHelper
+ * (HierarchicalChromosome.cu:1252-1449) and its kernel
+ * parallelFind3DSmoothPosition (:420-464).  genomic_pos [n_beads] sorted
+ * (the level order after HierarchicalChromosome::levelDown, :232-257), xyz
+ * [3*n_beads]; first_end / last_start = end of the first bead, start of the
+ * last.  Every output is written (the reference kernel leaves outputs beyond
+ * one grid unwritten); arithmetic is the reference's CPU path's, bit for bit. */
+pmmc_status pmmc_spline_positions(pmmc_ctx *ctx, int32_t n_beads, const int32_t *genomic_pos,
+                                  const float *xyz, int32_t first_end, int32_t last_start,
+                                  int64_t pos_start, int32_t resolution_bp, int64_t n_out,
+                                  float *out_xyz);
+
 #ifdef __cplusplus
 }
 #endif

@@ -194,6 +194,9 @@ SIGNATURES = {
     "pmmc_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_uint64)]),
     "pmmc_server_step": (C.c_int, [C.c_void_p, i32p, i32p, i32p]),
     "pmmc_server_done": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "pmmc_spline_count": (C.c_int64, [C.c_int64, C.c_int64, C.c_int32]),
+    "pmmc_spline_positions": (C.c_int, [C.c_void_p, C.c_int32, i32p, f32p, C.c_int32, C.c_int32,
+                                        C.c_int64, C.c_int32, C.c_int64, f32p]),
     # include/pmmc_levels.h
     "pmmc_level_params_default": (None, [C.POINTER(LevelParams)]),
     "pmmc_host_last_error": (C.c_char_p, []),

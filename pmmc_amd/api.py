@@ -155,6 +155,21 @@ class Context:
             msg = self.lib.pmmc_last_error(self.h).decode() or self.lib.pmmc_host_last_error().decode()
             raise PmmcError(st, msg)
 
+    def spline_positions(self, genomic_pos, xyz, first_end: int, last_start: int, pos_start: int,
+                         resolution_bp: int, n_out: int | None = None, pos_stop: int | None = None):
+        """pmmc_spline_positions: Catmull-Rom points at pos_start + k*resolution_bp
+        (n_out points, or the reference's count for [pos_start, pos_stop])."""
+        gp = np.ascontiguousarray(genomic_pos, np.int32)
+        x = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        if n_out is None:
+            n_out = self.lib.pmmc_spline_count(int(pos_start), int(pos_stop), int(resolution_bp))
+        out = np.empty((int(n_out), 3), np.float32)
+        self._check(self.lib.pmmc_spline_positions(
+            self.h, len(gp), gp.ctypes.data_as(A.i32p), x.ctypes.data_as(A.f32p), int(first_end),
+            int(last_start), int(pos_start), int(resolution_bp), int(n_out),
+            out.ctypes.data_as(A.f32p)))
+        return out
+
     @property
     def cu_count(self) -> int:
         return self.lib.pmmc_ctx_cu_count(self.h)
