@@ -273,6 +273,36 @@ pmmc_status pmmc_heatmap_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t level, const
                                const pmmc_heatmap_level_opts *o, uint64_t seed, uint32_t segment,
                                pmmc_level_report *report);
 
+/* ---- reconstruction-quality metrics (host) -------------------------------- */
+/* xyz arrays are [3*n] float, bead-major; maps are [n*n] float row-major. */
+
+/* Pearson correlation with FP64 sums in input order; 0 for n <= 0 or a
+ * degenerate denominator <- MetricsFramework::pearsonCorrelation
+ * (MetricsFramework.hpp:238-266), BenchmarkRunner::calcDistanceCorrelation
+ * (BenchmarkRunner.hpp:340-368). */
+float pmmc_pearson(const float *a, const float *b, int64_t n);
+
+/* Pearson correlation of the pairwise bead distances of two structures over
+ * the pairs (i, j), i < j, both multiples of `step` <-
+ * MetricsFramework::distanceCorrelation (MetricsFramework.hpp:268-286).
+ * step <= 0 selects BenchmarkRunner::runSingle's rule, n / 500 when n > 500
+ * and 1 otherwise (BenchmarkRunner.hpp:314-331). */
+float pmmc_distance_correlation(const float *xyz_a, const float *xyz_b, int32_t n, int32_t step);
+
+/* FP32 pairwise distance map of a structure (0 on the diagonal). */
+pmmc_status pmmc_distance_map(const float *xyz, int32_t n, float *out);
+
+/* Stratum-adjusted correlation of two maps: per-diagonal Pearson for
+ * k = 1..n-2, weighted by (n-k)·exp(-k²/(2·h²)) <-
+ * MetricsFramework::structuralSimilarity / perDiagonalCorrelation
+ * (MetricsFramework.hpp:419-479; the reference's default h is 5). */
+float pmmc_scc(const float *a, const float *b, int32_t n, int32_t h_param);
+
+/* Least RMSD between two structures over proper rotations and translations
+ * <- Chromosome::calcRMSD (Chromosome.hpp:613-634), fast_rmsd
+ * (thirdparty/rmsd.cpp:395). */
+float pmmc_rmsd(const float *xyz_a, const float *xyz_b, int32_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,13 @@
 //   score     calcScoreHeatmapActiveRegion (:3583-3612),
 //             calcScoreDistancesActiveRegion (:3215-3247),
 //             calcScoreStructureSmooth (:3414-3451)
+//   anneal    the reference's CPU annealers: MonteCarloHeatmap (:1424-1565),
+//             MonteCarloArcs (:4670-4796), MonteCarloArcsSmooth (:4798-5047),
+//             driven by glibc rand() after srand(seed), as the reference's
+//             `-j seed` option seeds them (main.cpp:2833-2845)
+//   metrics   MetricsFramework::pearsonCorrelation, distanceCorrelation,
+//             structuralSimilarity (MetricsFramework.hpp:238-479) and
+//             Chromosome::calcRMSD -> fast_rmsd (Chromosome.hpp:613-634)
 //
 // The reference keeps these members private; this driver is a white-box test,
 // so it lifts the access specifiers before including the reference header.
@@ -23,6 +30,7 @@
 #define private public
 #define protected public
 #include <LooperSolver.hpp>
+#include <MetricsFramework.hpp>
 #undef private
 #undef protected
 #include <Static.hpp>
@@ -206,6 +214,132 @@ static void mode_score(LooperSolver &s) {
          Settings::weightDistSmooth, Settings::weightAngleSmooth);
 }
 
+// anneal: level seed step_size, then the level's settings
+//   (max_temp dt tjs tjc improvement min_successes steps use2d), then
+//   heatmap: n diag, positions, fixed flags, n*n heatmap_dist
+//   arcs:    n k_stretch k_squeeze, positions, n*n heatmap_exp_dist_anchor
+//   smooth:  n k_stretch k_squeeze k_angular w_dist w_angle, positions,
+//            fixed flags, dist_to_next
+// Prints the returned score, the number of moves and the final positions.
+static void mode_anneal(LooperSolver &s) {
+  const int level = geti();
+  const unsigned seed = static_cast<unsigned>(std::stoul(tok()));
+  const float step = getf();
+  const float maxT = getf(), dt = getf(), tjs = getf(), tjc = getf(), impr = getf();
+  const int min_succ = geti(), steps = geti();
+  Settings::use2D = geti() != 0;
+  int n = geti();
+  if (level == 1) {
+    Settings::maxTempHeatmap = maxT;
+    Settings::dtTempHeatmap = dt;
+    Settings::tempJumpScaleHeatmap = tjs;
+    Settings::tempJumpCoefHeatmap = tjc;
+    Settings::MCstopConditionImprovementHeatmap = impr;
+    Settings::MCstopConditionMinSuccessesHeatmap = min_succ;
+    Settings::MCstopConditionStepsHeatmap = steps;
+  } else if (level == 2) {
+    Settings::maxTemp = maxT;
+    Settings::dtTemp = dt;
+    Settings::tempJumpScale = tjs;
+    Settings::tempJumpCoef = tjc;
+    Settings::MCstopConditionImprovement = impr;
+    Settings::MCstopConditionMinSuccesses = min_succ;
+    Settings::MCstopConditionSteps = steps;
+  } else {
+    Settings::maxTempSmooth = maxT;
+    Settings::dtTempSmooth = dt;
+    Settings::tempJumpScaleSmooth = tjs;
+    Settings::tempJumpCoefSmooth = tjc;
+    Settings::MCstopConditionImprovementSmooth = impr;
+    Settings::MCstopConditionMinSuccessesSmooth = min_succ;
+    Settings::MCstopConditionStepsSmooth = steps;
+    Settings::useCTCFMotifOrientation = false;
+  }
+  int diag = 1;
+  if (level == 1) diag = geti();
+  if (level == 2) {
+    Settings::springConstantStretchArcs = getf();
+    Settings::springConstantSqueezeArcs = getf();
+  }
+  if (level == 3) {
+    Settings::springConstantStretch = getf();
+    Settings::springConstantSqueeze = getf();
+    Settings::springAngularConstant = getf();
+    Settings::weightDistSmooth = getf();
+    Settings::weightAngleSmooth = getf();
+  }
+  s.clusters.assign(n, Cluster());
+  s.active_region.clear();
+  for (int i = 0; i < n; ++i) {
+    float x = getf(), y = getf(), z = getf();
+    s.clusters[i].pos.set(x, y, z);
+    s.active_region.push_back(i);
+  }
+  if (level != 2)
+    for (int i = 0; i < n; ++i) s.clusters[i].is_fixed = geti() != 0;
+  if (level == 1) {
+    s.heatmap_dist.init(n);
+    s.heatmap_dist.diagonal_size = diag;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) s.heatmap_dist.v[i][j] = getf();
+    s.heatmap_chromosome_boundaries.clear();
+  } else if (level == 2) {
+    s.heatmap_exp_dist_anchor.init(n);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) s.heatmap_exp_dist_anchor.v[i][j] = getf();
+    s.loop_constraints.clear();
+  } else {
+    for (int i = 0; i < n; ++i) s.clusters[i].dist_to_next = getf();
+  }
+  srand(seed);
+  const long long before = s.total_mc_steps;
+  double score = 0.0;
+  if (level == 1)
+    score = s.MonteCarloHeatmap(step);
+  else if (level == 2)
+    score = s.MonteCarloArcs(step);
+  else
+    score = s.MonteCarloArcsSmooth(step, false);
+  printf("\n@@anneal %a %lld\n", score, static_cast<long long>(s.total_mc_steps - before));
+  for (int i = 0; i < n; ++i) {
+    const vector3 &p = s.clusters[i].pos;
+    printf("%a %a %a\n", p.x, p.y, p.z);
+  }
+}
+
+// metrics: n h, two structures (n xyz each), two n*n maps, a step for
+// distanceCorrelation.
+static void mode_metrics() {
+  const int n = geti(), h = geti();
+  Chromosome a, b;
+  for (Chromosome *c : {&a, &b}) {
+    for (int i = 0; i < n; ++i) {
+      float x = getf(), y = getf(), z = getf();
+      c->points.push_back(vector3(x, y, z));
+    }
+    c->size = n;
+  }
+  Heatmap ha, hb;
+  for (Heatmap *m : {&ha, &hb}) {
+    m->init(n);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) m->v[i][j] = getf();
+  }
+  const int step = geti();
+  std::vector<float> fa, fb;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      fa.push_back(ha.v[i][j]);
+      fb.push_back(hb.v[i][j]);
+    }
+  MetricsFramework mf;
+  const float pear = mf.pearsonCorrelation(fa, fb);
+  const float dcor = mf.distanceCorrelation(a, b, step);
+  const float scc = mf.structuralSimilarity(ha, hb, h);
+  const float rmsd = a.calcRMSD(b);
+  printf("\n@@metrics %a %a %a %a\n", pear, dcor, scc, rmsd);
+}
+
 int main() {
   Settings settings;
   settings.init();
@@ -242,6 +376,10 @@ int main() {
     mode_distheat(s);
   } else if (mode == "score") {
     mode_score(s);
+  } else if (mode == "anneal") {
+    mode_anneal(s);
+  } else if (mode == "metrics") {
+    mode_metrics();
   } else {
     fprintf(stderr, "unknown mode %s\n", mode.c_str());
     return 2;

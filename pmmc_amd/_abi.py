@@ -230,6 +230,11 @@ SIGNATURES = {
     "pmmc_heatmap_level": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, f32p, C.c_int32,
                                      C.POINTER(LevelParams), C.POINTER(HeatmapLevelOpts), C.c_uint64,
                                      C.c_uint32, C.POINTER(LevelReport)]),
+    "pmmc_pearson": (C.c_float, [f32p, f32p, C.c_int64]),
+    "pmmc_distance_correlation": (C.c_float, [f32p, f32p, C.c_int32, C.c_int32]),
+    "pmmc_distance_map": (C.c_int, [f32p, C.c_int32, f32p]),
+    "pmmc_scc": (C.c_float, [f32p, f32p, C.c_int32, C.c_int32]),
+    "pmmc_rmsd": (C.c_float, [f32p, f32p, C.c_int32]),
 }
 
 _lib = None

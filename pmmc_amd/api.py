@@ -255,8 +255,10 @@ class Context:
 
     # ---- batched ----
     def anneal_batch(self, problems, schedules, seeds, segments=None, n_chains=0,
-                     blocks_per_step=0, steps_per_graph=0, use_graph=True):
-        """Run many problems concurrently; returns list of (pos, stats)."""
+                     blocks_per_step=0, steps_per_graph=0, use_graph=True, chain0s=None):
+        """Run many problems concurrently; returns list of (pos, stats).
+        Job i uses seed seeds[i], segment segments[i] (default i) and chain ids
+        chain0s[i] (default 0) onwards."""
         k = len(problems)
         jobs = (A.Job * k)()
         keep, outs, stats = [], [], []
@@ -271,7 +273,8 @@ class Context:
             nc = n_chains[i] if isinstance(n_chains, (list, tuple, np.ndarray)) else n_chains
             sch = schedules[i] if isinstance(schedules, (list, tuple)) else schedules
             jobs[i] = A.Job(p.level, C.cast(C.pointer(cp), C.c_void_p), sch,
-                            A.Run(int(seeds[i]), int(seg), 0, int(nc)),
+                            A.Run(int(seeds[i]), int(seg), 0 if chain0s is None else int(chain0s[i]),
+                                  int(nc)),
                             o.ctypes.data_as(A.f32p), C.pointer(s))
         opts = A.BatchOpts(blocks_per_step, steps_per_graph, 1 if use_graph else 0)
         self._check(self.lib.pmmc_anneal_batch(self.h, jobs, k, C.byref(opts)))

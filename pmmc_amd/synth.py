@@ -151,3 +151,65 @@ def batch_arcs_problems(n_segments: int = 32, n_seeds: int = 16, seed: int = 0,
             probs.append(Arcs(pos=pos.astype(np.float32), exp_dist=base.exp_dist))
             meta.append((s, e, n))
     return probs, meta
+
+
+def heatmap_avg(h: np.ndarray) -> float:
+    """Heatmap::getAvg (Heatmap.hpp:440-451): FP32 sequential mean of the
+    upper triangle from the first diagonal holding a value > 1e-6
+    (getDiagonalSize, :356-367)."""
+    h = np.asarray(h, np.float32)
+    n = h.shape[0]
+    w0 = 0
+    for w in range(n):
+        if np.any(np.diagonal(h, w) > 1e-6):
+            w0 = w
+            break
+    seq = h[np.triu_indices(n, w0)]
+    if seq.size == 0:
+        return 0.0
+    return float(np.float32(np.cumsum(seq, dtype=np.float32)[-1] / np.float32(seq.size)))
+
+
+# Annealing settings of auxiliary/configs/default.ini per level
+# ([simulation_heatmap], [simulation_arcs], and [simulation_arcs_smooth] with
+# the 200k-move milestone and 0.9995 cooling of the 20,000-bead smooth case).
+CONFIG_SETTINGS = {
+    1: dict(max_temp=5.0, dt_temp=0.9999, tjs=50.0, tjc=20.0, improvement=0.99,
+            min_successes=10, steps=50000),
+    2: dict(max_temp=5.0, dt_temp=0.9999, tjs=50.0, tjc=20.0, improvement=0.975,
+            min_successes=100, steps=50000),
+    3: dict(max_temp=5.0, dt_temp=0.9995, tjs=50.0, tjc=20.0, improvement=0.99,
+            min_successes=50, steps=200000),
+}
+
+
+def config_case(level: int, seed: int, smooth_anchors: int = 400, smooth_add: int = 49):
+    """One seeded instance of the plain workloads, as the reference level
+    drivers would hand it to the annealer: (problem, step_size, truth).
+
+    level 1: 64-bead heatmap problem (init U[0,10)^3 from `seed`, target from a
+             random walk), step = getAvg(h) * noise_lvl (0.5), as
+             reconstructClustersHeatmapSingleLevel (LooperSolver.hpp:1263-1270);
+             truth = the random-walk polymer.
+    level 2: the 400-anchor / 1,200-arc segment (targets fixed, initial
+             positions from `seed`), step = noise_arcs (0.01) x mean
+             genomicLengthToDistance of consecutive anchors (:4339-4354);
+             truth = None (no ground-truth structure: the arcs are random).
+    level 3: smooth_anchors fixed anchors with smooth_add beads between
+             consecutive ones (20,000 beads for 400 x 49), dist_to_next = 1,
+             step = noise_smooth (5.0) x mean dist_to_next (:4351-4354);
+             truth = None."""
+    if level == 1:
+        prob, truth = heatmap_problem(n=64, seed=seed)
+        return prob, float(np.float32(heatmap_avg(prob.dist) * np.float32(0.5))), truth
+    if level == 2:
+        anchors, _, _ = arcs_inputs(n=400, n_arcs=1200, seed=0)
+        prob = arcs_problem(n=400, n_arcs=1200, seed=0, init_seed=1000 + seed)
+        g = anchors.mean(axis=1)
+        d = 1.0 + 0.5 * np.power(np.diff(g) / 1000.0, 0.75)
+        return prob, float(np.float32(d.sum() / len(anchors) * 0.01)), None
+    if level == 3:
+        prob = smooth_problem(n_anchors=smooth_anchors, add=smooth_add, seed=seed)
+        n = len(prob.pos)
+        return prob, float(np.float32(np.float32(n - 1) / np.float32(n) * np.float32(5.0))), None
+    raise ValueError(level)

@@ -1,0 +1,44 @@
+"""The product's annealers on the GPU agree with the reference's own CPU
+annealers over 16 seeds, within the bands stated in tests/band_common.py:
+final energy, moves, and for the heatmap level the distance-map Pearson, SCC
+and RMSD to the synthetic ground truth (computed with the product's metrics).
+
+Each (seed, chain) sample is one single-chain job of one pmmc_anneal_batch
+launch per level (CPU-preset schedule, seed 1 + s, segment s, chain 64*k);
+the first seed's chains are also checked bit-for-bit against the oracle."""
+import numpy as np
+import pytest
+
+import band_common as B
+from oracle import binding as O
+from pmmc_amd import metrics as PM
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("level", [1, 2, 3])
+def test_product_annealing_within_band_of_reference(ctx, level):
+    jobs = B.jobs(level)
+    cases = {r["seed"]: B.case(level, r["seed"]) for r in B.REF[str(level)]}
+    print(f"band level {level}: {len(jobs)} single-chain jobs", flush=True)
+    res = ctx.anneal_batch([cases[s][0] for s, _ in jobs], [cases[s][1] for s, _ in jobs],
+                           seeds=[1 + s for s, _ in jobs], segments=[s for s, _ in jobs],
+                           chain0s=[c for _, c in jobs], n_chains=1)
+    results = {}
+    for (s, chain), (pos, st) in zip(jobs, res):
+        prob, _, truth = cases[s]
+        rec = {"energy": st["energy"], "moves": st["moves"]}
+        rec.update(B.quality(level, pos, prob, truth, PM))
+        results[(s, chain)] = rec
+    stats = B.compare(level, results)
+    print(level, stats, flush=True)
+    # bit-exact against the oracle on the first seed's chains (2 for smooth)
+    s0 = jobs[0][0]
+    prob, sched, _ = cases[s0]
+    P = O.Problem(prob)
+    for k, ((s, chain), (pos, st)) in enumerate(zip(jobs, res)):
+        if s != s0 or k >= (2 if level == 3 else B.K):
+            continue
+        opos, ost = P.anneal(O.schedule_from(sched), 1 + s, s, chain)
+        assert st["moves"] == ost["moves"]
+        assert np.array_equal(pos, opos)
