@@ -4,8 +4,12 @@ final energy, moves, and for the heatmap level the distance-map Pearson, SCC
 and RMSD to the synthetic ground truth (computed with the product's metrics).
 
 Each (seed, chain) sample is one single-chain job of one pmmc_anneal_batch
-launch per level (CPU-preset schedule, seed 1 + s, segment s, chain 64*k);
-the first seed's chains are also checked bit-for-bit against the oracle."""
+launch per level (CPU-preset schedule, seed 1 + s, segment s, chain 64*k).
+For the heatmap level the first seed's chains are also checked bit-for-bit
+against the oracle.  The arc and smooth chains here run 0.5 M and 7.5 M moves
+into convergence, where the kernels' FP32 ΔE and the oracle's FP64 ΔE
+eventually decide a near-tie differently and the trajectories part; their
+bit-exactness is pinned on shorter chains (test_gpu_parity.py)."""
 import numpy as np
 import pytest
 
@@ -32,12 +36,14 @@ def test_product_annealing_within_band_of_reference(ctx, level):
         results[(s, chain)] = rec
     stats = B.compare(level, results)
     print(level, stats, flush=True)
-    # bit-exact against the oracle on the first seed's chains (2 for smooth)
+    if level != 1:
+        return
+    # bit-exact against the oracle on the first seed's chains
     s0 = jobs[0][0]
     prob, sched, _ = cases[s0]
     P = O.Problem(prob)
     for k, ((s, chain), (pos, st)) in enumerate(zip(jobs, res)):
-        if s != s0 or k >= (2 if level == 3 else B.K):
+        if s != s0:
             continue
         opos, ost = P.anneal(O.schedule_from(sched), 1 + s, s, chain)
         assert st["moves"] == ost["moves"]
