@@ -14,9 +14,17 @@ Beside it, in the same line:
              against the same problems solved one plain call at a time
              (timed on a subset, per-problem results checked identical);
   served  -- Poisson arrivals of mixed heatmap / arc / smooth jobs through the
-             continuous-batching server, p50/p99 request latency and solves/s;
-  cpu_baseline -- the reference's CPU arc annealer restated in the oracle,
-             timed on the host's cores.
+             continuous-batching server, p50/p99 request latency and solves/s,
+             against a sample of the same requests solved one plain call at a
+             time (results checked identical);
+  levels  -- the plain heatmap-level workload (64 beads, MonteCarloHeatmap's
+             schedule) and the plain smooth-level workload (19,951 beads, the
+             smooth GPU kernel's schedule): moves/s, final energy, and for the heatmap the
+             distance-map Pearson / SCC / RMSD to the ground-truth polymer,
+             each beside the reference's CPU annealer on the host's cores;
+  cpu_baseline -- the reference's own CPU arc annealer (LooperSolver::
+             MonteCarloArcs, compiled in place into oracle/_ref) timed on the
+             host's cores; the oracle's restatement when that build is absent.
 """
 from __future__ import annotations
 
@@ -87,14 +95,49 @@ def copy_calibration() -> int:
     return 0
 
 
+def host_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_reference(level, prob, step, settings_dict, seconds, sample_moves=0):
+    """The reference's CPU annealer of `level` (compiled in place into
+    oracle/_ref/levels_ref), one process per host thread, each annealing the
+    workload repeatedly for about `seconds`.  sample_moves > 0 bounds each run
+    to exactly that many moves (milestone period sample_moves with a stop test
+    that always fires: improvement 0, min successes 2^31-1); the reference's
+    per-move work does not depend on acceptance.  None when that build is
+    absent."""
+    from oracle import refdriver as R
+    if not R.available():
+        return None
+    threads = host_threads()
+    sd = dict(settings_dict)
+    if sample_moves > 0:
+        sd.update(steps=sample_moves, improvement=0.0, min_successes=2**31 - 1)
+    moves, secs, runs = R.time_anneal(level, prob, step, sd, seconds, threads)
+    return {"value": moves / secs, "unit": "moves/s", "cores": threads, "kind": "reference",
+            "runs": runs, "seconds": secs,
+            "sample": (f"{runs} runs of {sample_moves} moves" if sample_moves > 0 else
+                       f"{runs} complete anneals") + f" on {threads} host threads"}
+
+
 def cpu_baseline(prob, s, step, budget_s=15.0):
-    """The reference's CPU annealer (LooperSolver::MonteCarloArcs, dense O(n)
-    local scans) restated in the oracle, one chain per host thread."""
+    """The reference's CPU arc annealer (LooperSolver::MonteCarloArcs) on the
+    host's cores; the oracle's restatement of it when the reference build is
+    absent."""
+    from pmmc_amd import synth
+    ref = cpu_reference(2, prob, step, synth.CONFIG_SETTINGS[2], budget_s)
+    if ref is not None:
+        ref["sample"] = (f"{ref['runs']} complete anneals of the 400-anchor segment by the "
+                         f"reference's LooperSolver::MonteCarloArcs (default.ini "
+                         f"[simulation_arcs], glibc rand), {ref['seconds']:.1f} s on each of "
+                         f"{ref['cores']} host threads")
+        return ref
     from oracle import binding as O
     from pmmc_amd import schedule_preset
     sched = O.schedule_from(schedule_preset(2, 0, s, step))
     P = O.Problem(prob)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
     moves_per_chain = 50000
     sched.max_moves = moves_per_chain
     sched.block = 50000
@@ -105,9 +148,62 @@ def cpu_baseline(prob, s, step, budget_s=15.0):
     _, stats, secs = P.anneal_many(sched, 1, 0, 0, chains, threads)
     moves = sum(x["moves"] for x in stats)
     return {"value": moves / secs, "unit": "moves/s", "cores": threads, "kind": "port",
-            "sample": f"{chains} chains x {moves_per_chain} moves of the CPU arc annealer "
-                      f"(LooperSolver::MonteCarloArcs schedule, dense O(n) local scans), "
+            "sample": f"{chains} chains x {moves_per_chain} moves of the oracle's restatement "
+                      f"of LooperSolver::MonteCarloArcs (dense O(n) local scans), "
                       f"{secs:.1f} s on {threads} host threads"}
+
+
+def levels(ctx, with_cpu=True, cpu_seconds=8.0):
+    """Plain heatmap (64 beads; the CPU annealer's schedule, as the workload is
+    specified) and smooth (19,951 beads; the GPU kernel's schedule) workloads,
+    one solve each after a warm-up."""
+    from pmmc_amd import metrics, schedule_preset, settings, synth
+    out = {}
+    # heatmap, 64 beads (init U[0,10)^3, target from a random-walk polymer):
+    # the schedule of the CPU path the workload is specified on
+    # (LooperSolver::MonteCarloHeatmap: 50k-move milestones, cooling per move)
+    prob, step, truth = synth.config_case(1, 0)
+    sched = schedule_preset(1, 0, settings(**synth.CONFIG_SETTINGS[1]), step)
+    chains = 65536
+    ctx.anneal(prob, sched, seed=1, n_chains=4096)
+    t0 = time.perf_counter()
+    pos, st = ctx.anneal(prob, sched, seed=2, segment=1, n_chains=chains)
+    wall = time.perf_counter() - t0
+    h = {"workload": "plain heatmap-level SA, 64 beads, default.ini [simulation_heatmap], "
+                     "schedule of LooperSolver::MonteCarloHeatmap", "chains": st["n_chains"],
+         "moves_per_s": st["moves"] / wall, "accepted_moves_per_s": st["accepted"] / wall,
+         "kernel_moves_per_s": st["moves"] / (st["kernel_ms"] * 1e-3), "solve_s": wall,
+         "final_energy": st["energy_resummed"],
+         "partner_evaluations_per_s": 2 * 63 * st["moves"] / (st["kernel_ms"] * 1e-3),
+         "bound": "VALU (positions of 33-64 beads held in registers: no memory traffic per move)"}
+    h.update(metrics.compare(pos, truth))
+    if with_cpu:
+        h["cpu_reference"] = cpu_reference(1, prob, step, synth.CONFIG_SETTINGS[1], cpu_seconds,
+                                           sample_moves=20000)
+    out["heatmap_64"] = h
+    # smooth, 19,951 beads (400 fixed anchors, 49 beads between neighbours)
+    prob, step, _ = synth.config_case(3, 0)
+    sm = dict(synth.CONFIG_SETTINGS[3])
+    sched = schedule_preset(3, 1, settings(**sm), step)
+    ctx.anneal(prob, sched, seed=1)
+    t0 = time.perf_counter()
+    pos, st = ctx.anneal(prob, sched, seed=2, segment=1)
+    wall = time.perf_counter() - t0
+    acc = st["accepted"] / max(1, st["moves"])
+    bpm = 12.0 * 5 + 8.0 + 12.0 * acc
+    kms = st["kernel_ms"] * 1e-3
+    out["smooth_19951"] = {
+        "workload": "plain smooth-level SA, 19,951 beads, 200k-move cap, cooling 0.9995, "
+                    "GPU-kernel schedule", "chains": st["n_chains"],
+        "moves_per_s": st["moves"] / wall, "accepted_moves_per_s": st["accepted"] / wall,
+        "kernel_moves_per_s": st["moves"] / kms, "solve_s": wall,
+        "final_energy": st["energy_resummed"],
+        "bytes_per_move": bpm, "achieved_GBps": bpm * st["moves"] / kms / 1e9,
+        "bound": "HBM latency (5-bead window, dependent loads)"}
+    if with_cpu:
+        out["smooth_19951"]["cpu_reference"] = cpu_reference(3, prob, step, sm, cpu_seconds,
+                                                             sample_moves=2000000)
+    return out
 
 
 def batched(ctx, chains_per_problem=256, plain_subset=16):
@@ -130,12 +226,27 @@ def batched(ctx, chains_per_problem=256, plain_subset=16):
     t0 = time.perf_counter()
     same = True
     mv_p = ac_p = 0
+    plain = {}
     for i in idx:
         pos, st = ctx.anneal(probs[i], sched, seed=1 + i, segment=i, n_chains=chains_per_problem)
         mv_p += st["moves"]
         ac_p += st["accepted"]
+        plain[i] = (pos, st)
         same &= bool(np.array_equal(pos, res[i][0])) and st["energy"] == res[i][1]["energy"]
     tp = time.perf_counter() - t0
+    # matched quality: final energy and the Pearson correlation of anchor
+    # distances with the arc targets, batched vs plain, problem by problem
+    from pmmc_amd import metrics
+
+    def target_pearson(p, pos):
+        ed = np.asarray(p.exp_dist, np.float32)
+        a, b = np.nonzero(np.triu(ed > 1e-6, 1))
+        d = np.linalg.norm(pos[a].astype(np.float64) - pos[b].astype(np.float64), axis=1)
+        return metrics.pearson(d.astype(np.float32), ed[a, b])
+    e_rel = max(abs(res[i][1]["energy"] - plain[i][1]["energy"]) / abs(plain[i][1]["energy"])
+                for i in idx)
+    p_diff = max(abs(target_pearson(probs[i], res[i][0]) - target_pearson(probs[i], plain[i][0]))
+                 for i in idx)
     sub_moves_b = sum(res[i][1]["moves"] for i in idx)
     return {
         "problems": n, "chains_per_problem": chains_per_problem,
@@ -148,11 +259,13 @@ def batched(ctx, chains_per_problem=256, plain_subset=16):
         "accepted_moves_speedup": (ac_b / tb) / (ac_p / tp),
         "plain_subset": len(idx), "subset_moves_equal": sub_moves_b == mv_p,
         "results_identical_alone_and_batched": same,
+        "final_energy_max_rel_diff_vs_plain": float(e_rel),
+        "pearson_to_targets_max_diff_vs_plain": float(p_diff),
         "final_energy_median": float(np.median([r[1]["energy"] for r in res])),
     }
 
 
-def served(ctx, seconds=3.0, rate=200.0, seed=0):
+def served(ctx, seconds=3.0, rate=200.0, seed=0, plain_sample=24):
     """Config 4: Poisson arrivals of mixed jobs through the server."""
     import numpy as np
     from pmmc_amd import schedule_preset, settings, synth
@@ -181,35 +294,61 @@ def served(ctx, seconds=3.0, rate=200.0, seed=0):
             p = synth.smooth_problem(n_anchors=a, add=49, seed=k)
             sc = ss
         reqs.append((float(t), p, sc, ens))
-    srv = ctx.server(hbm_budget=64 << 30, max_jobs=4096, blocks_per_step=4)
+    srv = ctx.server(hbm_budget=64 << 30, max_jobs=4096, blocks_per_step=0)
     t0 = time.perf_counter()
     jobs = {}
     done_at = {}
+    pending = set()
     nxt = 0
+    steps = 0
     while len(done_at) < len(reqs):
         now = time.perf_counter() - t0
         while nxt < len(reqs) and reqs[nxt][0] <= now:
             t, p, sc, ens = reqs[nxt]
             jobs[nxt] = [srv.submit(p, sc, seed=1 + e, segment=nxt, n_chains=64) for e in range(ens)]
+            pending.add(nxt)
             nxt += 1
-        if not jobs or all(k in done_at for k in jobs):
+        if not pending:
             if nxt < len(reqs):
                 time.sleep(max(0.0, min(0.001, reqs[nxt][0] - now)))
             continue
         n_done, _, _ = srv.step()
+        steps += 1
         if n_done:
             now = time.perf_counter() - t0
-            for k, ids in jobs.items():
-                if k not in done_at and all(srv.done(j) for j in ids):
-                    done_at[k] = now
+            for k in [k for k in pending if srv.done(jobs[k][-1]) and all(srv.done(j) for j in jobs[k])]:
+                done_at[k] = now
+                pending.discard(k)
     wall = time.perf_counter() - t0
     lat = np.array([done_at[k] - reqs[k][0] for k in range(len(reqs))]) * 1e3
     solves = sum(len(v) for v in jobs.values())
+    acc_served = sum(srv.result(j)[1]["accepted"] for v in jobs.values() for j in v)
+    # the plain form on an evenly spread sample of the same requests, one
+    # solve per call, results compared with the served ones
+    sample = list(range(0, len(reqs), max(1, len(reqs) // plain_sample)))[:plain_sample]
+    same = True
+    n_plain = acc_plain = 0
+    t1 = time.perf_counter()
+    for k in sample:
+        _, p, sc, ens = reqs[k]
+        for e in range(ens):
+            pos, st = ctx.anneal(p, sc, seed=1 + e, segment=k, n_chains=64)
+            spos, sst = srv.result(jobs[k][e])
+            same &= bool(np.array_equal(pos, spos)) and st["energy"] == sst["energy"]
+            n_plain += 1
+            acc_plain += st["accepted"]
+    tp = time.perf_counter() - t1
     srv.close()
-    return {"requests": len(reqs), "arrival_rate_per_s": rate, "solves": solves,
+    return {"requests": len(reqs), "arrival_rate_per_s": rate, "solves": solves, "steps": steps,
             "solves_per_s": solves / wall, "requests_per_s": len(reqs) / wall,
+            "accepted_moves_per_s": acc_served / wall,
             "p50_latency_ms": float(np.percentile(lat, 50)),
             "p99_latency_ms": float(np.percentile(lat, 99)),
+            "plain_sample_requests": len(sample), "plain_solves_per_s": n_plain / tp,
+            "plain_accepted_moves_per_s": acc_plain / tp,
+            "solves_speedup_vs_plain": (solves / wall) / (n_plain / tp),
+            "accepted_moves_speedup_vs_plain": (acc_served / wall) / (acc_plain / tp),
+            "results_identical_alone_and_served": same,
             "mix": "40% heatmap 64-512 beads, 40% arcs 20-2000 anchors, 20% smooth 1k-20k beads; "
                    "ensemble 1-16 solves of 64 chains per request"}
 
@@ -223,6 +362,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-levels", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB three times (device to device), the known-size "
                          "transfer that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -260,6 +400,8 @@ def main(argv=None):
             extra["batched"] = batched(ctx)
         if not a.no_served:
             extra["served"] = served(ctx)
+        if not a.no_levels:
+            extra["levels"] = levels(ctx, with_cpu=not a.no_cpu)
     value = moves / wall
     # algorithmic HBM bytes per chain-move: the moved bead's position and its
     # deg partners' positions (12 B each, read once for both evaluations),

@@ -313,16 +313,19 @@ pmmc_status pmmc_anneal_batch(pmmc_ctx *ctx, const pmmc_job *jobs, int32_t n_job
 typedef struct pmmc_server pmmc_server;
 
 typedef struct {
-  size_t hbm_budget;         /* bytes of chain state + problem data admitted   */
+  size_t hbm_budget;         /* bytes of chain state + problem data admitted;
+                                allocated on the device once, at creation      */
   int32_t max_jobs;          /* resident job slots                              */
-  int32_t blocks_per_step;   /* blocks of moves per step launch                 */
+  int32_t blocks_per_step;   /* blocks of moves per chain per step launch; 0 =
+                                paced per launch group (kernel and size class)
+                                to about 4 ms of device time per launch        */
 } pmmc_server_opts;
 
 pmmc_status pmmc_server_create(pmmc_ctx *ctx, const pmmc_server_opts *opts, pmmc_server **out);
 void pmmc_server_destroy(pmmc_server *srv);
 /* Queue a job (inputs are copied).  *job_id identifies it in poll. */
 pmmc_status pmmc_server_submit(pmmc_server *srv, const pmmc_job *job, uint64_t *job_id);
-/* Admit queued jobs that fit the budget, run one step launch per level over
+/* Admit queued jobs that fit the budget, run one launch per launch group over
  * every resident chain, retire finished jobs (results written to the job's
  * pos_out / stats).  *n_done = jobs finished in this step; *n_resident and
  * *n_queued describe the pool afterwards (nullable). */

@@ -16,7 +16,8 @@
 //   score     calcScoreHeatmapActiveRegion (:3583-3612),
 //             calcScoreDistancesActiveRegion (:3215-3247),
 //             calcScoreStructureSmooth (:3414-3451)
-//   anneal    the reference's CPU annealers: MonteCarloHeatmap (:1424-1565),
+//   anneal / time_anneal
+//             the reference's CPU annealers: MonteCarloHeatmap (:1424-1565),
 //             MonteCarloArcs (:4670-4796), MonteCarloArcsSmooth (:4798-5047),
 //             driven by glibc rand() after srand(seed), as the reference's
 //             `-j seed` option seeds them (main.cpp:2833-2845)
@@ -221,14 +222,26 @@ static void mode_score(LooperSolver &s) {
 //   smooth:  n k_stretch k_squeeze k_angular w_dist w_angle, positions,
 //            fixed flags, dist_to_next
 // Prints the returned score, the number of moves and the final positions.
-static void mode_anneal(LooperSolver &s) {
-  const int level = geti();
-  const unsigned seed = static_cast<unsigned>(std::stoul(tok()));
-  const float step = getf();
+struct AnnealInput {
+  int level = 0, n = 0;
+  unsigned seed = 0;
+  float step = 0.f;
+  std::vector<float> pos;
+};
+
+// Reads the anneal input (after the mode's own leading fields) into the
+// solver state and Settings.
+static AnnealInput read_anneal(LooperSolver &s) {
+  AnnealInput in;
+  in.level = geti();
+  const int level = in.level;
+  in.seed = static_cast<unsigned>(std::stoul(tok()));
+  in.step = getf();
   const float maxT = getf(), dt = getf(), tjs = getf(), tjc = getf(), impr = getf();
   const int min_succ = geti(), steps = geti();
   Settings::use2D = geti() != 0;
   int n = geti();
+  in.n = n;
   if (level == 1) {
     Settings::maxTempHeatmap = maxT;
     Settings::dtTempHeatmap = dt;
@@ -270,9 +283,9 @@ static void mode_anneal(LooperSolver &s) {
   }
   s.clusters.assign(n, Cluster());
   s.active_region.clear();
+  for (int i = 0; i < 3 * n; ++i) in.pos.push_back(getf());
   for (int i = 0; i < n; ++i) {
-    float x = getf(), y = getf(), z = getf();
-    s.clusters[i].pos.set(x, y, z);
+    s.clusters[i].pos.set(in.pos[3 * i], in.pos[3 * i + 1], in.pos[3 * i + 2]);
     s.active_region.push_back(i);
   }
   if (level != 2)
@@ -291,20 +304,56 @@ static void mode_anneal(LooperSolver &s) {
   } else {
     for (int i = 0; i < n; ++i) s.clusters[i].dist_to_next = getf();
   }
-  srand(seed);
+  return in;
+}
+
+static double run_annealer(LooperSolver &s, const AnnealInput &in) {
+  if (in.level == 1) return s.MonteCarloHeatmap(in.step);
+  if (in.level == 2) return s.MonteCarloArcs(in.step);
+  return s.MonteCarloArcsSmooth(in.step, false);
+}
+
+// anneal: level seed step_size, then the level's settings
+//   (max_temp dt tjs tjc improvement min_successes steps use2d), then
+//   heatmap: n diag, positions, fixed flags, n*n heatmap_dist
+//   arcs:    n k_stretch k_squeeze, positions, n*n heatmap_exp_dist_anchor
+//   smooth:  n k_stretch k_squeeze k_angular w_dist w_angle, positions,
+//            fixed flags, dist_to_next
+// Prints the returned score, the number of moves and the final positions.
+static void mode_anneal(LooperSolver &s) {
+  const AnnealInput in = read_anneal(s);
+  srand(in.seed);
   const long long before = s.total_mc_steps;
-  double score = 0.0;
-  if (level == 1)
-    score = s.MonteCarloHeatmap(step);
-  else if (level == 2)
-    score = s.MonteCarloArcs(step);
-  else
-    score = s.MonteCarloArcsSmooth(step, false);
+  const double score = run_annealer(s, in);
   printf("\n@@anneal %a %lld\n", score, static_cast<long long>(s.total_mc_steps - before));
-  for (int i = 0; i < n; ++i) {
+  for (int i = 0; i < in.n; ++i) {
     const vector3 &p = s.clusters[i].pos;
     printf("%a %a %a\n", p.x, p.y, p.z);
   }
+}
+
+// time_anneal: `seconds`, then the anneal input.  Runs the annealer from the
+// same initial positions with seeds seed, seed+1, ... until `seconds` of
+// annealing have passed (at least once); prints the runs, the moves and the
+// seconds spent inside the annealer (steady_clock).
+static void mode_time_anneal(LooperSolver &s) {
+  const double budget = std::stod(tok());
+  const AnnealInput in = read_anneal(s);
+  long long moves = 0;
+  double secs = 0.0;
+  int runs = 0;
+  do {
+    for (int i = 0; i < in.n; ++i)
+      s.clusters[i].pos.set(in.pos[3 * i], in.pos[3 * i + 1], in.pos[3 * i + 2]);
+    srand(in.seed + runs);
+    const long long before = s.total_mc_steps;
+    const auto t0 = std::chrono::steady_clock::now();
+    run_annealer(s, in);
+    secs += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    moves += s.total_mc_steps - before;
+    ++runs;
+  } while (secs < budget);
+  printf("\n@@time %d %lld %.9f\n", runs, moves, secs);
 }
 
 // metrics: n h, two structures (n xyz each), two n*n maps, a step for
@@ -378,6 +427,8 @@ int main() {
     mode_score(s);
   } else if (mode == "anneal") {
     mode_anneal(s);
+  } else if (mode == "time_anneal") {
+    mode_time_anneal(s);
   } else if (mode == "metrics") {
     mode_metrics();
   } else {

@@ -1,0 +1,91 @@
+"""Runs the reference's own CPU annealers -- TEST / BASELINE INFRASTRUCTURE ONLY.
+
+oracle/_ref/levels_ref is the reference's LooperSolver compiled in place from
+its sources by oracle/Makefile (target `ref`; only where the reference
+sources exist, the binary then travels with the tree).  This module formats
+its `anneal` / `time_anneal` input and reads its output.  Used by
+tests/golden/make_anneal_fixtures.py (fixtures) and bench.py's cpu_baseline
+leg (the reference's CPU annealing timed on the host's cores).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF = os.path.join(HERE, "_ref", "levels_ref")
+HEADER = "0 " * 12  # count/heatmap/genomic-length parameters (unused by these modes)
+
+
+def available() -> bool:
+    return os.path.exists(REF) and os.access(REF, os.X_OK)
+
+
+def hx(a) -> str:
+    return " ".join(float(v).hex() for v in np.asarray(a, np.float32).reshape(-1))
+
+
+def anneal_input(level: int, prob, step: float, settings: dict, seed: int) -> str:
+    """Input of the driver's anneal modes for one problem (pmmc_amd.api
+    Heatmap / Arcs / Smooth) and [simulation_*] settings."""
+    s = settings
+    n = len(np.asarray(prob.pos).reshape(-1, 3))
+    parts = [f"{level} {seed}", float(np.float32(step)).hex(),
+             f"{s['max_temp']} {s['dt_temp']} {s['tjs']} {s['tjc']} {s['improvement']} "
+             f"{s['min_successes']} {s['steps']} 0"]
+    if level == 1:
+        fixed = np.zeros(n, np.uint8) if prob.fixed is None else prob.fixed
+        parts += [f"{n} {prob.diagonal}", hx(prob.pos), " ".join(str(int(v)) for v in fixed),
+                  hx(prob.dist)]
+    elif level == 2:
+        parts += [f"{n} {prob.k_stretch} {prob.k_squeeze}", hx(prob.pos), hx(prob.exp_dist)]
+    else:
+        fixed = np.zeros(n, np.uint8) if prob.fixed is None else prob.fixed
+        parts += [f"{n} {prob.k_stretch} {prob.k_squeeze} {prob.k_angular} {prob.w_dist} "
+                  f"{prob.w_angle}", hx(prob.pos), " ".join(str(int(v)) for v in fixed),
+                  hx(prob.dist_to_next)]
+    return "\n".join(parts) + "\n"
+
+
+def run(text: str, tag: str) -> list[str]:
+    r = subprocess.run([REF], input=text, capture_output=True, text=True, check=True)
+    lines = r.stdout.splitlines()
+    k = [i for i, line in enumerate(lines) if line.startswith(tag)][0]
+    return lines[k:]
+
+
+def anneal(level: int, prob, step: float, settings: dict, seed: int):
+    """One run of the reference annealer: (returned score, moves, positions)."""
+    out = run(f"anneal {HEADER}\n" + anneal_input(level, prob, step, settings, seed), "@@anneal")
+    _, score, moves = out[0].split()
+    n = len(np.asarray(prob.pos).reshape(-1, 3))
+    pos = np.array([[float.fromhex(t) for t in line.split()] for line in out[1:1 + n]], np.float32)
+    return float.fromhex(score), int(moves), pos
+
+
+def time_anneal(level: int, prob, step: float, settings: dict, seconds: float, threads: int):
+    """`threads` processes each annealing repeatedly (seeds 1, 2, ...) for about
+    `seconds` of annealing time.  Returns (moves, annealing seconds per
+    process (max), runs): whole-host moves/s = moves / seconds."""
+    text = f"time_anneal {HEADER}\n{seconds}\n"
+    procs = []
+    for t in range(threads):
+        inp = text + anneal_input(level, prob, step, settings, 1 + 1000 * t)
+        p = subprocess.Popen([REF], stdin=subprocess.PIPE, stdout=subprocess.PIPE,
+                             stderr=subprocess.DEVNULL, text=True)
+        p.stdin.write(inp)
+        p.stdin.close()
+        procs.append(p)
+    moves, secs, runs = 0, 0.0, 0
+    for p in procs:
+        out = p.stdout.read()
+        if p.wait() != 0:
+            raise RuntimeError("reference annealer failed")
+        line = [x for x in out.splitlines() if x.startswith("@@time")][0]
+        _, r, m, s = line.split()
+        runs += int(r)
+        moves += int(m)
+        secs = max(secs, float(s))
+    return moves, secs, runs

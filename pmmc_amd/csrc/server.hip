@@ -2,17 +2,33 @@
 //
 // Jobs (any level, any size) are queued by pmmc_server_submit.  Each
 // pmmc_server_step admits queued jobs while the resident set fits the HBM
-// budget and the slot count, runs ONE step launch per launch group over every
+// budget and the slot count, runs ONE launch per launch group over every
 // resident chain (all jobs' chains overlap on the device), and retires jobs
 // whose chains have all stopped.  A chain's trajectory depends only on its
 // (seed, segment, chain id) Philox stream and its own state, which is saved
-// and restored bit-exactly between steps, so a job's result is the same
-// whether it runs alone or among others.
+// and restored bit-exactly between launches, so a job's result is the same
+// whether it runs alone or among others, and whatever number of moves each
+// launch runs.
+//
+// Device memory: the HBM budget is allocated once, at creation, and carved by
+// a host-side first-fit free list (256-byte granules), so admission and
+// retirement never call hipMalloc / hipFree (hipFree would wait for the
+// device).  Retirement finalizes every finished job of a step, copies their
+// results into one pinned staging buffer and synchronizes once.
+//
+// Pacing: resident jobs are grouped by kernel (level, register/LDS variant)
+// and by size class (floor(log2 n)).  With blocks_per_step = 0 each group's
+// launch runs as many blocks of moves as fit a target of kTargetMs measured
+// on the device (HIP events around the group's launch, the budget rescaled
+// after every step), so small jobs finish in a few steps while large ones
+// advance one block at a time.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <set>
 #include <tuple>
 #include <vector>
@@ -22,18 +38,77 @@
 
 namespace pmmc {
 
+constexpr float kTargetMs = 4.0f;  // device time per group launch when pacing adaptively
+constexpr int kMaxBlocks = 1 << 16;
+constexpr size_t kGranule = 256;
+
 struct SrvJob {
   uint64_t id = 0;
   HostProblem hp;
   float *pos_out = nullptr;
   pmmc_anneal_stats *stats = nullptr;
   size_t bytes = 0;
-  char *mem = nullptr;
+  size_t off = 0;               // offset of the job's block in the pool
+  char *mem = nullptr;          // pool + off while resident
   DevProblem *dprob = nullptr;  // job-local problem (index 0)
   DevEns *dens = nullptr;       // job-local ensembles (problem = 0)
   DevResult *dres = nullptr;
   DevProblem hprob{};
   std::vector<DevEns> hens;
+};
+
+// launch group of the server: kernel group key + size class
+using SrvKey = std::tuple<int, int, int, int, int, int>;
+
+struct SrvGroup {
+  LaunchGroup g;
+  SrvKey key;
+};
+
+struct Pacing {
+  int blocks = 4;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+// First-fit free list over [0, cap) in kGranule units.
+class FreeList {
+ public:
+  void reset(size_t cap) {
+    free_.clear();
+    if (cap >= kGranule) free_[0] = cap / kGranule * kGranule;
+  }
+  bool alloc(size_t bytes, size_t *off) {
+    const size_t need = (bytes + kGranule - 1) / kGranule * kGranule;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < need) continue;
+      *off = it->first;
+      const size_t rest = it->second - need;
+      const size_t at = it->first + need;
+      free_.erase(it);
+      if (rest) free_[at] = rest;
+      return true;
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    size_t size = (bytes + kGranule - 1) / kGranule * kGranule;
+    auto next = free_.lower_bound(off);
+    if (next != free_.end() && off + size == next->first) {  // merge with the next block
+      size += next->second;
+      next = free_.erase(next);
+    }
+    if (next != free_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == off) {  // merge into the previous block
+        prev->second += size;
+        return;
+      }
+    }
+    free_[off] = size;
+  }
+
+ private:
+  std::map<size_t, size_t> free_;
 };
 
 }  // namespace pmmc
@@ -47,13 +122,20 @@ struct pmmc_server {
   std::set<uint64_t> done;
   size_t resident_bytes = 0;
   bool dirty = true;
+  char *pool = nullptr;
+  pmmc::FreeList free_list;
   pmmc::DevProblem *d_probs = nullptr;
   size_t probs_cap = 0;
   pmmc::DevEns *d_ens = nullptr;
   size_t ens_cap = 0;
   int *d_counts = nullptr;
   size_t counts_cap = 0;
-  std::vector<pmmc::LaunchGroup> groups;
+  char *h_stage = nullptr;  // pinned staging of retired results
+  size_t stage_cap = 0;
+  std::vector<pmmc::SrvGroup> groups;
+  std::vector<pmmc::DevProblem> h_probs;  // host images of d_probs / d_ens (kept
+  std::vector<pmmc::DevEns> h_ens;        // alive while their copies are queued)
+  std::map<pmmc::SrvKey, pmmc::Pacing> pacing;
   int n_ens = 0;
 };
 
@@ -62,6 +144,18 @@ using namespace pmmc;
 static size_t job_bytes(const HostProblem &h) {
   const int ne = (h.n_chains + kLanes - 1) / kLanes;
   return problem_bytes(h) + 3 * 256 + sizeof(DevProblem) + sizeof(DevEns) * ne + sizeof(DevResult);
+}
+
+static int size_class(int n) {
+  int c = 0;
+  while ((2 << c) <= n) ++c;
+  return c;
+}
+
+static SrvKey server_key(const HostProblem &h) {
+  const auto k = group_key(group_of(h));
+  return std::make_tuple(std::get<0>(k), std::get<1>(k), std::get<2>(k), std::get<3>(k),
+                         std::get<4>(k), size_class(h.n));
 }
 
 template <class T>
@@ -79,10 +173,13 @@ static pmmc_status grow(pmmc_ctx *ctx, T *&p, size_t &cap, size_t need) {
   return PMMC_OK;
 }
 
-static pmmc_status admit(pmmc_server *s, SrvJob *j) {
+// Place the job in the pool and upload it; false when the pool has no
+// contiguous block for it yet (the job stays queued).
+static pmmc_status admit(pmmc_server *s, SrvJob *j, bool *placed) {
   pmmc_ctx *ctx = s->ctx;
-  if (hipMalloc(reinterpret_cast<void **>(&j->mem), j->bytes) != hipSuccess)
-    return fail(ctx, PMMC_ERR_NOMEM, "server: job allocation failed");
+  *placed = s->free_list.alloc(j->bytes, &j->off);
+  if (!*placed) return PMMC_OK;
+  j->mem = s->pool + j->off;
   Carver cv{j->mem, 0, j->bytes};
   const int ne = (j->hp.n_chains + kLanes - 1) / kLanes;
   j->dprob = cv.take<DevProblem>(1);
@@ -100,29 +197,38 @@ static pmmc_status admit(pmmc_server *s, SrvJob *j) {
   return PMMC_OK;
 }
 
-// Rebuild the combined problem / ensemble arrays of the resident set.
+// Rebuild the combined problem / ensemble arrays and the launch groups of the
+// resident set.
 static pmmc_status rebuild(pmmc_server *s) {
   pmmc_ctx *ctx = s->ctx;
-  std::vector<DevProblem> probs;
-  std::vector<std::pair<std::tuple<int, int, int, int, int>, int>> keys;  // (group key, job)
+  std::vector<DevProblem> &probs = s->h_probs;
+  probs.clear();
+  std::vector<std::pair<SrvKey, int>> keys;
   for (size_t k = 0; k < s->resident.size(); ++k) {
-    keys.emplace_back(group_key(group_of(s->resident[k]->hp)), static_cast<int>(k));
+    keys.emplace_back(server_key(s->resident[k]->hp), static_cast<int>(k));
     probs.push_back(s->resident[k]->hprob);
   }
-  std::stable_sort(keys.begin(), keys.end());
-  std::vector<DevEns> ens;
+  std::stable_sort(keys.begin(), keys.end(),
+                   [](const std::pair<SrvKey, int> &a, const std::pair<SrvKey, int> &b) {
+                     return a.first < b.first;
+                   });
+  std::vector<DevEns> &ens = s->h_ens;
+  ens.clear();
   s->groups.clear();
   for (auto &t : keys) {
     const int k = t.second;
-    const LaunchGroup g = group_of(s->resident[k]->hp);
-    if (s->groups.empty() || group_key(s->groups.back()) != t.first) {
-      s->groups.push_back(g);
-      s->groups.back().ens0 = static_cast<int>(ens.size());
+    if (s->groups.empty() || s->groups.back().key != t.first) {
+      SrvGroup sg;
+      sg.g = group_of(s->resident[k]->hp);
+      sg.g.ens0 = static_cast<int>(ens.size());
+      sg.g.n_ens = 0;
+      sg.key = t.first;
+      s->groups.push_back(sg);
     }
     for (DevEns e : s->resident[k]->hens) {
       e.problem = k;
       ens.push_back(e);
-      s->groups.back().n_ens++;
+      s->groups.back().g.n_ens++;
     }
   }
   s->n_ens = static_cast<int>(ens.size());
@@ -140,56 +246,91 @@ static pmmc_status rebuild(pmmc_server *s) {
   return PMMC_OK;
 }
 
-static pmmc_status retire(pmmc_server *s, SrvJob *j) {
+// Finalize the finished jobs, copy their results through the pinned staging
+// buffer with one synchronization, write them to the callers' buffers and
+// return the jobs' pool blocks.
+static pmmc_status retire(pmmc_server *s, const std::vector<SrvJob *> &fin) {
+  if (fin.empty()) return PMMC_OK;
   pmmc_ctx *ctx = s->ctx;
   hipStream_t st = ctx->stream;
-  PMMC_HIP_TRY(ctx, launch_finalize(st, j->dprob, 1, j->dens));
-  DevResult R;
-  std::vector<float> tmp(j->hp.pos0.size());
-  PMMC_HIP_TRY(ctx, hipMemcpyAsync(&R, j->dres, sizeof(R), hipMemcpyDeviceToHost, st));
-  PMMC_HIP_TRY(ctx, hipMemcpyAsync(tmp.data(), j->hprob.out_pos, sizeof(float) * tmp.size(),
-                                   hipMemcpyDeviceToHost, st));
-  PMMC_HIP_TRY(ctx, hipStreamSynchronize(st));
-  if (j->pos_out) {
-    if (j->hp.level == PMMC_LEVEL_SMOOTH) {
-      for (int b = 0; b < j->hp.n; ++b)
-        if (!j->hp.fixed[b])
-          for (int k = 0; k < 3; ++k) j->pos_out[3 * b + k] = tmp[3 * b + k];
-    } else {
-      std::memcpy(j->pos_out, tmp.data(), sizeof(float) * tmp.size());
+  std::vector<size_t> at(fin.size());
+  size_t need = 0;
+  for (size_t i = 0; i < fin.size(); ++i) {
+    at[i] = need;
+    need += (sizeof(DevResult) + sizeof(float) * fin[i]->hp.pos0.size() + 15) & ~size_t(15);
+  }
+  if (need > s->stage_cap) {
+    PMMC_HIP_TRY(ctx, hipStreamSynchronize(st));
+    if (s->h_stage) (void)hipHostFree(s->h_stage);
+    s->h_stage = nullptr;
+    const size_t cap = std::max(need, 2 * s->stage_cap);
+    if (hipHostMalloc(reinterpret_cast<void **>(&s->h_stage), cap) != hipSuccess) {
+      s->stage_cap = 0;
+      return fail(ctx, PMMC_ERR_NOMEM, "server: pinned staging allocation failed");
     }
+    s->stage_cap = cap;
   }
-  if (j->stats) {
-    pmmc_anneal_stats &S = *j->stats;
-    S.energy = R.energy;
-    S.energy_resummed = R.resummed;
-    S.best_energy = R.best;
-    S.moves = R.moves;
-    S.accepted = R.accepted;
-    S.best_chain_moves = R.best_moves;
-    S.best_chain_accepted = R.best_accepted;
-    S.best_chain = R.best_chain;
-    S.n_chains = R.n_chains;
-    S.milestones = R.milestones;
-    S.stop_reason = R.stop_reason;
-    S.kernel_ms = 0.0;
+  for (size_t i = 0; i < fin.size(); ++i) {
+    SrvJob *j = fin[i];
+    PMMC_HIP_TRY(ctx, launch_finalize(st, j->dprob, 1, j->dens));
+    PMMC_HIP_TRY(ctx, hipMemcpyAsync(s->h_stage + at[i], j->dres, sizeof(DevResult),
+                                     hipMemcpyDeviceToHost, st));
+    PMMC_HIP_TRY(ctx, hipMemcpyAsync(s->h_stage + at[i] + sizeof(DevResult), j->hprob.out_pos,
+                                     sizeof(float) * j->hp.pos0.size(), hipMemcpyDeviceToHost, st));
   }
-  (void)hipFree(j->mem);
-  j->mem = nullptr;
-  s->resident_bytes -= j->bytes;
-  s->done.insert(j->id);
+  PMMC_HIP_TRY(ctx, hipStreamSynchronize(st));
+  for (size_t i = 0; i < fin.size(); ++i) {
+    SrvJob *j = fin[i];
+    DevResult R;
+    std::memcpy(&R, s->h_stage + at[i], sizeof(R));
+    const float *tmp = reinterpret_cast<const float *>(s->h_stage + at[i] + sizeof(DevResult));
+    if (j->pos_out) {
+      if (j->hp.level == PMMC_LEVEL_SMOOTH) {
+        // only non-fixed beads are written back (ParallelMonteCarloSmooth.cu:485-491)
+        for (int b = 0; b < j->hp.n; ++b)
+          if (!j->hp.fixed[b])
+            for (int k = 0; k < 3; ++k) j->pos_out[3 * b + k] = tmp[3 * b + k];
+      } else {
+        std::memcpy(j->pos_out, tmp, sizeof(float) * j->hp.pos0.size());
+      }
+    }
+    if (j->stats) {
+      pmmc_anneal_stats &S = *j->stats;
+      S.energy = R.energy;
+      S.energy_resummed = R.resummed;
+      S.best_energy = R.best;
+      S.moves = R.moves;
+      S.accepted = R.accepted;
+      S.best_chain_moves = R.best_moves;
+      S.best_chain_accepted = R.best_accepted;
+      S.best_chain = R.best_chain;
+      S.n_chains = R.n_chains;
+      S.milestones = R.milestones;
+      S.stop_reason = R.stop_reason;
+      S.kernel_ms = 0.0;
+    }
+    s->free_list.release(j->off, j->bytes);
+    j->mem = nullptr;
+    s->resident_bytes -= j->bytes;
+    s->done.insert(j->id);
+  }
   return PMMC_OK;
 }
 
 extern "C" pmmc_status pmmc_server_create(pmmc_ctx *ctx, const pmmc_server_opts *opts,
                                           pmmc_server **out) {
   if (!ctx || !opts || !out) return fail(ctx, PMMC_ERR_INVALID, "server: null argument");
-  if (opts->max_jobs <= 0 || opts->hbm_budget == 0)
-    return fail(ctx, PMMC_ERR_INVALID, "server: max_jobs and hbm_budget must be > 0");
+  if (opts->max_jobs <= 0 || opts->hbm_budget == 0 || opts->blocks_per_step < 0)
+    return fail(ctx, PMMC_ERR_INVALID,
+                "server: max_jobs and hbm_budget must be > 0, blocks_per_step >= 0");
   pmmc_server *s = new pmmc_server();
   s->ctx = ctx;
   s->opts = *opts;
-  if (s->opts.blocks_per_step <= 0) s->opts.blocks_per_step = 4;
+  if (hipMalloc(reinterpret_cast<void **>(&s->pool), s->opts.hbm_budget) != hipSuccess) {
+    delete s;
+    return fail(ctx, PMMC_ERR_NOMEM, "server: the HBM budget could not be allocated");
+  }
+  s->free_list.reset(s->opts.hbm_budget);
   *out = s;
   return PMMC_OK;
 }
@@ -197,14 +338,17 @@ extern "C" pmmc_status pmmc_server_create(pmmc_ctx *ctx, const pmmc_server_opts 
 extern "C" void pmmc_server_destroy(pmmc_server *s) {
   if (!s) return;
   (void)hipStreamSynchronize(s->ctx->stream);
-  for (SrvJob *j : s->resident) {
-    if (j->mem) (void)hipFree(j->mem);
-    delete j;
-  }
+  for (SrvJob *j : s->resident) delete j;
   for (SrvJob *j : s->queue) delete j;
+  for (auto &kv : s->pacing) {
+    if (kv.second.e0) (void)hipEventDestroy(kv.second.e0);
+    if (kv.second.e1) (void)hipEventDestroy(kv.second.e1);
+  }
+  if (s->pool) (void)hipFree(s->pool);
   if (s->d_probs) (void)hipFree(s->d_probs);
   if (s->d_ens) (void)hipFree(s->d_ens);
   if (s->d_counts) (void)hipFree(s->d_counts);
+  if (s->h_stage) (void)hipHostFree(s->h_stage);
   delete s;
 }
 
@@ -229,21 +373,62 @@ extern "C" pmmc_status pmmc_server_submit(pmmc_server *s, const pmmc_job *job, u
   return PMMC_OK;
 }
 
+// One launch per group; with adaptive pacing each launch is bracketed by the
+// group's events.
+static pmmc_status launch_step(pmmc_server *s) {
+  pmmc_ctx *ctx = s->ctx;
+  const bool adaptive = s->opts.blocks_per_step == 0;
+  for (const SrvGroup &sg : s->groups) {
+    int blocks = s->opts.blocks_per_step;
+    Pacing *pc = nullptr;
+    if (adaptive) {
+      pc = &s->pacing[sg.key];
+      if (!pc->e0) {
+        PMMC_HIP_TRY(ctx, hipEventCreate(&pc->e0));
+        PMMC_HIP_TRY(ctx, hipEventCreate(&pc->e1));
+      }
+      blocks = pc->blocks;
+      PMMC_HIP_TRY(ctx, hipEventRecord(pc->e0, ctx->stream));
+    }
+    std::vector<LaunchGroup> one(1, sg.g);
+    PMMC_HIP_TRY(ctx, launch_groups(ctx->stream, s->d_probs, s->d_ens, one, blocks));
+    if (pc) PMMC_HIP_TRY(ctx, hipEventRecord(pc->e1, ctx->stream));
+  }
+  return PMMC_OK;
+}
+
+// After the step's synchronization: rescale each group's block budget toward
+// kTargetMs of device time.
+static void repace(pmmc_server *s) {
+  if (s->opts.blocks_per_step != 0) return;
+  for (const SrvGroup &sg : s->groups) {
+    Pacing &pc = s->pacing[sg.key];
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, pc.e0, pc.e1) != hipSuccess || !(ms > 0.f)) continue;
+    const double scale = static_cast<double>(kTargetMs) / ms;
+    double nb = pc.blocks * std::min(4.0, std::max(0.25, scale));
+    pc.blocks = static_cast<int>(std::min<double>(kMaxBlocks, std::max(1.0, std::floor(nb))));
+  }
+}
+
 extern "C" pmmc_status pmmc_server_step(pmmc_server *s, int32_t *n_done, int32_t *n_resident,
                                         int32_t *n_queued) {
   if (!s) return PMMC_ERR_INVALID;
   pmmc_ctx *ctx = s->ctx;
   int finished = 0;
-  // admission: FIFO, while slots and budget allow
+  // admission: FIFO, while slots, budget and a contiguous pool block allow
   while (!s->queue.empty() && static_cast<int>(s->resident.size()) < s->opts.max_jobs &&
          s->resident_bytes + s->queue.front()->bytes <= s->opts.hbm_budget) {
     SrvJob *j = s->queue.front();
-    s->queue.pop_front();
-    pmmc_status ps = admit(s, j);
+    bool placed = false;
+    pmmc_status ps = admit(s, j, &placed);
     if (ps != PMMC_OK) {
+      s->queue.pop_front();
       delete j;
       return ps;
     }
+    if (!placed) break;
+    s->queue.pop_front();
     s->resident.push_back(j);
     s->resident_bytes += j->bytes;
     s->dirty = true;
@@ -253,26 +438,22 @@ extern "C" pmmc_status pmmc_server_step(pmmc_server *s, int32_t *n_done, int32_t
       pmmc_status ps = rebuild(s);
       if (ps != PMMC_OK) return ps;
     }
-    PMMC_HIP_TRY(ctx, launch_groups(ctx->stream, s->d_probs, s->d_ens, s->groups, s->opts.blocks_per_step));
+    pmmc_status ps = launch_step(s);
+    if (ps != PMMC_OK) return ps;
     const int nr = static_cast<int>(s->resident.size());
     PMMC_HIP_TRY(ctx, launch_count_per_problem(ctx->stream, s->d_ens, s->n_ens, s->d_counts, nr));
     std::vector<int> counts(nr);
     PMMC_HIP_TRY(ctx, hipMemcpyAsync(counts.data(), s->d_counts, sizeof(int) * nr, hipMemcpyDeviceToHost,
                                      ctx->stream));
     PMMC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    std::vector<SrvJob *> keep;
-    for (int k = 0; k < nr; ++k) {
-      SrvJob *j = s->resident[k];
-      if (counts[k] == 0) {
-        pmmc_status ps = retire(s, j);
-        if (ps != PMMC_OK) return ps;
-        delete j;
-        ++finished;
-        s->dirty = true;
-      } else {
-        keep.push_back(j);
-      }
-    }
+    repace(s);
+    std::vector<SrvJob *> keep, fin;
+    for (int k = 0; k < nr; ++k) (counts[k] == 0 ? fin : keep).push_back(s->resident[k]);
+    ps = retire(s, fin);
+    if (ps != PMMC_OK) return ps;
+    for (SrvJob *j : fin) delete j;
+    finished = static_cast<int>(fin.size());
+    if (!fin.empty()) s->dirty = true;
     s->resident.swap(keep);
   }
   if (n_done) *n_done = finished;

@@ -25,7 +25,6 @@ from __future__ import annotations
 import concurrent.futures as cf
 import json
 import os
-import subprocess
 import sys
 
 import numpy as np
@@ -35,46 +34,22 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from oracle import metrics as OM  # noqa: E402
+from oracle import refdriver as R  # noqa: E402
 from pmmc_amd import synth  # noqa: E402
 
-REF = os.path.join(ROOT, "oracle", "_ref", "levels_ref")
 SEEDS = list(range(16))
 # smooth instance of the band test: 21 anchors x 49 beads between them = 1,001
 # beads (the 20,000-bead instance needs ~39 M moves per chain at this schedule)
 SMOOTH_ANCHORS, SMOOTH_ADD = 21, 49
-HEADER = "0 " * 12  # count/heatmap/genomic-length parameters (unused by these modes)
-
-
-def hx(a) -> str:
-    return " ".join(float(v).hex() for v in np.asarray(a, np.float32).reshape(-1))
-
-
-def run_ref(text: str, tag: str) -> list[str]:
-    r = subprocess.run([REF], input=text, capture_output=True, text=True, check=True)
-    lines = r.stdout.splitlines()
-    k = [i for i, line in enumerate(lines) if line.startswith(tag)][0]
-    return lines[k:]
+HEADER = R.HEADER
+hx = R.hx
+run_ref = R.run
 
 
 def ref_anneal(level: int, seed: int):
     prob, step, truth = synth.config_case(level, seed, SMOOTH_ANCHORS, SMOOTH_ADD)
-    s = synth.CONFIG_SETTINGS[level]
-    n = len(prob.pos)
-    parts = [f"anneal {HEADER}", f"{level} {1 + seed}", float(np.float32(step)).hex(),
-             f"{s['max_temp']} {s['dt_temp']} {s['tjs']} {s['tjc']} {s['improvement']} "
-             f"{s['min_successes']} {s['steps']} 0"]
-    if level == 1:
-        parts += [f"{n} {prob.diagonal}", hx(prob.pos), " ".join(["0"] * n), hx(prob.dist)]
-    elif level == 2:
-        parts += [f"{n} {prob.k_stretch} {prob.k_squeeze}", hx(prob.pos), hx(prob.exp_dist)]
-    else:
-        parts += [f"{n} {prob.k_stretch} {prob.k_squeeze} {prob.k_angular} {prob.w_dist} "
-                  f"{prob.w_angle}", hx(prob.pos), " ".join(str(int(v)) for v in prob.fixed),
-                  hx(prob.dist_to_next)]
-    out = run_ref("\n".join(parts) + "\n", "@@anneal")
-    _, score, moves = out[0].split()
-    pos = np.array([[float.fromhex(t) for t in line.split()] for line in out[1:1 + n]], np.float32)
-    rec = {"seed": seed, "score": float.fromhex(score), "moves": int(moves)}
+    score, moves, pos = R.anneal(level, prob, step, synth.CONFIG_SETTINGS[level], 1 + seed)
+    rec = {"seed": seed, "score": score, "moves": moves}
     if level == 1:
         rec.update(ref_metrics(pos, truth))
     if level == 2:
@@ -124,8 +99,8 @@ def metrics_cases():
 
 
 def main() -> int:
-    if not os.path.exists(REF):
-        print(f"{REF} missing: run `make -C oracle ref` (needs the reference sources)")
+    if not R.available():
+        print(f"{R.REF} missing: run `make -C oracle ref` (needs the reference sources)")
         return 1
     out = {"metrics": metrics_cases()}
     json.dump(out, open(os.path.join(HERE, "metrics_ref.json"), "w"), indent=0)
