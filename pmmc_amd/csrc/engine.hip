@@ -148,10 +148,8 @@ __device__ void exact_smooth(const DevProblem &P, const float *x, double *red, d
   out[1] = block_sum_d(sa, red);
 }
 
-// One workgroup (256 threads) per problem: initial running total and offset.
-__global__ __launch_bounds__(256) void k_init_problem(DevProblem *probs) {
-  __shared__ double red[256];
-  DevProblem &P = probs[blockIdx.x];
+// Initial running total and offset of one problem (one 256-thread workgroup).
+__device__ void init_problem_body(DevProblem &P, double *red) {
   const float *x = P.pos0;
   const float r = P.sched.sphere_radius;
   double E0 = 0.0, off = 0.0;
@@ -175,14 +173,26 @@ __global__ __launch_bounds__(256) void k_init_problem(DevProblem *probs) {
   }
 }
 
-// One wavefront per ensemble: replicate pos0 into the SoA lanes, init state.
-__global__ __launch_bounds__(64) void k_init_ens(const DevProblem *__restrict__ probs,
-                                                 const DevEns *__restrict__ ens) {
-  const DevEns it = ens[blockIdx.x];
-  const DevProblem &P = probs[it.problem];
+__global__ __launch_bounds__(256) void k_init_problem(DevProblem *probs) {
+  __shared__ double red[256];
+  init_problem_body(probs[blockIdx.x], red);
+}
+
+__global__ __launch_bounds__(256) void k_init_problem_refs(const JobRef *__restrict__ refs) {
+  __shared__ double red[256];
+  init_problem_body(*refs[blockIdx.x].prob, red);
+}
+
+// Ensemble initialisation: workgroup x of the ensemble replicates its share of
+// pos0 into the SoA lanes (256 elements per pass, consecutive threads write
+// consecutive addresses); workgroup 0 also sets the 64 lanes' state.
+__device__ void init_ens_body(const DevProblem &P, const DevEns &it) {
+  const size_t total = static_cast<size_t>(3 * P.n) * kLanes;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    it.pos[i] = P.pos0[i / kLanes];
+  if (blockIdx.x != 0 || threadIdx.x >= kLanes) return;
   const int lane = threadIdx.x;
-  const int n3 = 3 * P.n;
-  for (int i = 0; i < n3; ++i) it.pos[static_cast<size_t>(i) * kLanes + lane] = P.pos0[i];
   EnsState *st = it.st;
   const double E0 = *P.E0;
   int status = 0;
@@ -200,6 +210,21 @@ __global__ __launch_bounds__(64) void k_init_ens(const DevProblem *__restrict__ 
   st->misses[lane] = 0;
   st->milestones[lane] = 0;
   st->status[lane] = status;
+}
+
+__global__ __launch_bounds__(256) void k_init_ens(const DevProblem *__restrict__ probs,
+                                                  const DevEns *__restrict__ ens, int n_ens) {
+  for (int e = blockIdx.y; e < n_ens; e += gridDim.y) {
+    const DevEns it = ens[e];
+    init_ens_body(probs[it.problem], it);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_init_ens_refs(const EnsRef *__restrict__ refs, int n) {
+  for (int e = blockIdx.y; e < n; e += gridDim.y) {
+    const DevEns it = *refs[e].ens;
+    init_ens_body(*refs[e].prob, it);
+  }
 }
 
 // ------------------------------------------------------------------ //
@@ -377,14 +402,20 @@ hipError_t launch_count_per_problem(hipStream_t s, const DevEns *ens, int n_ens,
   return hipGetLastError();
 }
 
+// Shared memory of the finalize workgroup.
+struct FinalizeLds {
+  double red[256];
+  double bE[256];
+  int bC[256];
+  long long sm[256], sa[256];
+};
+
 // One workgroup (256 threads) per problem: best chain, AoS gather, FP64
-// return value, move counts.
-__global__ __launch_bounds__(256) void k_finalize(DevProblem *probs, const DevEns *__restrict__ ens) {
-  __shared__ double red[256];
-  __shared__ double bE[256];
-  __shared__ int bC[256];
-  __shared__ long long sm[256], sa[256];
-  DevProblem &P = probs[blockIdx.x];
+// return value, move counts.  ens: the array P.ens0 indexes.
+__device__ void finalize_body(DevProblem &P, const DevEns *__restrict__ ens, FinalizeLds &L) {
+  double *red = L.red, *bE = L.bE;
+  int *bC = L.bC;
+  long long *sm = L.sm, *sa = L.sa;
   const int t = threadIdx.x;
   double myE = INFINITY;
   int myC = INT_MAX;
@@ -458,6 +489,16 @@ __global__ __launch_bounds__(256) void k_finalize(DevProblem *probs, const DevEn
   }
 }
 
+__global__ __launch_bounds__(256) void k_finalize(DevProblem *probs, const DevEns *__restrict__ ens) {
+  __shared__ FinalizeLds L;
+  finalize_body(probs[blockIdx.x], ens, L);
+}
+
+__global__ __launch_bounds__(256) void k_finalize_refs(const JobRef *__restrict__ refs) {
+  __shared__ FinalizeLds L;
+  finalize_body(*refs[blockIdx.x].prob, refs[blockIdx.x].ens, L);
+}
+
 // ------------------------------------------------------------------ //
 // host side
 // ------------------------------------------------------------------ //
@@ -526,9 +567,34 @@ hipError_t launch_groups(hipStream_t s, const DevProblem *probs, const DevEns *e
   return err;
 }
 
-hipError_t launch_init(hipStream_t s, DevProblem *probs, int n_probs, const DevEns *ens, int n_ens) {
+// Workgroups per ensemble for replicating n*3*64 floats (256 per pass, at
+// most 64 workgroups per ensemble), and ensembles per grid row.
+static dim3 init_grid(int max_n, int n_ens) {
+  const size_t total = static_cast<size_t>(3 * std::max(max_n, 1)) * kLanes;
+  const unsigned gx = static_cast<unsigned>(std::min<size_t>(64, (total + 255) / 256));
+  const unsigned gy = static_cast<unsigned>(std::min(std::max(n_ens, 1), 65535));
+  return dim3(gx, gy);
+}
+
+hipError_t launch_init(hipStream_t s, DevProblem *probs, int n_probs, const DevEns *ens, int n_ens,
+                       int max_n) {
+  if (n_probs <= 0 || n_ens <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_init_problem, dim3(n_probs), dim3(256), 0, s, probs);
-  hipLaunchKernelGGL(k_init_ens, dim3(n_ens), dim3(kLanes), 0, s, probs, ens);
+  hipLaunchKernelGGL(k_init_ens, init_grid(max_n, n_ens), dim3(256), 0, s, probs, ens, n_ens);
+  return hipGetLastError();
+}
+
+hipError_t launch_init_refs(hipStream_t s, const JobRef *jobs, int n_jobs, const EnsRef *ens,
+                            int n_ens, int max_n) {
+  if (n_jobs <= 0 || n_ens <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_init_problem_refs, dim3(n_jobs), dim3(256), 0, s, jobs);
+  hipLaunchKernelGGL(k_init_ens_refs, init_grid(max_n, n_ens), dim3(256), 0, s, ens, n_ens);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_refs(hipStream_t s, const JobRef *jobs, int n_jobs) {
+  if (n_jobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_finalize_refs, dim3(n_jobs), dim3(256), 0, s, jobs);
   return hipGetLastError();
 }
 
@@ -671,7 +737,9 @@ pmmc_status batch_setup(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &b
     err = hipMemcpyAsync(bd.ens, bd.hens.data(), sizeof(DevEns) * bd.hens.size(),
                          hipMemcpyHostToDevice, s);
   if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("upload: ") + hipGetErrorString(err));
-  err = launch_init(s, bd.probs, static_cast<int>(hp.size()), bd.ens, bd.n_ens_total);
+  int max_n = 1;
+  for (auto &h : hp) max_n = std::max(max_n, h.n);
+  err = launch_init(s, bd.probs, static_cast<int>(hp.size()), bd.ens, bd.n_ens_total, max_n);
   if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("init: ") + hipGetErrorString(err));
   return PMMC_OK;
 }

@@ -87,6 +87,19 @@ struct DevEns {
   EnsState *st;
 };
 
+// a problem and the ensemble array its ens0 indexes (server jobs keep both
+// in their own pool block)
+struct JobRef {
+  DevProblem *prob;
+  const DevEns *ens;
+};
+
+// one ensemble of a problem, addressed directly
+struct EnsRef {
+  const DevProblem *prob;
+  const DevEns *ens;
+};
+
 struct DevResult {
   double energy, resummed, best;
   int64_t moves, accepted, best_moves, best_accepted;

@@ -73,7 +73,13 @@ void fill_problem(Carver &cv, hipStream_t s, const HostProblem &h, int index, De
                   DevProblem &P, std::vector<DevEns> &ens, hipError_t &err);
 hipError_t launch_groups(hipStream_t s, const DevProblem *probs, const DevEns *ens,
                          const std::vector<LaunchGroup> &groups, int max_blocks);
-hipError_t launch_init(hipStream_t s, DevProblem *probs, int n_probs, const DevEns *ens, int n_ens);
+// max_n: largest bead count among the problems (sizes the replication grid)
+hipError_t launch_init(hipStream_t s, DevProblem *probs, int n_probs, const DevEns *ens, int n_ens,
+                       int max_n);
+// the same for problems and ensembles addressed through references (server)
+hipError_t launch_init_refs(hipStream_t s, const JobRef *jobs, int n_jobs, const EnsRef *ens,
+                            int n_ens, int max_n);
+hipError_t launch_finalize_refs(hipStream_t s, const JobRef *jobs, int n_jobs);
 // register-resident heatmap kernel (heatmap_reg.hip) for heatmaps of <= 64 beads
 int hreg_halves(const HostProblem &h);
 hipError_t launch_heatmap_reg(hipStream_t s, const DevProblem *probs, const DevEns *ens, int n_ens,

@@ -135,6 +135,12 @@ struct pmmc_server {
   std::vector<pmmc::SrvGroup> groups;
   std::vector<pmmc::DevProblem> h_probs;  // host images of d_probs / d_ens (kept
   std::vector<pmmc::DevEns> h_ens;        // alive while their copies are queued)
+  pmmc::JobRef *d_jobrefs = nullptr;      // jobs initialised / finalised by one launch
+  size_t jobrefs_cap = 0;
+  pmmc::EnsRef *d_ensrefs = nullptr;
+  size_t ensrefs_cap = 0;
+  std::vector<pmmc::JobRef> h_jobrefs;
+  std::vector<pmmc::EnsRef> h_ensrefs;
   std::map<pmmc::SrvKey, pmmc::Pacing> pacing;
   int n_ens = 0;
 };
@@ -192,7 +198,6 @@ static pmmc_status admit(pmmc_server *s, SrvJob *j, bool *placed) {
     err = hipMemcpyAsync(j->dprob, &j->hprob, sizeof(DevProblem), hipMemcpyHostToDevice, ctx->stream);
   if (err == hipSuccess)
     err = hipMemcpyAsync(j->dens, j->hens.data(), sizeof(DevEns) * ne, hipMemcpyHostToDevice, ctx->stream);
-  if (err == hipSuccess) err = launch_init(ctx->stream, j->dprob, 1, j->dens, ne);
   if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("server admit: ") + hipGetErrorString(err));
   return PMMC_OK;
 }
@@ -270,9 +275,15 @@ static pmmc_status retire(pmmc_server *s, const std::vector<SrvJob *> &fin) {
     }
     s->stage_cap = cap;
   }
+  s->h_jobrefs.clear();
+  for (SrvJob *j : fin) s->h_jobrefs.push_back(JobRef{j->dprob, j->dens});
+  pmmc_status ps = grow(ctx, s->d_jobrefs, s->jobrefs_cap, s->h_jobrefs.size());
+  if (ps != PMMC_OK) return ps;
+  PMMC_HIP_TRY(ctx, hipMemcpyAsync(s->d_jobrefs, s->h_jobrefs.data(), sizeof(JobRef) * s->h_jobrefs.size(),
+                                   hipMemcpyHostToDevice, st));
+  PMMC_HIP_TRY(ctx, launch_finalize_refs(st, s->d_jobrefs, static_cast<int>(fin.size())));
   for (size_t i = 0; i < fin.size(); ++i) {
     SrvJob *j = fin[i];
-    PMMC_HIP_TRY(ctx, launch_finalize(st, j->dprob, 1, j->dens));
     PMMC_HIP_TRY(ctx, hipMemcpyAsync(s->h_stage + at[i], j->dres, sizeof(DevResult),
                                      hipMemcpyDeviceToHost, st));
     PMMC_HIP_TRY(ctx, hipMemcpyAsync(s->h_stage + at[i] + sizeof(DevResult), j->hprob.out_pos,
@@ -348,6 +359,8 @@ extern "C" void pmmc_server_destroy(pmmc_server *s) {
   if (s->d_probs) (void)hipFree(s->d_probs);
   if (s->d_ens) (void)hipFree(s->d_ens);
   if (s->d_counts) (void)hipFree(s->d_counts);
+  if (s->d_jobrefs) (void)hipFree(s->d_jobrefs);
+  if (s->d_ensrefs) (void)hipFree(s->d_ensrefs);
   if (s->h_stage) (void)hipHostFree(s->h_stage);
   delete s;
 }
@@ -370,6 +383,30 @@ extern "C" pmmc_status pmmc_server_submit(pmmc_server *s, const pmmc_job *job, u
   j->stats = job->stats;
   s->queue.push_back(j);
   if (job_id) *job_id = j->id;
+  return PMMC_OK;
+}
+
+// Initial totals and chain state of the jobs admitted in this step: one
+// launch pair over all of them.
+static pmmc_status init_jobs(pmmc_server *s, const std::vector<SrvJob *> &fresh) {
+  pmmc_ctx *ctx = s->ctx;
+  s->h_jobrefs.clear();
+  s->h_ensrefs.clear();
+  int max_n = 1;
+  for (SrvJob *j : fresh) {
+    s->h_jobrefs.push_back(JobRef{j->dprob, j->dens});
+    for (size_t e = 0; e < j->hens.size(); ++e) s->h_ensrefs.push_back(EnsRef{j->dprob, j->dens + e});
+    max_n = std::max(max_n, j->hp.n);
+  }
+  pmmc_status ps = grow(ctx, s->d_jobrefs, s->jobrefs_cap, s->h_jobrefs.size());
+  if (ps == PMMC_OK) ps = grow(ctx, s->d_ensrefs, s->ensrefs_cap, s->h_ensrefs.size());
+  if (ps != PMMC_OK) return ps;
+  PMMC_HIP_TRY(ctx, hipMemcpyAsync(s->d_jobrefs, s->h_jobrefs.data(), sizeof(JobRef) * s->h_jobrefs.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+  PMMC_HIP_TRY(ctx, hipMemcpyAsync(s->d_ensrefs, s->h_ensrefs.data(), sizeof(EnsRef) * s->h_ensrefs.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+  PMMC_HIP_TRY(ctx, launch_init_refs(ctx->stream, s->d_jobrefs, static_cast<int>(s->h_jobrefs.size()),
+                                     s->d_ensrefs, static_cast<int>(s->h_ensrefs.size()), max_n));
   return PMMC_OK;
 }
 
@@ -416,6 +453,7 @@ extern "C" pmmc_status pmmc_server_step(pmmc_server *s, int32_t *n_done, int32_t
   if (!s) return PMMC_ERR_INVALID;
   pmmc_ctx *ctx = s->ctx;
   int finished = 0;
+  std::vector<SrvJob *> fresh;
   // admission: FIFO, while slots, budget and a contiguous pool block allow
   while (!s->queue.empty() && static_cast<int>(s->resident.size()) < s->opts.max_jobs &&
          s->resident_bytes + s->queue.front()->bytes <= s->opts.hbm_budget) {
@@ -430,8 +468,13 @@ extern "C" pmmc_status pmmc_server_step(pmmc_server *s, int32_t *n_done, int32_t
     if (!placed) break;
     s->queue.pop_front();
     s->resident.push_back(j);
+    fresh.push_back(j);
     s->resident_bytes += j->bytes;
     s->dirty = true;
+  }
+  if (!fresh.empty()) {
+    pmmc_status ps = init_jobs(s, fresh);
+    if (ps != PMMC_OK) return ps;
   }
   if (!s->resident.empty()) {
     if (s->dirty) {
