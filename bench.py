@@ -164,8 +164,10 @@ def levels(ctx, with_cpu=True, cpu_seconds=8.0):
     # (LooperSolver::MonteCarloHeatmap: 50k-move milestones, cooling per move)
     prob, step, truth = synth.config_case(1, 0)
     sched = schedule_preset(1, 0, settings(**synth.CONFIG_SETTINGS[1]), step)
-    chains = 65536
-    ctx.anneal(prob, sched, seed=1, n_chains=4096)
+    chains = 262144
+    warm = schedule_preset(1, 0, settings(**synth.CONFIG_SETTINGS[1]), step)
+    warm.max_moves = 1024
+    ctx.anneal(prob, warm, seed=1, n_chains=chains)
     t0 = time.perf_counter()
     pos, st = ctx.anneal(prob, sched, seed=2, segment=1, n_chains=chains)
     wall = time.perf_counter() - t0

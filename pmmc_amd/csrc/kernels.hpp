@@ -88,6 +88,13 @@ __device__ __forceinline__ void milestone(const pmmc_schedule &S, double E, doub
 // level energies, FP32 move deltas
 // ------------------------------------------------------------------ //
 
+#ifndef PMMC_HEAT_CHUNK
+#define PMMC_HEAT_CHUNK 16
+#endif
+// partner positions loaded per round trip of the heatmap partner loop
+constexpr int kHeatChunk = PMMC_HEAT_CHUNK;
+static_assert(kHeatChunk % 2 == 0 && kLanes % kHeatChunk == 0, "even chunk dividing 64");
+
 // heatmap: exact change of the full (double-counted) score for moving p from
 // xp to xn; LooperSolver.hpp:3583-3617 summed over all beads.
 __device__ __forceinline__ float heatmap_delta(const DevProblem &P, const PosView &X, int p,
@@ -97,35 +104,35 @@ __device__ __forceinline__ float heatmap_delta(const DevProblem &P, const PosVie
   const int lane = threadIdx.x & (kLanes - 1);
   // The weight row of the (wave-uniform) bead p is read 64 entries at a time
   // with one coalesced vector load; partner j's weight is then broadcast from
-  // lane j%64 with v_readlane.  Partner terms alternate between two
-  // accumulators by the parity of j.
+  // lane j%64 with v_readlane.  Partners are taken kHeatChunk at a time: all
+  // their position loads are issued before any is consumed.  Partner j's term
+  // goes to accumulator j%2, in increasing j; past the last partner the chunk
+  // is padded with weight 0 (a term of exactly 0).
   const float *__restrict__ w1 = P.W + static_cast<size_t>(p) * P.ldw;
   float acc0 = 0.f, acc1 = 0.f;
   for (int c = 0; c < n; c += kLanes) {
     const int cnt = min(kLanes, n - c);
     const float wv = lane < cnt ? w1[c + lane] : 0.f;
-    int jj = 0;
-#pragma unroll 4
-    for (; jj + 1 < cnt; jj += 2) {
-      const float wa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), jj));
-      const float wb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), jj + 1));
-      const int j = c + jj;
-      const float xa = X.get(j, 0), ya = X.get(j, 1), za = X.get(j, 2);
-      const float xb = X.get(j + 1, 0), yb = X.get(j + 1, 1), zb = X.get(j + 1, 2);
-      const float a0 = fmaf(dist3(px, py, pz, xa, ya, za), wa, -1.f);
-      const float a1 = fmaf(dist3(nx, ny, nz, xa, ya, za), wa, -1.f);
-      const float b0 = fmaf(dist3(px, py, pz, xb, yb, zb), wb, -1.f);
-      const float b1 = fmaf(dist3(nx, ny, nz, xb, yb, zb), wb, -1.f);
-      acc0 = fmaf(a1 - a0, a1 + a0, acc0);
-      acc1 = fmaf(b1 - b0, b1 + b0, acc1);
-    }
-    if (jj < cnt) {  // cnt odd: only in the last chunk
-      const float wa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), jj));
-      const int j = c + jj;
-      const float xa = X.get(j, 0), ya = X.get(j, 1), za = X.get(j, 2);
-      const float a0 = fmaf(dist3(px, py, pz, xa, ya, za), wa, -1.f);
-      const float a1 = fmaf(dist3(nx, ny, nz, xa, ya, za), wa, -1.f);
-      acc0 = fmaf(a1 - a0, a1 + a0, acc0);
+    for (int jj = 0; jj < cnt; jj += kHeatChunk) {
+      float xs[kHeatChunk], ys[kHeatChunk], zs[kHeatChunk];
+#pragma unroll
+      for (int u = 0; u < kHeatChunk; ++u) {
+        const int j = min(c + jj + u, n - 1);
+        xs[u] = X.get(j, 0);
+        ys[u] = X.get(j, 1);
+        zs[u] = X.get(j, 2);
+      }
+#pragma unroll
+      for (int u = 0; u < kHeatChunk; u += 2) {
+        const float wa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), jj + u));
+        const float wb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), jj + u + 1));
+        const float a0 = fmaf(dist3(px, py, pz, xs[u], ys[u], zs[u]), wa, -1.f);
+        const float a1 = fmaf(dist3(nx, ny, nz, xs[u], ys[u], zs[u]), wa, -1.f);
+        const float b0 = fmaf(dist3(px, py, pz, xs[u + 1], ys[u + 1], zs[u + 1]), wb, -1.f);
+        const float b1 = fmaf(dist3(nx, ny, nz, xs[u + 1], ys[u + 1], zs[u + 1]), wb, -1.f);
+        acc0 = fmaf(a1 - a0, a1 + a0, acc0);
+        acc1 = fmaf(b1 - b0, b1 + b0, acc1);
+      }
     }
   }
   float d = acc0 + acc1;
@@ -135,13 +142,22 @@ __device__ __forceinline__ float heatmap_delta(const DevProblem &P, const PosVie
   for (int c = 0; c < n; c += kLanes) {
     const int cnt = min(kLanes, n - c);
     const float wv = lane < cnt ? w2[c + lane] : 0.f;
-    for (int jj = 0; jj < cnt; ++jj) {
-      const float w = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), jj));
-      const int j = c + jj;
-      const float xa = X.get(j, 0), ya = X.get(j, 1), za = X.get(j, 2);
-      const float a0 = fmaf(dist3(px, py, pz, xa, ya, za), w, -1.f);
-      const float a1 = fmaf(dist3(nx, ny, nz, xa, ya, za), w, -1.f);
-      acc2 = fmaf(a1 - a0, a1 + a0, acc2);
+    for (int jj = 0; jj < cnt; jj += kHeatChunk) {
+      float xs[kHeatChunk], ys[kHeatChunk], zs[kHeatChunk];
+#pragma unroll
+      for (int u = 0; u < kHeatChunk; ++u) {
+        const int j = min(c + jj + u, n - 1);
+        xs[u] = X.get(j, 0);
+        ys[u] = X.get(j, 1);
+        zs[u] = X.get(j, 2);
+      }
+#pragma unroll
+      for (int u = 0; u < kHeatChunk; ++u) {
+        const float w = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), jj + u));
+        const float a0 = fmaf(dist3(px, py, pz, xs[u], ys[u], zs[u]), w, -1.f);
+        const float a1 = fmaf(dist3(nx, ny, nz, xs[u], ys[u], zs[u]), w, -1.f);
+        acc2 = fmaf(a1 - a0, a1 + a0, acc2);
+      }
     }
   }
   return d + acc2;
