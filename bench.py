@@ -164,7 +164,7 @@ def levels(ctx, with_cpu=True, cpu_seconds=8.0):
     # (LooperSolver::MonteCarloHeatmap: 50k-move milestones, cooling per move)
     prob, step, truth = synth.config_case(1, 0)
     sched = schedule_preset(1, 0, settings(**synth.CONFIG_SETTINGS[1]), step)
-    chains = 262144
+    chains = 131072  # one round of the register-resident kernel (8 ensembles per CU)
     warm = schedule_preset(1, 0, settings(**synth.CONFIG_SETTINGS[1]), step)
     warm.max_moves = 1024
     ctx.anneal(prob, warm, seed=1, n_chains=chains)

@@ -168,13 +168,18 @@ pmmc_status check_schedule(pmmc_ctx *ctx, const pmmc_schedule &s) {
   return PMMC_OK;
 }
 
-int default_chains(const pmmc_ctx *ctx, int /*level*/, int n) {
-  // sixteen 64-chain ensembles (wavefronts) per compute unit -- enough
-  // resident wavefronts to hide the HBM latency of the partner loads --
-  // capped so the chains' positions take at most 32 GiB of HBM
+int default_chains(const pmmc_ctx *ctx, const HostProblem &h) {
+  // Whole rounds of the level kernel's resident wavefronts: the ensembles
+  // (64 chains each) that one CU holds at once, times the rounds needed for
+  // at least sixteen per CU -- enough wavefronts to hide the HBM latency of
+  // the partner loads, and no partial last round (a partial round runs its
+  // few waves alone, which measured 10-20% slower) -- capped so the chains'
+  // positions take at most 32 GiB of HBM.
   const int cus = ctx && ctx->cu_count > 0 ? ctx->cu_count : 256;
-  const size_t per_ens = static_cast<size_t>(n > 0 ? n : 1) * 3 * kLanes * sizeof(float);
-  size_t ens = static_cast<size_t>(16) * cus;
+  const int res = resident_ensembles_per_cu(h);
+  const int rounds = (16 + res - 1) / res;
+  const size_t per_ens = static_cast<size_t>(h.n > 0 ? h.n : 1) * 3 * kLanes * sizeof(float);
+  size_t ens = static_cast<size_t>(res) * rounds * cus;
   const size_t cap = (size_t(32) << 30) / per_ens;
   if (ens > cap) ens = cap > 0 ? cap : 1;
   return static_cast<int>(ens) * kLanes;
@@ -321,7 +326,7 @@ pmmc_status prepare_problem(pmmc_ctx *ctx, int level, const void *problem, const
   h.segment = run.segment;
   if (run.chain0 % kLanes != 0) return fail(ctx, PMMC_ERR_INVALID, "run.chain0 must be a multiple of 64");
   h.chain0 = run.chain0;
-  h.n_chains = run.n_chains > 0 ? run.n_chains : default_chains(ctx, level, h.n);
+  h.n_chains = run.n_chains > 0 ? run.n_chains : default_chains(ctx, h);
   if (static_cast<uint64_t>(h.chain0) + h.n_chains > 0x7FFFFFFFull)
     return fail(ctx, PMMC_ERR_INVALID, "chain ids exceed 2^31");
   return PMMC_OK;

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -242,12 +243,12 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
   if (__ballot(status == 0) == 0) return;
 
   const int n = P.n;
-  PosView X;
+  PosViewT<STAGE> X;
   if constexpr (STAGE) {
     for (int i = 0; i < 3 * n; ++i) lds[i * kLanes + lane] = it.pos[static_cast<size_t>(i) * kLanes + lane];
     X.p = lds + lane;
   } else {
-    X.p = it.pos + lane;
+    X = global_view(it.pos, lane);
   }
 
   const pmmc_schedule S = P.sched;
@@ -273,13 +274,53 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
     const uint64_t am = __ballot(active);
     if (am == 0) break;
     const uint32_t m0 = readlane_u32(iters, __builtin_ctzll(am));
+    if constexpr (LEVEL == PMMC_LEVEL_ARCS && !STAGE) {
+      // Software-pipelined arc moves: the bead of move m+1 comes from the
+      // pick stream, independent of the chains' state, so its CSR row and
+      // the positions it reads are fetched before move m is evaluated (their
+      // HBM latency overlaps move m's arithmetic); after move m's decision
+      // the fetched copies of the bead move m changed are patched.  Same
+      // draws, values and arithmetic as the plain loop below.
+      U4 rbn = draw(seed, m0 >> 2, group, segment, LEVEL, kSubBead);
+      ArcsFetch f;
+      arcs_fetch(P, X, cst(P.free_list)[pick(pick_word(rbn, m0), static_cast<uint32_t>(nfree))], f);
+      for (int k = 0; k < S.block; ++k) {
+        const uint32_t m = m0 + static_cast<uint32_t>(k);
+        const ArcsFetch c = f;
+        const bool more = k + 1 < S.block;
+        if (more) {
+          const uint32_t mn = m + 1;
+          if ((mn & 3u) == 0) rbn = draw(seed, mn >> 2, group, segment, LEVEL, kSubBead);
+          arcs_fetch(P, X, cst(P.free_list)[pick(pick_word(rbn, mn), static_cast<uint32_t>(nfree))], f);
+        }
+        const U4 rc = draw(seed, m, chain, segment, LEVEL, kSubChain);
+        const float dx = disp(rc.x, step), dy = disp(rc.y, step);
+        const float dz = S.use2d ? 0.f : disp(rc.z, step);
+        const float nx = move_coord(c.px, dx), ny = move_coord(c.py, dy), nz = move_coord(c.pz, dz);
+        float dE = arcs_delta_fetched(P, X, c, nx, ny, nz);
+        if (r > 0.f) dE += sphere_penalty(nx, ny, nz, r) - sphere_penalty(c.px, c.py, c.pz, r);
+        const double En = E + static_cast<double>(dE);
+        const bool ok = active && metropolis(S, E, En, T, u01(rc.w));
+        if (ok) {
+          X.set(c.p, 0, nx);
+          X.set(c.p, 1, ny);
+          X.set(c.p, 2, nz);
+          E = En;
+          ++msucc;
+          ++acc;
+          best = E < best ? E : best;
+        }
+        if (more) arcs_patch(f, c.p, ok, nx, ny, nz);
+        if (active && !S.cool_per_block) T *= S.dt;
+      }
+    } else {
     U4 rb = {0, 0, 0, 0};
     for (int k = 0; k < S.block; ++k) {
       const uint32_t m = m0 + static_cast<uint32_t>(k);
       if (k == 0 || (m & 3u) == 0) rb = draw(seed, m >> 2, group, segment, LEVEL, kSubBead);
       const uint32_t sel = m & 3u;
       const uint32_t word = sel == 0 ? rb.x : (sel == 1 ? rb.y : (sel == 2 ? rb.z : rb.w));
-      const int p = P.free_list[pick(word, static_cast<uint32_t>(nfree))];
+      const int p = cst(P.free_list)[pick(word, static_cast<uint32_t>(nfree))];
       const float px = X.get(p, 0), py = X.get(p, 1), pz = X.get(p, 2);
       const U4 rc = draw(seed, m, chain, segment, LEVEL, kSubChain);
       const float dx = disp(rc.x, step), dy = disp(rc.y, step);
@@ -300,8 +341,8 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
           qy[o] = in ? X.get(idx, 1) : 0.f;
           qz[o] = in ? X.get(idx, 2) : 0.f;
         }
-        const float d0m = p > 0 ? fmaxf(P.dtn[p - 1], 1e-6f) : 1.f;
-        const float d0p = p < n - 1 ? fmaxf(P.dtn[p], 1e-6f) : 1.f;
+        const float d0m = p > 0 ? fmaxf(cst(P.dtn)[p - 1], 1e-6f) : 1.f;
+        const float d0p = p < n - 1 ? fmaxf(cst(P.dtn)[p], 1e-6f) : 1.f;
         const float eo = smooth_local(P, p, qx, qy, qz, px, py, pz, d0m, d0p);
         const float en = smooth_local(P, p, qx, qy, qz, nx, ny, nz, d0m, d0p);
         dE = en - eo;
@@ -319,6 +360,7 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
         best = E < best ? E : best;
       }
       if (active && !S.cool_per_block) T *= S.dt;
+    }
     }
     bool due = false;
     if (active) {
@@ -361,7 +403,7 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
     st->status[lane] = status;
   }
   if constexpr (STAGE) {
-    for (int i = 0; i < 3 * n; ++i) it.pos[static_cast<size_t>(i) * kLanes + lane] = lds[i * kLanes + lane];
+    for (int i = 0; i < 3 * n; ++i) it.pos[static_cast<size_t>(i) * kLanes + lane] = X.p[i * kLanes];
   }
 }
 
@@ -513,6 +555,42 @@ __global__ __launch_bounds__(256) void k_finalize_refs(const JobRef *__restrict_
 static constexpr int kStageMaxBytes = PMMC_STAGE_MAX_BYTES;
 
 bool stage_ok(int n) { return static_cast<size_t>(n) * 3 * kLanes * sizeof(float) <= kStageMaxBytes; }
+
+int resident_ensembles_per_cu(const HostProblem &h) {
+  static std::atomic<int> cache[16];
+  const bool hreg = !stage_ok(h.n) && hreg_halves(h) > 0;
+  const bool stage = stage_ok(h.n);
+  const int key = (h.level & 3) * 4 + (hreg ? 1 : 0) + (stage ? 2 : 0);
+  const int known = cache[key].load(std::memory_order_relaxed);
+  if (known > 0) return known;
+  int blocks = 0;
+  hipError_t e = hipErrorInvalidValue;
+  if (hreg) {
+    blocks = hreg_resident_per_cu(!h.symmetric);
+    e = blocks > 0 ? hipSuccess : hipErrorInvalidValue;
+  } else {
+    const size_t lds = stage ? static_cast<size_t>(h.n) * 3 * kLanes * sizeof(float) : 0;
+    switch (h.level) {
+      case PMMC_LEVEL_HEATMAP:
+        e = stage ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_anneal<PMMC_LEVEL_HEATMAP, true>, kLanes, lds)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_anneal<PMMC_LEVEL_HEATMAP, false>, kLanes, 0);
+        break;
+      case PMMC_LEVEL_ARCS:
+        e = stage ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_anneal<PMMC_LEVEL_ARCS, true>, kLanes, lds)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_anneal<PMMC_LEVEL_ARCS, false>, kLanes, 0);
+        break;
+      default:
+        e = stage ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_anneal<PMMC_LEVEL_SMOOTH, true>, kLanes, lds)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_anneal<PMMC_LEVEL_SMOOTH, false>, kLanes, 0);
+    }
+  }
+  if (e != hipSuccess || blocks <= 0) {
+    (void)hipGetLastError();
+    blocks = 16;
+  }
+  cache[key].store(blocks, std::memory_order_relaxed);
+  return blocks;
+}
 
 LaunchGroup group_of(const HostProblem &h) {
   LaunchGroup g{};
@@ -924,8 +1002,7 @@ __global__ __launch_bounds__(64) void k_delta(const DevProblem *__restrict__ pro
   const int lane = threadIdx.x;
   const int i = blockIdx.x;
   if (i >= n_prop) return;
-  PosView X;
-  X.p = ens[0].pos + lane;
+  const PosView X = global_view(ens[0].pos, lane);
   const int n = P.n;
   const int p = __builtin_amdgcn_readfirstlane(beads[i]);
   const float px = X.get(p, 0), py = X.get(p, 1), pz = X.get(p, 2);

@@ -40,7 +40,10 @@ pmmc_status prepare_problem(pmmc_ctx *ctx, int level, const void *problem, const
                             const pmmc_run &run, HostProblem &out);
 pmmc_status check_schedule(pmmc_ctx *ctx, const pmmc_schedule &s);
 int autotune_blocks_per_step(const std::vector<HostProblem> &hp);
-int default_chains(const pmmc_ctx *ctx, int level, int n);
+int default_chains(const pmmc_ctx *ctx, const HostProblem &h);
+// ensembles (one wavefront each) of h's annealing kernel one CU holds at once
+// (the runtime's occupancy figure for that kernel; 16 if the query fails)
+int resident_ensembles_per_cu(const HostProblem &h);
 
 // contiguous range of ensembles launched together with one kernel:
 // (level, LDS staging and its n, register-resident heatmap halves, asymmetry)
@@ -82,6 +85,7 @@ hipError_t launch_init_refs(hipStream_t s, const JobRef *jobs, int n_jobs, const
 hipError_t launch_finalize_refs(hipStream_t s, const JobRef *jobs, int n_jobs);
 // register-resident heatmap kernel (heatmap_reg.hip) for heatmaps of <= 64 beads
 int hreg_halves(const HostProblem &h);
+int hreg_resident_per_cu(bool asym);
 hipError_t launch_heatmap_reg(hipStream_t s, const DevProblem *probs, const DevEns *ens, int n_ens,
                               int max_blocks, int halves, bool asym);
 hipError_t launch_finalize(hipStream_t s, DevProblem *probs, int n_probs, const DevEns *ens);

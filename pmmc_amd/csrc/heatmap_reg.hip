@@ -173,7 +173,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       if (k == 0 || (m & 3u) == 0) rb = draw(seed, m >> 2, group, segment, PMMC_LEVEL_HEATMAP, kSubBead);
       const uint32_t sel = m & 3u;
       const uint32_t word = sel == 0 ? rb.x : (sel == 1 ? rb.y : (sel == 2 ? rb.z : rb.w));
-      const int p = __builtin_amdgcn_readfirstlane(P.free_list[pick(word, static_cast<uint32_t>(nfree))]);
+      const int p = cst(P.free_list)[pick(word, static_cast<uint32_t>(nfree))];
       float px = 0.f, py = 0.f, pz = 0.f;
       get_bead<NB>(X, Y, Z, p, px, py, pz);
       const U4 rc = draw(seed, m, chain, segment, PMMC_LEVEL_HEATMAP, kSubChain);
@@ -205,8 +205,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       const bool resum_now = S.resum > 0 && ((miles + 1) % S.resum) == 0;
       if (__ballot(due && resum_now) != 0) {
         store_all<NB>(g, n, X, Y, Z);
-        PosView V;
-        V.p = g;
+        const PosView V = global_view(it.pos, lane);
         Er = resum_heatmap(P, V, r);
       }
       if (due) {
@@ -237,6 +236,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 int hreg_halves(const HostProblem &h) {
   if (h.level != PMMC_LEVEL_HEATMAP || h.n <= 32 || h.n > 64) return 0;
   return h.ldw == 64 ? 2 : 0;
+}
+
+int hreg_resident_per_cu(bool asym) {
+  int blocks = 0;
+  const size_t lds = static_cast<size_t>(64) * 64 * sizeof(float) * (asym ? 2 : 1);
+  const hipError_t e =
+      asym ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_anneal_hreg<64, true>, kLanes, lds)
+           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_anneal_hreg<64, false>, kLanes, lds);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return blocks;
 }
 
 template <int NB, bool ASYM>

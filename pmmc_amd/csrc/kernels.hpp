@@ -28,12 +28,82 @@ __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int lane) {
   return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), lane));
 }
 
-// Position accessor: element (bead j, axis k) of this lane's chain.
-struct PosView {
-  float *p;  // base + lane
+#define PMMC_AS_GLOBAL __attribute__((address_space(1)))
+#define PMMC_AS_CONST __attribute__((address_space(4)))
+
+// A problem's static tables (free list, CSR, weight rows, dist_to_next) are
+// never written while a kernel runs.  Reading them through the constant
+// address space lets the compiler use scalar loads for wave-uniform
+// addresses (the moved bead and its CSR row are wave-uniform).
+template <class T>
+__device__ __forceinline__ const PMMC_AS_CONST T *cst(const T *p) {
+  return (const PMMC_AS_CONST T *)p;
+}
+
+// One CSR entry / pair entry read field by field through the constant
+// address space (scalar loads for a wave-uniform index).
+__device__ __forceinline__ ArcEntry load_entry(const ArcEntry *t, int i) {
+  const PMMC_AS_CONST ArcEntry *q = cst(t) + i;
+  ArcEntry e;
+  e.j = q->j;
+  e.c = q->c;
+  e.ka = q->ka;
+  e.kb = q->kb;
+  return e;
+}
+
+__device__ __forceinline__ PairEntry load_pair(const PairEntry *t, int i) {
+  const PMMC_AS_CONST PairEntry *q = cst(t) + i;
+  PairEntry e;
+  e.i = q->i;
+  e.e.j = q->e.j;
+  e.e.c = q->e.c;
+  e.e.ka = q->e.ka;
+  e.e.kb = q->e.kb;
+  return e;
+}
+
+// Position accessors: element (bead j, axis k) of this lane's chain.
+template <bool LDS>
+struct PosViewT;
+
+// positions in HBM: the ensemble's base (wave-uniform) and this lane's column,
+// so each partner costs one address computation and its three axes are
+// immediate offsets of one global load form
+template <>
+struct PosViewT<false> {
+  float PMMC_AS_GLOBAL *base;
+  unsigned lane;
+  __device__ __forceinline__ float get(int j, int k) const {
+    return (base + 3 * j * kLanes)[k * kLanes + lane];
+  }
+  __device__ __forceinline__ void set(int j, int k, float v) const {
+    (base + 3 * j * kLanes)[k * kLanes + lane] = v;
+  }
+};
+
+// positions staged in LDS: p = lds + lane
+template <>
+struct PosViewT<true> {
+  float *p;
   __device__ __forceinline__ float get(int j, int k) const { return p[(3 * j + k) * kLanes]; }
   __device__ __forceinline__ void set(int j, int k, float v) const { p[(3 * j + k) * kLanes] = v; }
 };
+
+using PosView = PosViewT<false>;
+
+__device__ __forceinline__ PosView global_view(float *pos, unsigned lane) {
+  PosView v;
+  v.base = (float PMMC_AS_GLOBAL *)pos;
+  v.lane = lane;
+  return v;
+}
+
+// Pick word of move m from the draw of its group of four moves.
+__device__ __forceinline__ uint32_t pick_word(const U4 &rb, uint32_t m) {
+  const uint32_t sel = m & 3u;
+  return sel == 0 ? rb.x : (sel == 1 ? rb.y : (sel == 2 ? rb.z : rb.w));
+}
 
 __device__ __forceinline__ bool metropolis(const pmmc_schedule &S, double E, double En, double T,
                                            float U) {
@@ -97,7 +167,8 @@ static_assert(kHeatChunk % 2 == 0 && kLanes % kHeatChunk == 0, "even chunk divid
 
 // heatmap: exact change of the full (double-counted) score for moving p from
 // xp to xn; LooperSolver.hpp:3583-3617 summed over all beads.
-__device__ __forceinline__ float heatmap_delta(const DevProblem &P, const PosView &X, int p,
+template <class View>
+__device__ __forceinline__ float heatmap_delta(const DevProblem &P, const View &X, int p,
                                                float px, float py, float pz, float nx, float ny,
                                                float nz) {
   const int n = P.n;
@@ -108,7 +179,7 @@ __device__ __forceinline__ float heatmap_delta(const DevProblem &P, const PosVie
   // their position loads are issued before any is consumed.  Partner j's term
   // goes to accumulator j%2, in increasing j; past the last partner the chunk
   // is padded with weight 0 (a term of exactly 0).
-  const float *__restrict__ w1 = P.W + static_cast<size_t>(p) * P.ldw;
+  const float PMMC_AS_GLOBAL *w1 = (const float PMMC_AS_GLOBAL *)P.W + static_cast<size_t>(p) * P.ldw;
   float acc0 = 0.f, acc1 = 0.f;
   for (int c = 0; c < n; c += kLanes) {
     const int cnt = min(kLanes, n - c);
@@ -137,7 +208,7 @@ __device__ __forceinline__ float heatmap_delta(const DevProblem &P, const PosVie
   }
   float d = acc0 + acc1;
   if (P.symmetric) return 2.f * d;
-  const float *__restrict__ w2 = P.W2 + static_cast<size_t>(p) * P.ldw;
+  const float PMMC_AS_GLOBAL *w2 = (const float PMMC_AS_GLOBAL *)P.W2 + static_cast<size_t>(p) * P.ldw;
   float acc2 = 0.f;
   for (int c = 0; c < n; c += kLanes) {
     const int cnt = min(kLanes, n - c);
@@ -163,37 +234,115 @@ __device__ __forceinline__ float heatmap_delta(const DevProblem &P, const PosVie
   return d + acc2;
 }
 
+// Arc + loop terms of one chunk of CSR entries of bead p, in entry order:
+// acc += (d1 - c)^2 k1 - (d0 - c)^2 k0 for the first `cnt` entries, with d0 /
+// d1 the distances from the old / new position of p to the partner.
+__device__ __forceinline__ void arcs_chunk(const ArcEntry (&en)[kArcsChunk], const float (&xj)[kArcsChunk],
+                                           const float (&yj)[kArcsChunk], const float (&zj)[kArcsChunk],
+                                           int cnt, float px, float py, float pz, float nx, float ny,
+                                           float nz, float &acc) {
+#pragma unroll
+  for (int u = 0; u < kArcsChunk; ++u) {
+    if (u < cnt) {
+      const float t0 = dist3(px, py, pz, xj[u], yj[u], zj[u]) - en[u].c;
+      const float t1 = dist3(nx, ny, nz, xj[u], yj[u], zj[u]) - en[u].c;
+      const float k0 = t0 >= 0.f ? en[u].ka : en[u].kb;
+      const float k1 = t1 >= 0.f ? en[u].ka : en[u].kb;
+      acc += t1 * t1 * k1 - t0 * t0 * k0;
+    }
+  }
+}
+
+// Entries [e, e + kArcsChunk) of bead p (clamped to e1 - 1) and the positions
+// of their partners.  The (wave-uniform) entry loads and then the
+// 3*kArcsChunk per-lane position loads are all issued before any is consumed.
+template <class View>
+__device__ __forceinline__ void arcs_load_chunk(const DevProblem &P, const View &X, int e, int e1,
+                                                ArcEntry (&en)[kArcsChunk], float (&xj)[kArcsChunk],
+                                                float (&yj)[kArcsChunk], float (&zj)[kArcsChunk]) {
+#pragma unroll
+  for (int u = 0; u < kArcsChunk; ++u) en[u] = load_entry(P.ent, min(e + u, e1 - 1));
+#pragma unroll
+  for (int u = 0; u < kArcsChunk; ++u) {
+    const int j = en[u].j & 0x7fffffff;
+    xj[u] = X.get(j, 0);
+    yj[u] = X.get(j, 1);
+    zj[u] = X.get(j, 2);
+  }
+}
+
 // arcs: change of arc + loop terms of bead p (ParallelMonteCarloArcs.cu:294-351,
 // LooperSolver.hpp:3250-3281, :3355-3371).  Term (d-c)^2 * (d >= c ? ka : kb).
-__device__ __forceinline__ float arcs_delta(const DevProblem &P, const PosView &X, int p, float px,
+template <class View>
+__device__ __forceinline__ float arcs_delta(const DevProblem &P, const View &X, int p, float px,
                                             float py, float pz, float nx, float ny, float nz) {
-  const int e0 = P.off[p], e1 = P.off[p + 1];
+  const int e0 = cst(P.off)[p], e1 = cst(P.off)[p + 1];
   float acc = 0.f;
-  // kArcsChunk entries per round trip: the (wave-uniform) entry loads and
-  // then the 3*kArcsChunk per-lane position loads are all issued before any
-  // is consumed.
   for (int e = e0; e < e1; e += kArcsChunk) {
     ArcEntry en[kArcsChunk];
     float xj[kArcsChunk], yj[kArcsChunk], zj[kArcsChunk];
+    arcs_load_chunk(P, X, e, e1, en, xj, yj, zj);
+    arcs_chunk(en, xj, yj, zj, e1 - e, px, py, pz, nx, ny, nz, acc);
+  }
+  return acc;
+}
+
+// Everything one arc-level move reads before its decision, fetched ahead of
+// time: the bead, its CSR range, its first kArcsChunk entries and the
+// positions of the bead and of those partners.
+struct ArcsFetch {
+  int p, e0, e1;
+  ArcEntry en[kArcsChunk];
+  float px, py, pz;
+  float xj[kArcsChunk], yj[kArcsChunk], zj[kArcsChunk];
+};
+
+template <class View>
+__device__ __forceinline__ void arcs_fetch(const DevProblem &P, const View &X, int p, ArcsFetch &f) {
+  f.p = p;
+  f.e0 = cst(P.off)[p];
+  f.e1 = cst(P.off)[p + 1];
+  f.px = X.get(p, 0);
+  f.py = X.get(p, 1);
+  f.pz = X.get(p, 2);
+  if (f.e1 > f.e0) arcs_load_chunk(P, X, f.e0, f.e1, f.en, f.xj, f.yj, f.zj);
+}
+
+// The prefetched move reads positions as they were before the current move;
+// when the current move changed bead q (accepted per lane: ok), the copies of
+// q among them take its new position.
+__device__ __forceinline__ void arcs_patch(ArcsFetch &f, int q, bool ok, float qx, float qy,
+                                           float qz) {
+  if (f.p == q) {
+    f.px = ok ? qx : f.px;
+    f.py = ok ? qy : f.py;
+    f.pz = ok ? qz : f.pz;
+  }
+  const int cnt = min(kArcsChunk, f.e1 - f.e0);
 #pragma unroll
-    for (int u = 0; u < kArcsChunk; ++u) en[u] = P.ent[min(e + u, e1 - 1)];
-#pragma unroll
-    for (int u = 0; u < kArcsChunk; ++u) {
-      const int j = en[u].j & 0x7fffffff;
-      xj[u] = X.get(j, 0);
-      yj[u] = X.get(j, 1);
-      zj[u] = X.get(j, 2);
+  for (int u = 0; u < kArcsChunk; ++u) {
+    if (u < cnt && (f.en[u].j & 0x7fffffff) == q) {
+      f.xj[u] = ok ? qx : f.xj[u];
+      f.yj[u] = ok ? qy : f.yj[u];
+      f.zj[u] = ok ? qz : f.zj[u];
     }
-#pragma unroll
-    for (int u = 0; u < kArcsChunk; ++u) {
-      if (e + u < e1) {
-        const float t0 = dist3(px, py, pz, xj[u], yj[u], zj[u]) - en[u].c;
-        const float t1 = dist3(nx, ny, nz, xj[u], yj[u], zj[u]) - en[u].c;
-        const float k0 = t0 >= 0.f ? en[u].ka : en[u].kb;
-        const float k1 = t1 >= 0.f ? en[u].ka : en[u].kb;
-        acc += t1 * t1 * k1 - t0 * t0 * k0;
-      }
-    }
+  }
+}
+
+// ΔE of a fetched move: the first chunk from the fetched registers, any
+// further chunks (beads with more than kArcsChunk entries) read now; same
+// terms and order as arcs_delta.
+template <class View>
+__device__ __forceinline__ float arcs_delta_fetched(const DevProblem &P, const View &X,
+                                                    const ArcsFetch &f, float nx, float ny,
+                                                    float nz) {
+  float acc = 0.f;
+  if (f.e1 > f.e0) arcs_chunk(f.en, f.xj, f.yj, f.zj, f.e1 - f.e0, f.px, f.py, f.pz, nx, ny, nz, acc);
+  for (int e = f.e0 + kArcsChunk; e < f.e1; e += kArcsChunk) {
+    ArcEntry en[kArcsChunk];
+    float xj[kArcsChunk], yj[kArcsChunk], zj[kArcsChunk];
+    arcs_load_chunk(P, X, e, f.e1, en, xj, yj, zj);
+    arcs_chunk(en, xj, yj, zj, f.e1 - e, f.px, f.py, f.pz, nx, ny, nz, acc);
   }
   return acc;
 }
@@ -252,13 +401,14 @@ __device__ __forceinline__ float smooth_local(const DevProblem &P, int p, const 
 // ------------------------------------------------------------------ //
 // milestone re-summation (FP32 terms, FP64 sum), per lane
 // ------------------------------------------------------------------ //
-__device__ double resum_heatmap(const DevProblem &P, const PosView &X, float r) {
+template <class View>
+__device__ double resum_heatmap(const DevProblem &P, const View &X, float r) {
   const int n = P.n;
   double tot = 0.0;
   const int lane = threadIdx.x & (kLanes - 1);
   for (int m = 0; m < n; ++m) {
     const float xm = X.get(m, 0), ym = X.get(m, 1), zm = X.get(m, 2);
-    const float *__restrict__ wr = P.W + static_cast<size_t>(m) * P.ldw;
+    const float PMMC_AS_GLOBAL *wr = (const float PMMC_AS_GLOBAL *)P.W + static_cast<size_t>(m) * P.ldw;
     float part = 0.f;
     const int j0 = P.symmetric ? m + 1 : 0;
     for (int c = j0; c < n; c += kLanes) {
@@ -284,7 +434,8 @@ __device__ double resum_heatmap(const DevProblem &P, const PosView &X, float r) 
 // round trip: the (wave-uniform) pair entries and then the 48 per-lane
 // position loads are all issued before any is consumed.  FP32 partial sums of
 // at most 64 pairs are added into the FP64 total.
-__device__ double resum_arcs(const DevProblem &P, const PosView &X, float r, double offset) {
+template <class View>
+__device__ double resum_arcs(const DevProblem &P, const View &X, float r, double offset) {
   const int n = P.n;
   const int np = P.npairs;
   double tot = offset;
@@ -293,7 +444,7 @@ __device__ double resum_arcs(const DevProblem &P, const PosView &X, float r, dou
     PairEntry pe[8];
     float xi[8], yi[8], zi[8], xj[8], yj[8], zj[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) pe[u] = P.pairs[min(q + u, np - 1)];
+    for (int u = 0; u < 8; ++u) pe[u] = load_pair(P.pairs, min(q + u, np - 1));
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int i = pe[u].i, j = pe[u].e.j & 0x7fffffff;
@@ -322,7 +473,8 @@ __device__ double resum_arcs(const DevProblem &P, const PosView &X, float r, dou
   return tot;
 }
 
-__device__ double resum_smooth(const DevProblem &P, const PosView &X, float r, double offset) {
+template <class View>
+__device__ double resum_smooth(const DevProblem &P, const View &X, float r, double offset) {
   const int n = P.n;
   double tot = offset;
   float part = 0.f;
@@ -332,7 +484,7 @@ __device__ double resum_smooth(const DevProblem &P, const PosView &X, float r, d
   float bx = X.get(0, 0), by = X.get(0, 1), bz = X.get(0, 2);
   for (int i = 0; i + 1 < n; ++i) {
     const float cx = X.get(i + 1, 0), cy = X.get(i + 1, 1), cz = X.get(i + 1, 2);
-    part += bond_f(dist3(bx, by, bz, cx, cy, cz), fmaxf(P.dtn[i], 1e-6f), P.ks, P.kq);
+    part += bond_f(dist3(bx, by, bz, cx, cy, cz), fmaxf(cst(P.dtn)[i], 1e-6f), P.ks, P.kq);
     if (i >= 1) part += angle_f(ax - bx, ay - by, az - bz, bx - cx, by - cy, bz - cz, P.ka);
     ax = bx;
     ay = by;
