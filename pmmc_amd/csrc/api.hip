@@ -28,6 +28,7 @@ extern "C" pmmc_status pmmc_ctx_create(int device, pmmc_ctx **out) {
   pmmc_ctx *c = new pmmc_ctx();
   c->device = device;
   c->cu_count = prop.multiProcessorCount;
+  c->hbm_bytes = prop.totalGlobalMem;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     delete c;
@@ -174,13 +175,15 @@ int default_chains(const pmmc_ctx *ctx, const HostProblem &h) {
   // at least sixteen per CU -- enough wavefronts to hide the HBM latency of
   // the partner loads, and no partial last round (a partial round runs its
   // few waves alone, which measured 10-20% slower) -- capped so the chains'
-  // positions take at most 32 GiB of HBM.
+  // positions take at most a third of the device's HBM (96 GB on MI355X; a
+  // 20,000-bead smooth problem needs 78 GB for one full round).
   const int cus = ctx && ctx->cu_count > 0 ? ctx->cu_count : 256;
   const int res = resident_ensembles_per_cu(h);
   const int rounds = (16 + res - 1) / res;
   const size_t per_ens = static_cast<size_t>(h.n > 0 ? h.n : 1) * 3 * kLanes * sizeof(float);
   size_t ens = static_cast<size_t>(res) * rounds * cus;
-  const size_t cap = (size_t(32) << 30) / per_ens;
+  const size_t hbm = ctx && ctx->hbm_bytes > 0 ? ctx->hbm_bytes : (size_t(96) << 30) * 3;
+  const size_t cap = (hbm / 3) / per_ens;
   if (ens > cap) ens = cap > 0 ? cap : 1;
   return static_cast<int>(ens) * kLanes;
 }

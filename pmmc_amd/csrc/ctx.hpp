@@ -16,6 +16,7 @@
 struct pmmc_ctx {
   int device = 0;
   int cu_count = 0;
+  size_t hbm_bytes = 0;  // the device's global memory
   hipStream_t stream = nullptr;
   uint64_t default_seed = 0;
   std::string last_error;

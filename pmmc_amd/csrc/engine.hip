@@ -313,6 +313,45 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
         if (more) arcs_patch(f, c.p, ok, nx, ny, nz);
         if (active && !S.cool_per_block) T *= S.dt;
       }
+    } else if constexpr (LEVEL == PMMC_LEVEL_SMOOTH && !STAGE) {
+      // Software-pipelined smooth moves, as the arc moves above: the window
+      // of move m+1 is fetched before move m is evaluated and patched after
+      // its decision.
+      U4 rbn = draw(seed, m0 >> 2, group, segment, LEVEL, kSubBead);
+      SmoothFetch f;
+      smooth_fetch(P, X, cst(P.free_list)[pick(pick_word(rbn, m0), static_cast<uint32_t>(nfree))], f);
+      for (int k = 0; k < S.block; ++k) {
+        const uint32_t m = m0 + static_cast<uint32_t>(k);
+        const SmoothFetch c = f;
+        const bool more = k + 1 < S.block;
+        if (more) {
+          const uint32_t mn = m + 1;
+          if ((mn & 3u) == 0) rbn = draw(seed, mn >> 2, group, segment, LEVEL, kSubBead);
+          smooth_fetch(P, X, cst(P.free_list)[pick(pick_word(rbn, mn), static_cast<uint32_t>(nfree))], f);
+        }
+        const float px = c.qx[2], py = c.qy[2], pz = c.qz[2];
+        const U4 rc = draw(seed, m, chain, segment, LEVEL, kSubChain);
+        const float dx = disp(rc.x, step), dy = disp(rc.y, step);
+        const float dz = S.use2d ? 0.f : disp(rc.z, step);
+        const float nx = move_coord(px, dx), ny = move_coord(py, dy), nz = move_coord(pz, dz);
+        const float eo = smooth_local(P, c.p, c.qx, c.qy, c.qz, px, py, pz, c.d0m, c.d0p);
+        const float en = smooth_local(P, c.p, c.qx, c.qy, c.qz, nx, ny, nz, c.d0m, c.d0p);
+        float dE = en - eo;
+        if (r > 0.f) dE += sphere_penalty(nx, ny, nz, r) - sphere_penalty(px, py, pz, r);
+        const double En = E + static_cast<double>(dE);
+        const bool ok = active && metropolis(S, E, En, T, u01(rc.w));
+        if (ok) {
+          X.set(c.p, 0, nx);
+          X.set(c.p, 1, ny);
+          X.set(c.p, 2, nz);
+          E = En;
+          ++msucc;
+          ++acc;
+          best = E < best ? E : best;
+        }
+        if (more) smooth_patch(f, c.p, ok, nx, ny, nz);
+        if (active && !S.cool_per_block) T *= S.dt;
+      }
     } else {
     U4 rb = {0, 0, 0, 0};
     for (int k = 0; k < S.block; ++k) {

@@ -398,6 +398,48 @@ __device__ __forceinline__ float smooth_local(const DevProblem &P, int p, const 
   return s;
 }
 
+// Everything one smooth-level move reads before its decision, fetched ahead
+// of time: the bead, the positions of beads p-2..p+2 (index 2 = p; zero
+// outside the chain) and the two rest lengths of its bonds.
+struct SmoothFetch {
+  int p;
+  float qx[5], qy[5], qz[5];
+  float d0m, d0p;
+};
+
+template <class View>
+__device__ __forceinline__ void smooth_fetch(const DevProblem &P, const View &X, int p, SmoothFetch &f) {
+  const int n = P.n;
+  f.p = p;
+#pragma unroll
+  for (int o = 0; o < 5; ++o) {
+    const int idx = p - 2 + o;
+    const bool in = idx >= 0 && idx < n;
+    f.qx[o] = in ? X.get(idx, 0) : 0.f;
+    f.qy[o] = in ? X.get(idx, 1) : 0.f;
+    f.qz[o] = in ? X.get(idx, 2) : 0.f;
+  }
+  f.d0m = p > 0 ? fmaxf(cst(P.dtn)[p - 1], 1e-6f) : 1.f;
+  f.d0p = p < n - 1 ? fmaxf(cst(P.dtn)[p], 1e-6f) : 1.f;
+}
+
+// The fetched window holds positions from before the current move; when the
+// current move changed bead q (per lane: ok), its copy takes the new position.
+__device__ __forceinline__ void smooth_patch(SmoothFetch &f, int q, bool ok, float x, float y,
+                                             float z) {
+  const int o = q - (f.p - 2);
+  if (o >= 0 && o < 5) {
+#pragma unroll
+    for (int u = 0; u < 5; ++u) {
+      if (u == o) {
+        f.qx[u] = ok ? x : f.qx[u];
+        f.qy[u] = ok ? y : f.qy[u];
+        f.qz[u] = ok ? z : f.qz[u];
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ //
 // milestone re-summation (FP32 terms, FP64 sum), per lane
 // ------------------------------------------------------------------ //
