@@ -282,6 +282,12 @@ static pmmc_status prep_arcs(pmmc_ctx *ctx, const pmmc_arcs_problem *p, HostProb
   for (int i = 0; i < n; ++i) h.off[i + 1] = h.off[i] + static_cast<int32_t>(rows[i].size());
   h.ent.reserve(h.off[n]);
   for (int i = 0; i < n; ++i) h.ent.insert(h.ent.end(), rows[i].begin(), rows[i].end());
+  for (const ArcEntry &e : h.ent) {
+    h.ej.push_back(e.j);
+    h.ec.push_back(e.c);
+    h.eka.push_back(e.ka);
+    h.ekb.push_back(e.kb);
+  }
   for (int i = 0; i < n; ++i)
     for (const ArcEntry &e : rows[i])
       if ((e.j & INT32_MAX) > i) h.pairs.push_back(PairEntry{i, e});

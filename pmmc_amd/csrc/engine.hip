@@ -728,6 +728,10 @@ static void size_problem(Sizer &z, const HostProblem &h) {
   z.add<float>(h.Hc.size());
   z.add<int32_t>(h.off.size());
   z.add<ArcEntry>(std::max<size_t>(1, h.ent.size()));
+  z.add<int32_t>(std::max<size_t>(1, h.ej.size()));
+  z.add<float>(std::max<size_t>(1, h.ec.size()));
+  z.add<float>(std::max<size_t>(1, h.eka.size()));
+  z.add<float>(std::max<size_t>(1, h.ekb.size()));
   z.add<PairEntry>(std::max<size_t>(1, h.pairs.size()));
   z.add<int32_t>(std::max<size_t>(1, h.zpairs.size()));
   z.add<float>(h.dtn.size());
@@ -769,6 +773,10 @@ void fill_problem(Carver &cv, hipStream_t s, const HostProblem &h, int index, De
   P.Hc = put(cv, s, h.Hc, err);
   P.off = put(cv, s, h.off, err);
   P.ent = put(cv, s, h.ent, err, 1);
+  P.ej = put(cv, s, h.ej, err, 1);
+  P.ec = put(cv, s, h.ec, err, 1);
+  P.eka = put(cv, s, h.eka, err, 1);
+  P.ekb = put(cv, s, h.ekb, err, 1);
   P.pairs = put(cv, s, h.pairs, err, 1);
   P.npairs = static_cast<int32_t>(h.pairs.size());
   P.zpairs = put(cv, s, h.zpairs, err, 1);

@@ -60,6 +60,8 @@ struct DevProblem {
   // arcs
   const int32_t *off;        // [n+1]
   const ArcEntry *ent;       // [nnz]
+  const int32_t *ej;         // [nnz] the entries' fields as separate arrays:
+  const float *ec, *eka, *ekb;  // partner | loop<<31, c, ka, kb
   const int32_t *zpairs;     // [2*nz] off-diagonal pairs with 0 <= e < 1e-6 (i<j)
   const PairEntry *pairs;    // [npairs] the CSR entries with i < j, row order
   int32_t nnz, nz, npairs, pad2;

@@ -24,6 +24,9 @@ struct HostProblem {
   // arcs CSR
   std::vector<int32_t> off;
   std::vector<ArcEntry> ent;
+  // the same entries as separate arrays (partner | loop<<31, c, ka, kb)
+  std::vector<int32_t> ej;
+  std::vector<float> ec, eka, ekb;
   std::vector<PairEntry> pairs;
   std::vector<int32_t> zpairs;
   float ks = 0.f, kq = 0.f;

@@ -287,12 +287,14 @@ __device__ __forceinline__ float arcs_delta(const DevProblem &P, const View &X, 
   return acc;
 }
 
-// Everything one arc-level move reads before its decision, fetched ahead of
-// time: the bead, its CSR range, its first kArcsChunk entries and the
-// positions of the bead and of those partners.
+// Everything one arc-level move reads from the chains' state before its
+// decision, fetched ahead of time: the bead, its CSR range, the partner
+// indices of its first kArcsChunk entries (slots past the range repeat the
+// last entry) and the positions of the bead and of those partners.  The
+// entries' coefficients are static and are read when the move is evaluated.
 struct ArcsFetch {
   int p, e0, e1;
-  ArcEntry en[kArcsChunk];
+  int j[kArcsChunk];
   float px, py, pz;
   float xj[kArcsChunk], yj[kArcsChunk], zj[kArcsChunk];
 };
@@ -305,39 +307,59 @@ __device__ __forceinline__ void arcs_fetch(const DevProblem &P, const View &X, i
   f.px = X.get(p, 0);
   f.py = X.get(p, 1);
   f.pz = X.get(p, 2);
-  if (f.e1 > f.e0) arcs_load_chunk(P, X, f.e0, f.e1, f.en, f.xj, f.yj, f.zj);
-}
-
-// The prefetched move reads positions as they were before the current move;
-// when the current move changed bead q (accepted per lane: ok), the copies of
-// q among them take its new position.
-__device__ __forceinline__ void arcs_patch(ArcsFetch &f, int q, bool ok, float qx, float qy,
-                                           float qz) {
-  if (f.p == q) {
-    f.px = ok ? qx : f.px;
-    f.py = ok ? qy : f.py;
-    f.pz = ok ? qz : f.pz;
-  }
-  const int cnt = min(kArcsChunk, f.e1 - f.e0);
+  if (f.e1 > f.e0) {
 #pragma unroll
-  for (int u = 0; u < kArcsChunk; ++u) {
-    if (u < cnt && (f.en[u].j & 0x7fffffff) == q) {
-      f.xj[u] = ok ? qx : f.xj[u];
-      f.yj[u] = ok ? qy : f.yj[u];
-      f.zj[u] = ok ? qz : f.zj[u];
+    for (int u = 0; u < kArcsChunk; ++u) f.j[u] = cst(P.ej)[min(f.e0 + u, f.e1 - 1)] & 0x7fffffff;
+#pragma unroll
+    for (int u = 0; u < kArcsChunk; ++u) {
+      f.xj[u] = X.get(f.j[u], 0);
+      f.yj[u] = X.get(f.j[u], 1);
+      f.zj[u] = X.get(f.j[u], 2);
     }
   }
 }
 
-// ΔE of a fetched move: the first chunk from the fetched registers, any
-// further chunks (beads with more than kArcsChunk entries) read now; same
-// terms and order as arcs_delta.
+// The prefetched move reads positions as they were before the current move;
+// when the current move changed bead q (accepted per lane: ok), the copies of
+// q among them take its new position.  Selects, no branches: a branch per
+// slot makes every fetched value a merge and costs register copies.
+__device__ __forceinline__ void arcs_patch(ArcsFetch &f, int q, bool ok, float qx, float qy,
+                                           float qz) {
+  const bool self = ok && f.p == q;
+  f.px = self ? qx : f.px;
+  f.py = self ? qy : f.py;
+  f.pz = self ? qz : f.pz;
+  const int cnt = min(kArcsChunk, f.e1 - f.e0);
+#pragma unroll
+  for (int u = 0; u < kArcsChunk; ++u) {
+    const bool hit = ok && u < cnt && f.j[u] == q;
+    f.xj[u] = hit ? qx : f.xj[u];
+    f.yj[u] = hit ? qy : f.yj[u];
+    f.zj[u] = hit ? qz : f.zj[u];
+  }
+}
+
+// ΔE of a fetched move: the first chunk from the fetched positions (its
+// coefficients read now), any further chunks (beads with more than
+// kArcsChunk entries) read now; same terms and order as arcs_delta.
 template <class View>
 __device__ __forceinline__ float arcs_delta_fetched(const DevProblem &P, const View &X,
                                                     const ArcsFetch &f, float nx, float ny,
                                                     float nz) {
   float acc = 0.f;
-  if (f.e1 > f.e0) arcs_chunk(f.en, f.xj, f.yj, f.zj, f.e1 - f.e0, f.px, f.py, f.pz, nx, ny, nz, acc);
+  const int cnt = min(kArcsChunk, f.e1 - f.e0);
+#pragma unroll
+  for (int u = 0; u < kArcsChunk; ++u) {
+    if (u < cnt) {
+      const int e = f.e0 + u;
+      const float c = cst(P.ec)[e], ka = cst(P.eka)[e], kb = cst(P.ekb)[e];
+      const float t0 = dist3(f.px, f.py, f.pz, f.xj[u], f.yj[u], f.zj[u]) - c;
+      const float t1 = dist3(nx, ny, nz, f.xj[u], f.yj[u], f.zj[u]) - c;
+      const float k0 = t0 >= 0.f ? ka : kb;
+      const float k1 = t1 >= 0.f ? ka : kb;
+      acc += t1 * t1 * k1 - t0 * t0 * k0;
+    }
+  }
   for (int e = f.e0 + kArcsChunk; e < f.e1; e += kArcsChunk) {
     ArcEntry en[kArcsChunk];
     float xj[kArcsChunk], yj[kArcsChunk], zj[kArcsChunk];
