@@ -153,15 +153,12 @@ def cpu_baseline(prob, s, step, budget_s=15.0):
                       f"{secs:.1f} s on {threads} host threads"}
 
 
-def levels(ctx, with_cpu=True, cpu_seconds=8.0):
-    """Plain heatmap (64 beads; the CPU annealer's schedule, as the workload is
-    specified) and smooth (19,951 beads; the GPU kernel's schedule) workloads,
-    one solve each after a warm-up."""
+def heatmap_leg(ctx, with_cpu=True, cpu_seconds=8.0):
+    """Plain heatmap workload, 64 beads (init U[0,10)^3, target from a
+    random-walk polymer), on the schedule of the CPU path the workload is
+    specified on (LooperSolver::MonteCarloHeatmap: 50k-move milestones,
+    cooling per move): one solve after a warm-up."""
     from pmmc_amd import metrics, schedule_preset, settings, synth
-    out = {}
-    # heatmap, 64 beads (init U[0,10)^3, target from a random-walk polymer):
-    # the schedule of the CPU path the workload is specified on
-    # (LooperSolver::MonteCarloHeatmap: 50k-move milestones, cooling per move)
     prob, step, truth = synth.config_case(1, 0)
     sched = schedule_preset(1, 0, settings(**synth.CONFIG_SETTINGS[1]), step)
     chains = 131072  # one round of the register-resident kernel (8 ensembles per CU)
@@ -182,8 +179,14 @@ def levels(ctx, with_cpu=True, cpu_seconds=8.0):
     if with_cpu:
         h["cpu_reference"] = cpu_reference(1, prob, step, synth.CONFIG_SETTINGS[1], cpu_seconds,
                                            sample_moves=20000)
-    out["heatmap_64"] = h
-    # smooth, 19,951 beads (400 fixed anchors, 49 beads between neighbours)
+    return h
+
+
+def smooth_leg(ctx, with_cpu=True, cpu_seconds=8.0):
+    """Plain smooth workload, 19,951 beads (400 fixed anchors, 49 beads
+    between neighbours), on the GPU kernel's schedule: one solve after a
+    warm-up solve."""
+    from pmmc_amd import schedule_preset, settings, synth
     prob, step, _ = synth.config_case(3, 0)
     sm = dict(synth.CONFIG_SETTINGS[3])
     sched = schedule_preset(3, 1, settings(**sm), step)
@@ -192,20 +195,32 @@ def levels(ctx, with_cpu=True, cpu_seconds=8.0):
     pos, st = ctx.anneal(prob, sched, seed=2, segment=1)
     wall = time.perf_counter() - t0
     acc = st["accepted"] / max(1, st["moves"])
-    bpm = 12.0 * 5 + 8.0 + 12.0 * acc
+    # per chain-move: the 5-bead window (12 B each), the two rest lengths and
+    # the free-list entry of the pick (12 B, one scalar read shared by the 64
+    # chains of an ensemble) and the 12 B write of an accepted move
+    bpm = 12.0 * 5 + 12.0 / 64 + 12.0 * acc
     kms = st["kernel_ms"] * 1e-3
-    out["smooth_19951"] = {
+    out = {
         "workload": "plain smooth-level SA, 19,951 beads, 200k-move cap, cooling 0.9995, "
                     "GPU-kernel schedule", "chains": st["n_chains"],
         "moves_per_s": st["moves"] / wall, "accepted_moves_per_s": st["accepted"] / wall,
         "kernel_moves_per_s": st["moves"] / kms, "solve_s": wall,
         "final_energy": st["energy_resummed"],
         "bytes_per_move": bpm, "achieved_GBps": bpm * st["moves"] / kms / 1e9,
-        "bound": "HBM latency (5-bead window, dependent loads)"}
+        "bound": "VALU and HBM together (~500 VALU instructions per 64-chain move; "
+                 "3.8 KB window read per 64-chain move)"}
     if with_cpu:
-        out["smooth_19951"]["cpu_reference"] = cpu_reference(3, prob, step, sm, cpu_seconds,
-                                                             sample_moves=2000000)
+        out["cpu_reference"] = cpu_reference(3, prob, step, sm, cpu_seconds,
+                                             sample_moves=2000000)
     return out
+
+
+LEGS = {"heatmap_64": heatmap_leg, "smooth_19951": smooth_leg}
+
+
+def levels(ctx, with_cpu=True, cpu_seconds=8.0):
+    """The heatmap and smooth workloads, one solve each."""
+    return {k: f(ctx, with_cpu, cpu_seconds) for k, f in LEGS.items()}
 
 
 def batched(ctx, chains_per_problem=256, plain_subset=16):
@@ -365,6 +380,8 @@ def main(argv=None):
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-levels", action="store_true")
+    ap.add_argument("--only", choices=sorted(LEGS),
+                    help="run one level workload alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB three times (device to device), the known-size "
                          "transfer that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -378,6 +395,11 @@ def main(argv=None):
 
     import numpy as np
     from pmmc_amd import Context, schedule_preset
+
+    if a.only:
+        with Context(0) as ctx:
+            print(json.dumps({a.only: LEGS[a.only](ctx, with_cpu=not a.no_cpu)}))
+        return 0
 
     prob, s, step, arcs = workload()
     sched = schedule_preset(2, 1, s, step)

@@ -38,10 +38,11 @@ class OrcSchedule(C.Structure):
 class OrcStats(C.Structure):
     _fields_ = [("energy", C.c_double), ("best_energy", C.c_double), ("T", C.c_double),
                 ("moves", C.c_int64), ("accepted", C.c_int64), ("milestones", C.c_int32),
-                ("stop_reason", C.c_int32)]
+                ("stop_reason", C.c_int32), ("near_ties", C.c_int32), ("pad_", C.c_int32),
+                ("first_near_tie", C.c_int64)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
 
 
 class OrcHeatmap(C.Structure):
@@ -98,11 +99,11 @@ def load():
         L.orc_anneal_chain.restype = C.c_int64
         L.orc_anneal_chain.argtypes = [C.c_int, C.c_void_p, C.POINTER(OrcSchedule), C.c_uint64,
                                        C.c_uint32, C.c_uint32, C.c_int, f32p,
-                                       C.POINTER(OrcStats), u8p, C.c_int]
+                                       C.POINTER(OrcStats), u8p, C.c_int, C.c_double]
         L.orc_anneal_many.restype = C.c_double
         L.orc_anneal_many.argtypes = [C.c_int, C.c_void_p, C.POINTER(OrcSchedule), C.c_uint64,
                                       C.c_uint32, C.c_uint32, C.c_int, C.c_int, f32p, f32p,
-                                      C.POINTER(OrcStats), C.c_int, C.c_int]
+                                      C.POINTER(OrcStats), C.c_int, C.c_int, C.c_double]
         L.orc_heatmap_sweep.restype = C.c_int
         L.orc_heatmap_sweep.argtypes = [C.POINTER(OrcHeatmap), C.POINTER(OrcSweep), C.c_uint64,
                                         C.c_uint32, f32p, C.POINTER(OrcStats)]
@@ -200,26 +201,26 @@ class Problem:
                 f(C.byref(self.c), _p(x, f32p), p, _p(q, f32p), r))
 
     def anneal(self, sched: OrcSchedule, seed, segment, chain, pick_shift=6, pos=None,
-               trace=False, full_rescore=0):
+               trace=False, full_rescore=0, tie_rel=0.0):
         L = load()
         x = (self.pos0 if pos is None else np.ascontiguousarray(pos, np.float32).reshape(-1)).copy()
         st = OrcStats()
         tr = np.zeros(int(min(sched.max_moves + sched.block, 1 << 26)), np.uint8) if trace else None
         L.orc_anneal_chain(self.level, self.ptr(), C.byref(sched), seed, segment, chain, pick_shift,
-                           _p(x, f32p), C.byref(st), _p(tr, u8p), full_rescore)
+                           _p(x, f32p), C.byref(st), _p(tr, u8p), full_rescore, float(tie_rel))
         d = st.as_dict()
         if trace:
             d["trace"] = tr[:d["moves"]].copy()
         return x.reshape(-1, 3), d
 
     def anneal_many(self, sched, seed, segment, chain0, n_chains, n_threads, pick_shift=6,
-                    full_rescore=0):
+                    full_rescore=0, tie_rel=0.0):
         L = load()
         out = np.zeros(n_chains * self.n * 3, np.float32)
         stats = (OrcStats * n_chains)()
         secs = L.orc_anneal_many(self.level, self.ptr(), C.byref(sched), seed, segment, chain0,
                                  n_chains, pick_shift, _p(self.pos0, f32p), _p(out, f32p), stats,
-                                 n_threads, full_rescore)
+                                 n_threads, full_rescore, float(tie_rel))
         return out.reshape(n_chains, self.n, 3), [s.as_dict() for s in stats], secs
 
 

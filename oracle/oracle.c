@@ -355,10 +355,34 @@ static double ctx_delta(const chain_ctx *c, float *pos, int p, const float *np) 
   }
 }
 
+/* |local before| + |local after| of moving bead p to np: the magnitude an
+ * FP32 evaluation of the move's ΔE is formed from */
+static double ctx_scale(const chain_ctx *c, float *pos, int p, const float *np) {
+  switch (c->level) {
+    case ORC_LEVEL_HEATMAP: {
+      const orc_heatmap *P = (const orc_heatmap *)c->prob;
+      const double a = orc_heatmap_local(P, pos, p);
+      float save[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+      pos[3 * p] = np[0]; pos[3 * p + 1] = np[1]; pos[3 * p + 2] = np[2];
+      const double b = orc_heatmap_local(P, pos, p);
+      pos[3 * p] = save[0]; pos[3 * p + 1] = save[1]; pos[3 * p + 2] = save[2];
+      return 2.0 * (fabs(a) + fabs(b));
+    }
+    case ORC_LEVEL_ARCS: {
+      const orc_arcs *P = (const orc_arcs *)c->prob;
+      return fabs(orc_arcs_local(P, pos, p, NULL, c->r)) + fabs(orc_arcs_local(P, pos, p, np, c->r));
+    }
+    default: {
+      const orc_smooth *P = (const orc_smooth *)c->prob;
+      return fabs(orc_smooth_local(P, pos, p, NULL, c->r)) + fabs(orc_smooth_local(P, pos, p, np, c->r));
+    }
+  }
+}
+
 int64_t orc_anneal_chain(int level, const void *problem, const orc_schedule *s,
                          uint64_t seed, uint32_t segment, uint32_t chain, int pick_shift,
                          float *pos, orc_stats *st, uint8_t *accept_trace,
-                         int full_rescore) {
+                         int full_rescore, double tie_rel) {
   chain_ctx c;
   memset(&c, 0, sizeof(c));
   c.level = level;
@@ -395,6 +419,9 @@ int64_t orc_anneal_chain(int level, const void *problem, const orc_schedule *s,
   int32_t msucc = 0, misses = 0, milestones = 0;
   double best = E;
   int reason = 0;
+  int32_t ties = 0;
+  int64_t first_tie = -1;
+  double drift = 0.0;  /* tie model: tolerance summed over accepted moves since a re-sum */
 
   if (!(E == E) || E < s->guard_energy) reason = 4;
   else if (nfree == 0) reason = 5;
@@ -415,10 +442,27 @@ int64_t orc_anneal_chain(int level, const void *problem, const orc_schedule *s,
       double dE = ctx_delta(&c, pos, p, np);
       double En = E + dE;
       int ok = s->strict_accept ? (En < E) : (En <= E);
-      if (!ok && T > 0.0 && (!s->positive_guard || E > 0.0)) {
-        double tp = s->tjs * exp(-s->tjc * (En / E) / T);
-        ok = (double)u01(r0[3]) < tp;
+      double tol = 0.0;
+      int tie = 0;
+      if (tie_rel > 0.0) {
+        tol = tie_rel * ctx_scale(&c, pos, p, np) + ORC_TIE_ABS;
+        tie = fabs(dE) <= tol;
       }
+      if (!ok && T > 0.0 && (!s->positive_guard || E > 0.0)) {
+        const double arg = s->tjc * (En / E) / T;
+        double tp = s->tjs * exp(-arg);
+        const double U = (double)u01(r0[3]);
+        ok = U < tp;
+        if (tie_rel > 0.0) {
+          const double B = tie_rel * (1.0 + fabs(arg)) + (s->tjc / T) * (tol + drift) / fabs(E);
+          tie = tie || fabs(U - tp) <= B * tp;
+        }
+      }
+      if (tie) {
+        if (first_tie < 0) first_tie = iters;
+        ++ties;
+      }
+      if (ok && tie_rel > 0.0) drift += tol;
       if (ok) {
         pos[3 * p] = np[0]; pos[3 * p + 1] = np[1]; pos[3 * p + 2] = np[2];
         E = En;
@@ -434,7 +478,10 @@ int64_t orc_anneal_chain(int level, const void *problem, const orc_schedule *s,
     step *= s->step_decay;
     if ((iters % s->milestone_period) < s->block) {
       ++milestones;
-      if (s->resum > 0 && milestones % s->resum == 0) E = ctx_total(&c, pos);
+      if (s->resum > 0 && milestones % s->resum == 0) {
+        E = ctx_total(&c, pos);
+        drift = 0.0;
+      }
       if (!isfinite(E)) { reason = 6; break; }
       int no_imp = (E > s->improvement * ms && msucc < s->min_successes) || E < s->small_energy ||
                    (s->ratio_stop > 0.0 && ms > 1e-10 && E / ms > s->ratio_stop);
@@ -458,6 +505,8 @@ int64_t orc_anneal_chain(int level, const void *problem, const orc_schedule *s,
   st->accepted = acc;
   st->milestones = milestones;
   st->stop_reason = reason;
+  st->near_ties = ties;
+  st->first_near_tie = first_tie;
   return iters;
 }
 
@@ -471,6 +520,7 @@ typedef struct {
   uint64_t seed;
   uint32_t segment, chain0;
   int n_chains, n, full_rescore, stride, pick_shift;
+  double tie_rel;
   int tid;
   const float *pos0;
   float *pos_out;
@@ -483,7 +533,7 @@ static void *many_worker(void *vp) {
     float *p = a->pos_out + (size_t)c * a->n * 3;
     memcpy(p, a->pos0, sizeof(float) * 3 * (size_t)a->n);
     orc_anneal_chain(a->level, a->problem, a->s, a->seed, a->segment, a->chain0 + (uint32_t)c,
-                     a->pick_shift, p, &a->stats[c], NULL, a->full_rescore);
+                     a->pick_shift, p, &a->stats[c], NULL, a->full_rescore, a->tie_rel);
   }
   return NULL;
 }
@@ -491,7 +541,7 @@ static void *many_worker(void *vp) {
 double orc_anneal_many(int level, const void *problem, const orc_schedule *s,
                        uint64_t seed, uint32_t segment, uint32_t chain0, int n_chains,
                        int pick_shift, const float *pos0, float *pos_out, orc_stats *stats,
-                       int n_threads, int full_rescore) {
+                       int n_threads, int full_rescore, double tie_rel) {
   int n = level == ORC_LEVEL_HEATMAP ? ((const orc_heatmap *)problem)->n
         : level == ORC_LEVEL_ARCS    ? ((const orc_arcs *)problem)->n
                                      : ((const orc_smooth *)problem)->n;
@@ -503,7 +553,7 @@ double orc_anneal_many(int level, const void *problem, const orc_schedule *s,
   clock_gettime(CLOCK_MONOTONIC, &t0);
   for (int t = 0; t < n_threads; ++t) {
     many_arg a = {level, problem, s, seed, segment, chain0, n_chains, n, full_rescore,
-                  n_threads, pick_shift, t, pos0, pos_out, stats};
+                  n_threads, pick_shift, tie_rel, t, pos0, pos_out, stats};
     args[t] = a;
     pthread_create(&th[t], NULL, many_worker, &args[t]);
   }

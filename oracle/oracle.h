@@ -79,6 +79,9 @@ typedef struct {
   int64_t accepted;
   int32_t milestones;
   int32_t stop_reason;   /* 0 running,1 stop rule,2 small,3 cap,4 guard,5 no free bead,6 non-finite */
+  int32_t near_ties;     /* decisions within FP32 error of their threshold (track_ties) */
+  int32_t pad_;
+  int64_t first_near_tie;  /* move index of the first, -1 if none          */
 } orc_stats;
 
 /* ---- Philox4x32-10 ---- */
@@ -155,11 +158,21 @@ double orc_smooth_local(const orc_smooth *P, const float *pos, int p, const floa
  * smooth: running total = weighted(initial) + Σ unweighted local deltas
  *   (MonteCarloArcsSmooth LooperSolver.hpp:4798-5047; .cu:231-349).
  * accept_trace (nullable, length >= moves): 1 per accepted move.
+ * tie_rel > 0 counts near ties (st->near_ties, st->first_near_tie):
+ *   decisions that an FP32 evaluation of the move with relative error
+ *   tie_rel could take the other way.  A move is one when |dE| <= tie_rel *
+ *   (|local before| + |local after|) + 1e-7, or, when the Metropolis test
+ *   decides, when |U - p| <= B * p with p the acceptance probability and B
+ *   its relative error carried through arg = tjc (E'/E) / T: tie_rel
+ *   (1 + |arg|) plus (tjc / T) times the ΔE error and the running total's
+ *   drift (the ΔE errors summed over accepted moves since the last
+ *   re-summation), over |E|.
  * Returns the number of counted moves. */
+#define ORC_TIE_ABS 1e-7
 int64_t orc_anneal_chain(int level, const void *problem, const orc_schedule *s,
                          uint64_t seed, uint32_t segment, uint32_t chain, int pick_shift,
                          float *pos, orc_stats *st, uint8_t *accept_trace,
-                         int full_rescore);
+                         int full_rescore, double tie_rel);
 
 /* Run n_chains chains (chain ids chain0..chain0+n_chains-1), each from the
  * same initial positions, on n_threads host threads.  pos_out [n_chains*n*3]
@@ -168,7 +181,7 @@ double orc_anneal_many(int level, const void *problem, const orc_schedule *s,
                        uint64_t seed, uint32_t segment, uint32_t chain0, int n_chains,
                        int pick_shift,
                        const float *pos0, float *pos_out, orc_stats *stats,
-                       int n_threads, int full_rescore);
+                       int n_threads, int full_rescore, double tie_rel);
 
 /* ================= heatmap sweep (reference GPU heatmap kernel) ================= */
 /* MonteCarloHeatmapKernel semantics (ParallelMonteCarloHeatmap.cu:217-363) with

@@ -314,29 +314,39 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
         if (active && !S.cool_per_block) T *= S.dt;
       }
     } else if constexpr (LEVEL == PMMC_LEVEL_SMOOTH && !STAGE) {
-      // Software-pipelined smooth moves, as the arc moves above: the window
-      // of move m+1 is fetched before move m is evaluated and patched after
-      // its decision.
-      U4 rbn = draw(seed, m0 >> 2, group, segment, LEVEL, kSubBead);
-      SmoothFetch f;
-      smooth_fetch(P, X, cst(P.free_list)[pick(pick_word(rbn, m0), static_cast<uint32_t>(nfree))], f);
-      for (int k = 0; k < S.block; ++k) {
+      // Software-pipelined smooth moves, as the arc moves above,
+      // kSmoothDepth moves deep: the windows of moves m+1..m+kSmoothDepth
+      // are in flight while move m is evaluated; after move m's decision
+      // every fetched window is patched.
+      constexpr int D = kSmoothDepth;
+      const int blk = S.block;
+      SmoothFetch f[D];
+      uint32_t mf = m0;  // the next move to fetch
+      U4 rbn = draw(seed, mf >> 2, group, segment, LEVEL, kSubBead);
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (d < blk) {
+          if (d > 0 && (mf & 3u) == 0) rbn = draw(seed, mf >> 2, group, segment, LEVEL, kSubBead);
+          smooth_fetch(P, X, cst(P.free_list)[pick(pick_word(rbn, mf), static_cast<uint32_t>(nfree))], f[d]);
+          ++mf;
+        }
+      }
+      for (int k = 0; k < blk; ++k) {
         const uint32_t m = m0 + static_cast<uint32_t>(k);
-        const SmoothFetch c = f;
-        const bool more = k + 1 < S.block;
-        if (more) {
-          const uint32_t mn = m + 1;
-          if ((mn & 3u) == 0) rbn = draw(seed, mn >> 2, group, segment, LEVEL, kSubBead);
-          smooth_fetch(P, X, cst(P.free_list)[pick(pick_word(rbn, mn), static_cast<uint32_t>(nfree))], f);
+        const SmoothFetch c = f[0];
+#pragma unroll
+        for (int d = 0; d + 1 < D; ++d) f[d] = f[d + 1];
+        if (k + D < blk) {
+          if ((mf & 3u) == 0) rbn = draw(seed, mf >> 2, group, segment, LEVEL, kSubBead);
+          smooth_fetch(P, X, cst(P.free_list)[pick(pick_word(rbn, mf), static_cast<uint32_t>(nfree))], f[D - 1]);
+          ++mf;
         }
         const float px = c.qx[2], py = c.qy[2], pz = c.qz[2];
         const U4 rc = draw(seed, m, chain, segment, LEVEL, kSubChain);
         const float dx = disp(rc.x, step), dy = disp(rc.y, step);
         const float dz = S.use2d ? 0.f : disp(rc.z, step);
         const float nx = move_coord(px, dx), ny = move_coord(py, dy), nz = move_coord(pz, dz);
-        const float eo = smooth_local(P, c.p, c.qx, c.qy, c.qz, px, py, pz, c.d0m, c.d0p);
-        const float en = smooth_local(P, c.p, c.qx, c.qy, c.qz, nx, ny, nz, c.d0m, c.d0p);
-        float dE = en - eo;
+        float dE = smooth_delta(P, c.p, c.qx, c.qy, c.qz, px, py, pz, nx, ny, nz, c.d0m, c.d0p);
         if (r > 0.f) dE += sphere_penalty(nx, ny, nz, r) - sphere_penalty(px, py, pz, r);
         const double En = E + static_cast<double>(dE);
         const bool ok = active && metropolis(S, E, En, T, u01(rc.w));
@@ -349,7 +359,9 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
           ++acc;
           best = E < best ? E : best;
         }
-        if (more) smooth_patch(f, c.p, ok, nx, ny, nz);
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+          if (k + 1 + d < blk) smooth_patch(f[d], c.p, ok, nx, ny, nz);
         if (active && !S.cool_per_block) T *= S.dt;
       }
     } else {
@@ -382,9 +394,7 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
         }
         const float d0m = p > 0 ? fmaxf(cst(P.dtn)[p - 1], 1e-6f) : 1.f;
         const float d0p = p < n - 1 ? fmaxf(cst(P.dtn)[p], 1e-6f) : 1.f;
-        const float eo = smooth_local(P, p, qx, qy, qz, px, py, pz, d0m, d0p);
-        const float en = smooth_local(P, p, qx, qy, qz, nx, ny, nz, d0m, d0p);
-        dE = en - eo;
+        dE = smooth_delta(P, p, qx, qy, qz, px, py, pz, nx, ny, nz, d0m, d0p);
       }
       if (r > 0.f) dE += sphere_penalty(nx, ny, nz, r) - sphere_penalty(px, py, pz, r);
       const double En = E + static_cast<double>(dE);
@@ -1071,8 +1081,7 @@ __global__ __launch_bounds__(64) void k_delta(const DevProblem *__restrict__ pro
     }
     const float d0m = p > 0 ? fmaxf(P.dtn[p - 1], 1e-6f) : 1.f;
     const float d0p = p < n - 1 ? fmaxf(P.dtn[p], 1e-6f) : 1.f;
-    dE = smooth_local(P, p, qx, qy, qz, nx, ny, nz, d0m, d0p) -
-         smooth_local(P, p, qx, qy, qz, px, py, pz, d0m, d0p);
+    dE = smooth_delta(P, p, qx, qy, qz, px, py, pz, nx, ny, nz, d0m, d0p);
   }
   const float r = P.sched.sphere_radius;
   if (r > 0.f) dE += sphere_penalty(nx, ny, nz, r) - sphere_penalty(px, py, pz, r);

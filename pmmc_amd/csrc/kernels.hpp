@@ -21,6 +21,16 @@ namespace pmmc {
 // arc CSR entries evaluated per round trip of the move's partner loop
 constexpr int kArcsChunk = PMMC_ARCS_CHUNK;
 
+#ifndef PMMC_SMOOTH_DEPTH
+#define PMMC_SMOOTH_DEPTH 1
+#endif
+// smooth-level moves whose windows are in flight ahead of the one evaluated.
+// Measured on the 19,951-bead smooth segment: 74.0 / 71.7 / 71.0 G moves/s
+// at depth 1 / 2 / 3 (each window costs 17 VGPRs, and depth 2 drops the
+// kernel from 5 to 4 waves per SIMD), so one move ahead is the default.
+constexpr int kSmoothDepth = PMMC_SMOOTH_DEPTH;
+static_assert(kSmoothDepth >= 1, "at least the next move is fetched ahead");
+
 // ------------------------------------------------------------------ //
 // helpers
 // ------------------------------------------------------------------ //
@@ -400,24 +410,75 @@ __device__ __forceinline__ float angle_f(float ux, float uy, float uz, float wx,
   return ang * ang * ang * ka;
 }
 
-// local window of bead p with centre position (cx,cy,cz); q* hold beads
-// p-2..p+2 (index 2 = p, ignored).  Unweighted (ParallelMonteCarloSmooth.cu:86-154).
-__device__ __forceinline__ float smooth_local(const DevProblem &P, int p, const float *qx,
-                                              const float *qy, const float *qz, float cx, float cy,
-                                              float cz, float d0m, float d0p) {
+// The move loops' form of the smooth-level ΔE.  Per evaluation only the two
+// bonds at the moved bead change length; the bonds (p-2,p-1) and (p+1,p+2)
+// are shared by both evaluations, so each length is computed once per move
+// (6 square roots instead of 16) and every bond length serves its bond term
+// and both adjacent angles.  Square roots and quotients use the hardware
+// v_sqrt_f32 / v_rcp_f32 (1 ulp each; the correctly rounded forms expand to
+// ~10-15 VALU instructions apiece and were half of this loop's VALU work),
+// and the angle's two quotient forms share one reciprocal, without a branch.
+// Local window of bead p, unweighted (ParallelMonteCarloSmooth.cu:86-154):
+// bonds (p-1,p), (p,p+1), then the angles at p-1, p, p+1.
+__device__ __forceinline__ float len3(float x, float y, float z) {
+  return __builtin_amdgcn_sqrtf(fmaf(x, x, fmaf(y, y, z * z)));
+}
+
+// bond term of length d with rest length d0 (rd0 = 1/d0)
+__device__ __forceinline__ float bond_r(float d, float d0, float rd0, float ks, float kq) {
+  const float diff = (d - d0) * rd0;
+  return diff * diff * (diff >= 0.f ? ks : kq);
+}
+
+// angle term of angle_f from bond vectors u, w with lengths lu, lw
+__device__ __forceinline__ float angle_l(float ux, float uy, float uz, float lu, float wx, float wy,
+                                         float wz, float lw, float ka) {
+  const float dot = fmaf(ux, wx, fmaf(uy, wy, uz * wz));
+  const float ll = lu * lw;
+  const float cx = uy * wz - uz * wy, cy = uz * wx - ux * wz, cz = ux * wy - uy * wx;
+  const float cr = fmaf(cx, cx, fmaf(cy, cy, cz * cz));
+  const bool acute = dot > 0.f;
+  // 1 - cos = |u x w|^2 / (ll (ll + u.w)) for u.w > 0, else (ll - u.w) / ll
+  const float num = acute ? cr : ll - dot;
+  const float den = acute ? ll * (ll + dot) : ll;
+  const float omc = fminf(fmaxf(num * __builtin_amdgcn_rcpf(den), 0.f), 2.f);
+  const float ang = 0.5f * omc;
+  // short bonds: no angle term (a zero length leaves a NaN or inf above,
+  // which the select drops)
+  return (lu > 1e-6f && lw > 1e-6f) ? ang * ang * ang * ka : 0.f;
+}
+
+// ΔE = local(new) - local(old) of bead p; q* hold beads p-2..p+2 (index 2
+// ignored; zero outside the chain), (px,py,pz) the old and (nx,ny,nz) the
+// new centre.
+__device__ __forceinline__ float smooth_delta(const DevProblem &P, int p, const float *qx,
+                                              const float *qy, const float *qz, float px, float py,
+                                              float pz, float nx, float ny, float nz, float d0m,
+                                              float d0p) {
   const int n = P.n;
-  float s = 0.f;
-  if (p > 0) s += bond_f(dist3(qx[1], qy[1], qz[1], cx, cy, cz), d0m, P.ks, P.kq);
-  if (p < n - 1) s += bond_f(dist3(cx, cy, cz, qx[3], qy[3], qz[3]), d0p, P.ks, P.kq);
-  if (p >= 2)  // vertex p-1
-    s += angle_f(qx[0] - qx[1], qy[0] - qy[1], qz[0] - qz[1], qx[1] - cx, qy[1] - cy, qz[1] - cz,
-                 P.ka);
-  if (p >= 1 && p + 1 <= n - 1)  // vertex p
-    s += angle_f(qx[1] - cx, qy[1] - cy, qz[1] - cz, cx - qx[3], cy - qy[3], cz - qz[3], P.ka);
-  if (p + 2 <= n - 1)  // vertex p+1
-    s += angle_f(cx - qx[3], cy - qy[3], cz - qz[3], qx[3] - qx[4], qy[3] - qy[4], qz[3] - qz[4],
-                 P.ka);
-  return s;
+  const bool bm = p > 0, bp = p < n - 1;
+  const bool am = p >= 2, ac = p >= 1 && p + 1 <= n - 1, ap = p + 2 <= n - 1;
+  const float ux = qx[0] - qx[1], uy = qy[0] - qy[1], uz = qz[0] - qz[1];
+  const float wx = qx[3] - qx[4], wy = qy[3] - qy[4], wz = qz[3] - qz[4];
+  const float l0 = am ? len3(ux, uy, uz) : 0.f;
+  const float l3 = ap ? len3(wx, wy, wz) : 0.f;
+  const float rm = __builtin_amdgcn_rcpf(d0m), rp = __builtin_amdgcn_rcpf(d0p);
+  float e[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const float cx = t ? nx : px, cy = t ? ny : py, cz = t ? nz : pz;
+    const float v1x = qx[1] - cx, v1y = qy[1] - cy, v1z = qz[1] - cz;  // bead p-1 -> p
+    const float v2x = cx - qx[3], v2y = cy - qy[3], v2z = cz - qz[3];  // bead p -> p+1
+    const float l1 = len3(v1x, v1y, v1z), l2 = len3(v2x, v2y, v2z);
+    float s = 0.f;
+    if (bm) s += bond_r(l1, d0m, rm, P.ks, P.kq);
+    if (bp) s += bond_r(l2, d0p, rp, P.ks, P.kq);
+    if (am) s += angle_l(ux, uy, uz, l0, v1x, v1y, v1z, l1, P.ka);
+    if (ac) s += angle_l(v1x, v1y, v1z, l1, v2x, v2y, v2z, l2, P.ka);
+    if (ap) s += angle_l(v2x, v2y, v2z, l2, wx, wy, wz, l3, P.ka);
+    e[t] = s;
+  }
+  return e[1] - e[0];
 }
 
 // Everything one smooth-level move reads before its decision, fetched ahead

@@ -301,6 +301,33 @@ def test_full_rescore_equals_incremental():
     assert sa["energy"] == pytest.approx(sb["energy"], rel=1e-9)
 
 
+@pytest.mark.parametrize("level", ["heatmap", "arcs", "smooth"])
+def test_near_tie_count(level):
+    """The near-tie counter (tie_rel): off at 0; it never changes the chain;
+    it grows with tie_rel; and at tie_rel = 1 every move counts, since |ΔE|
+    never exceeds |local before| + |local after| (a mis-scaled bound fails
+    this)."""
+    from pmmc_amd import synth
+    if level == "heatmap":
+        P = _small_heatmap()
+    elif level == "arcs":
+        P = O.Problem(synth.arcs_problem(n=40, n_arcs=120, seed=3))
+    else:
+        P = O.Problem(synth.smooth_problem(n_anchors=8, add=4, seed=3))
+    s = _sched(max_moves=1500, stop_rule=1, fails_threshold=10 ** 6, step=0.3)
+    x0, s0 = P.anneal(s, seed=5, segment=0, chain=2, trace=True)
+    assert s0["near_ties"] == 0 and s0["first_near_tie"] == -1
+    counts = []
+    for rel in (1e-7, 1e-5, 1e-3, 1.0):
+        x, st = P.anneal(s, seed=5, segment=0, chain=2, trace=True, tie_rel=rel)
+        assert np.array_equal(x, x0) and np.array_equal(st["trace"], s0["trace"])
+        assert st["energy"] == s0["energy"] and st["moves"] == s0["moves"]
+        counts.append(st["near_ties"])
+        assert (st["near_ties"] == 0) == (st["first_near_tie"] == -1)
+    assert counts == sorted(counts)
+    assert counts[-1] == s0["moves"]
+
+
 # ---------------- arc marking ----------------
 def test_mark_arcs_closed_form():
     anc_s = [100, 300, 150, 0, 500, 700]
