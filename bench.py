@@ -215,7 +215,43 @@ def smooth_leg(ctx, with_cpu=True, cpu_seconds=8.0):
     return out
 
 
-LEGS = {"heatmap_64": heatmap_leg, "smooth_19951": smooth_leg}
+def heatmap_sweep_leg(ctx, with_cpu=True, cpu_seconds=8.0):
+    """The reference GPU heatmap kernel's own algorithm (pmmc_heatmap_sweep)
+    on the 64-bead heatmap, launched as default.ini sizes that kernel:
+    blocks_multiplier 16 x 64 beads x 512 threads = 8,192 proposers per bead,
+    512 proposals each per iteration, step 0.75 x step_size, 3 milestone
+    fails ([cuda], [simulation_heatmap]).  One solve after a one-iteration
+    warm-up."""
+    from pmmc_amd import _abi as A
+    from pmmc_amd import metrics, synth
+    del with_cpu, cpu_seconds  # no CPU counterpart: the reference runs it on CUDA only
+    prob, step, truth = synth.config_case(1, 0)
+    cs = synth.CONFIG_SETTINGS[1]
+    # blocks_multiplier x n blocks of num_threads: 16 x 512 proposers per bead
+    proposers, inner = 16 * 512, 512
+    ss = A.SweepSchedule(cs["max_temp"], cs["dt_temp"], cs["improvement"], 1e-4, cs["tjs"],
+                         cs["tjc"], 0.75 * step, 0.95, 0.0, inner, proposers, 3, 1000, 0)
+    warm = A.SweepSchedule(cs["max_temp"], cs["dt_temp"], cs["improvement"], 1e-4, cs["tjs"],
+                           cs["tjc"], 0.75 * step, 0.95, 0.0, inner, proposers, 3, 1, 0)
+    ctx.heatmap_sweep(prob, warm, seed=1, segment=0)
+    t0 = time.perf_counter()
+    pos, st = ctx.heatmap_sweep(prob, ss, seed=2, segment=1)
+    wall = time.perf_counter() - t0
+    proposals = st["moves"]  # iterations x free beads x proposers x inner
+    kern = st["kernel_ms"] * 1e-3
+    out = {"workload": "reference GPU heatmap algorithm (parallel proposers, race-free sweep), "
+                       "64 beads, 8,192 proposers x 512 proposals per bead per iteration",
+           "iterations": st["milestones"], "proposals": proposals, "solve_s": wall,
+           "proposals_per_s": proposals / wall, "kernel_proposals_per_s": proposals / kern,
+           "partner_evaluations_per_s": proposals * (len(prob.pos) - 1) / kern,
+           "final_energy": st["energy"], "stop_reason": st["stop_reason"],
+           "bound": "VALU (snapshot and weight column in LDS)"}
+    out.update(metrics.compare(pos, truth))
+    return out
+
+
+LEGS = {"heatmap_64": heatmap_leg, "heatmap_64_sweep": heatmap_sweep_leg,
+        "smooth_19951": smooth_leg}
 
 
 def levels(ctx, with_cpu=True, cpu_seconds=8.0):
