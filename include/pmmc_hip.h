@@ -328,7 +328,9 @@ pmmc_status pmmc_server_submit(pmmc_server *srv, const pmmc_job *job, uint64_t *
 /* Admit queued jobs that fit the budget, run one launch per launch group over
  * every resident chain, retire finished jobs (results written to the job's
  * pos_out / stats).  *n_done = jobs finished in this step; *n_resident and
- * *n_queued describe the pool afterwards (nullable). */
+ * *n_queued describe the pool afterwards (nullable).  A job whose upload
+ * fails at admission is dropped (it never reports done) and the step returns
+ * that error; the server stays usable. */
 pmmc_status pmmc_server_step(pmmc_server *srv, int32_t *n_done, int32_t *n_resident,
                              int32_t *n_queued);
 /* 1 when job_id has finished (its outputs are written). */

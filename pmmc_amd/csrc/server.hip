@@ -198,7 +198,12 @@ static pmmc_status admit(pmmc_server *s, SrvJob *j, bool *placed) {
     err = hipMemcpyAsync(j->dprob, &j->hprob, sizeof(DevProblem), hipMemcpyHostToDevice, ctx->stream);
   if (err == hipSuccess)
     err = hipMemcpyAsync(j->dens, j->hens.data(), sizeof(DevEns) * ne, hipMemcpyHostToDevice, ctx->stream);
-  if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("server admit: ") + hipGetErrorString(err));
+  if (err != hipSuccess) {
+    s->free_list.release(j->off, j->bytes);
+    j->mem = nullptr;
+    *placed = false;
+    return fail(ctx, PMMC_ERR_HIP, std::string("server admit: ") + hipGetErrorString(err));
+  }
   return PMMC_OK;
 }
 
@@ -454,16 +459,19 @@ extern "C" pmmc_status pmmc_server_step(pmmc_server *s, int32_t *n_done, int32_t
   pmmc_ctx *ctx = s->ctx;
   int finished = 0;
   std::vector<SrvJob *> fresh;
-  // admission: FIFO, while slots, budget and a contiguous pool block allow
+  // admission: FIFO, while slots, budget and a contiguous pool block allow.
+  // A job whose upload fails is dropped (its block returned); the jobs
+  // admitted before it are still initialised, then the step reports the error.
+  pmmc_status admit_err = PMMC_OK;
   while (!s->queue.empty() && static_cast<int>(s->resident.size()) < s->opts.max_jobs &&
          s->resident_bytes + s->queue.front()->bytes <= s->opts.hbm_budget) {
     SrvJob *j = s->queue.front();
     bool placed = false;
-    pmmc_status ps = admit(s, j, &placed);
-    if (ps != PMMC_OK) {
+    admit_err = admit(s, j, &placed);
+    if (admit_err != PMMC_OK) {
       s->queue.pop_front();
       delete j;
-      return ps;
+      break;
     }
     if (!placed) break;
     s->queue.pop_front();
@@ -476,6 +484,7 @@ extern "C" pmmc_status pmmc_server_step(pmmc_server *s, int32_t *n_done, int32_t
     pmmc_status ps = init_jobs(s, fresh);
     if (ps != PMMC_OK) return ps;
   }
+  if (admit_err != PMMC_OK) return admit_err;
   if (!s->resident.empty()) {
     if (s->dirty) {
       pmmc_status ps = rebuild(s);
