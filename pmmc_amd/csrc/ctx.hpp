@@ -48,7 +48,7 @@ inline pmmc_status arena_reserve(pmmc_ctx *c, size_t bytes) {
   if (c->arena) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail(c, PMMC_ERR_HIP, hipGetErrorString(e));
-    hipFree(c->arena);
+    (void)hipFree(c->arena);
     c->arena = nullptr;
     c->arena_cap = 0;
   }

@@ -4,12 +4,20 @@
 // k_anneal<PMMC_LEVEL_HEATMAP> (engine.hip); only where the positions live
 // differs.  Each lane keeps its chain's whole structure in VGPRs (NB = 64
 // beads per axis), so the O(N) partner loop of a move reads no memory: it
-// is fully unrolled over constant register indices.  The moved bead p is
-// wave-uniform (shared pick stream); its position is read and written
-// through a uniform switch.  The weight table (16 KB for 64 beads) is copied to LDS once per
-// launch; a move reads row p with uniform-address ds_read_b128.  Positions go
-// back to the SoA array in HBM only for the milestone re-sum and at the end
-// of the launch.
+// is fully unrolled over constant register indices.  The weight table (16 KB
+// for 64 beads) is copied to LDS once per launch; a move reads row p with
+// uniform-address ds_read_b128.
+//
+// The moved bead p is wave-uniform (shared pick stream).  Indexing the
+// register arrays with it -- a 64-way switch, or vector halves with indexed
+// register moves -- made the compiler spill hundreds of registers, and so did
+// the O(N^2) milestone re-sum inlined into the kernel.  So: the SoA copy of
+// the positions in memory is kept current (an accepted move stores its bead
+// there too), the moved bead's old position is read from it one move ahead
+// (the pick stream does not depend on the chains' state; the prefetched copy
+// is patched when the current move changes that bead), the register copy is
+// updated by one select per register, and the re-sum is an out-of-line call
+// that reads the memory copy.  Symmetric tables: no spills in the move loop.
 #include <hip/hip_runtime.h>
 
 #include "ctx.hpp"
@@ -17,53 +25,6 @@
 #include "kernels.hpp"
 
 namespace pmmc {
-
-// Read / write bead p of the register arrays.  p is wave-uniform, so the
-// switch is a scalar branch tree and every case touches fixed registers
-// (dynamic indexing would force the arrays out of registers).
-#define PMMC_GET(J)  \
-  case (J):          \
-    px = X[J];       \
-    py = Y[J];       \
-    pz = Z[J];       \
-    break;
-#define PMMC_SET(J)          \
-  case (J):                  \
-    X[J] = ok ? nx : X[J];   \
-    Y[J] = ok ? ny : Y[J];   \
-    Z[J] = ok ? nz : Z[J];   \
-    break;
-#define PMMC_X4(M, B) M((B)) M((B) + 1) M((B) + 2) M((B) + 3)
-#define PMMC_X16(M, B) PMMC_X4(M, B) PMMC_X4(M, (B) + 4) PMMC_X4(M, (B) + 8) PMMC_X4(M, (B) + 12)
-#define PMMC_X32(M, B) PMMC_X16(M, B) PMMC_X16(M, (B) + 16)
-
-template <int NB>
-__device__ __forceinline__ void get_bead(const float (&X)[NB], const float (&Y)[NB],
-                                         const float (&Z)[NB], int p, float &px, float &py,
-                                         float &pz) {
-  static_assert(NB == 64, "64-bead form");
-  switch (p) { PMMC_X32(PMMC_GET, 0) PMMC_X32(PMMC_GET, 32) default: break; }
-}
-
-template <int NB>
-__device__ __forceinline__ void set_bead(float (&X)[NB], float (&Y)[NB], float (&Z)[NB], int p,
-                                         bool ok, float nx, float ny, float nz) {
-  static_assert(NB == 64, "64-bead form");
-  switch (p) { PMMC_X32(PMMC_SET, 0) PMMC_X32(PMMC_SET, 32) default: break; }
-}
-
-template <int NB>
-__device__ __forceinline__ void store_all(float *g, int n, const float (&X)[NB], const float (&Y)[NB],
-                                          const float (&Z)[NB]) {
-#pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    if (j < n) {
-      g[(3 * j + 0) * kLanes] = X[j];
-      g[(3 * j + 1) * kLanes] = Y[j];
-      g[(3 * j + 2) * kLanes] = Z[j];
-    }
-  }
-}
 
 // ΔE of moving p from (px,py,pz) to (nx,ny,nz): exact change of the
 // double-counted full score, as heatmap_delta (kernels.hpp), over registers.
@@ -112,6 +73,13 @@ __device__ __forceinline__ float hreg_delta(const float *lw, const float *lw2, c
     }
     return d + acc2;
   }
+}
+
+// Milestone re-sum over the memory copy of the positions, out of line so that
+// its loops do not share a register allocation with the move loop.
+__device__ __noinline__ double hreg_resum(const DevProblem &P, float *pos, int lane, float r) {
+  const PosView V = global_view(pos, lane);
+  return resum_heatmap(P, V, r);
 }
 
 // Occupancy target: 2 waves per SIMD (192 position VGPRs of 256).
@@ -167,15 +135,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     const uint64_t am = __ballot(active);
     if (am == 0) break;
     const uint32_t m0 = readlane_u32(iters, __builtin_ctzll(am));
-    U4 rb = {0, 0, 0, 0};
+    // The moved bead's old position comes from the SoA copy in memory (kept
+    // current on every accepted move), read one move ahead: the pick stream
+    // does not depend on the chains' state.
+    U4 rbn = draw(seed, m0 >> 2, group, segment, PMMC_LEVEL_HEATMAP, kSubBead);
+    int pn = cst(P.free_list)[pick(pick_word(rbn, m0), static_cast<uint32_t>(nfree))];
+    float qx = g[(3 * pn + 0) * kLanes], qy = g[(3 * pn + 1) * kLanes], qz = g[(3 * pn + 2) * kLanes];
     for (int k = 0; k < S.block; ++k) {
       const uint32_t m = m0 + static_cast<uint32_t>(k);
-      if (k == 0 || (m & 3u) == 0) rb = draw(seed, m >> 2, group, segment, PMMC_LEVEL_HEATMAP, kSubBead);
-      const uint32_t sel = m & 3u;
-      const uint32_t word = sel == 0 ? rb.x : (sel == 1 ? rb.y : (sel == 2 ? rb.z : rb.w));
-      const int p = cst(P.free_list)[pick(word, static_cast<uint32_t>(nfree))];
-      float px = 0.f, py = 0.f, pz = 0.f;
-      get_bead<NB>(X, Y, Z, p, px, py, pz);
+      const int p = pn;
+      const float px = qx, py = qy, pz = qz;
+      const bool more = k + 1 < S.block;
+      if (more) {
+        const uint32_t mn = m + 1;
+        if ((mn & 3u) == 0) rbn = draw(seed, mn >> 2, group, segment, PMMC_LEVEL_HEATMAP, kSubBead);
+        pn = cst(P.free_list)[pick(pick_word(rbn, mn), static_cast<uint32_t>(nfree))];
+        qx = g[(3 * pn + 0) * kLanes];
+        qy = g[(3 * pn + 1) * kLanes];
+        qz = g[(3 * pn + 2) * kLanes];
+      }
       const U4 rc = draw(seed, m, chain, segment, PMMC_LEVEL_HEATMAP, kSubChain);
       const float dx = disp(rc.x, step), dy = disp(rc.y, step);
       const float dz = S.use2d ? 0.f : disp(rc.z, step);
@@ -184,7 +162,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       if (r > 0.f) dE += sphere_penalty(nx, ny, nz, r) - sphere_penalty(px, py, pz, r);
       const double En = E + static_cast<double>(dE);
       const bool ok = active && metropolis(S, E, En, T, u01(rc.w));
-      set_bead<NB>(X, Y, Z, p, ok, nx, ny, nz);
+      if (ok) {
+        g[(3 * p + 0) * kLanes] = nx;
+        g[(3 * p + 1) * kLanes] = ny;
+        g[(3 * p + 2) * kLanes] = nz;
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const bool hit = ok && j == p;
+        X[j] = hit ? nx : X[j];
+        Y[j] = hit ? ny : Y[j];
+        Z[j] = hit ? nz : Z[j];
+      }
+      // the next move's prefetched copy was read before this move's store
+      const bool same = ok && pn == p;
+      qx = same ? nx : qx;
+      qy = same ? ny : qy;
+      qz = same ? nz : qz;
       if (ok) {
         E = En;
         ++msucc;
@@ -204,9 +198,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       double Er = E;
       const bool resum_now = S.resum > 0 && ((miles + 1) % S.resum) == 0;
       if (__ballot(due && resum_now) != 0) {
-        store_all<NB>(g, n, X, Y, Z);
-        const PosView V = global_view(it.pos, lane);
-        Er = resum_heatmap(P, V, r);
+        Er = hreg_resum(P, it.pos, lane, r);
       }
       if (due) {
         ++miles;
@@ -215,7 +207,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       }
     }
   }
-  store_all<NB>(g, n, X, Y, Z);
+  // the memory copy of the positions is already current
   if (status == -1) return;  // padding lane of a partial ensemble
   st->E[lane] = E;
   st->T[lane] = T;
@@ -232,7 +224,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 
 // heatmaps of 33..64 beads (weight rows padded to 64).  Up to 32 beads the
 // partner loop over L2-resident positions is faster (N=24: 16.3 vs 10.3
-// G moves/s; N=64: 6.2 vs 9.2).
+// G moves/s).
 int hreg_halves(const HostProblem &h) {
   if (h.level != PMMC_LEVEL_HEATMAP || h.n <= 32 || h.n > 64) return 0;
   return h.ldw == 64 ? 2 : 0;
