@@ -174,7 +174,8 @@ def heatmap_leg(ctx, with_cpu=True, cpu_seconds=8.0):
          "kernel_moves_per_s": st["moves"] / (st["kernel_ms"] * 1e-3), "solve_s": wall,
          "final_energy": st["energy_resummed"],
          "partner_evaluations_per_s": 2 * 63 * st["moves"] / (st["kernel_ms"] * 1e-3),
-         "bound": "VALU (positions of 33-64 beads held in registers: no memory traffic per move)"}
+         "bound": "issue latency (positions of 33-64 beads held in 192 VGPRs: 2 waves per SIMD; "
+                  "12 B read + 12 B x acceptance written per chain-move)"}
     h.update(metrics.compare(pos, truth))
     if with_cpu:
         h["cpu_reference"] = cpu_reference(1, prob, step, synth.CONFIG_SETTINGS[1], cpu_seconds,
