@@ -40,6 +40,7 @@ sys.path.insert(0, ROOT)
 METRIC = "MC moves/s + batched solves/s vs plain GPU; p50/p99 latency at matched energy"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_anneal_arcs_n400_hbm_traffic.txt")
+PROBE_FILE = os.path.join(ROOT, "profiles", "probe_move_patterns.txt")
 
 
 def genomic_step(anchors, noise=0.01, power=0.75, scale=0.5, base=1.0):
@@ -64,6 +65,18 @@ def workload():
     anchors, arcs, _ = synth.arcs_inputs(n=400, n_arcs=1200, seed=0)
     prob = synth.arcs_problem(n=400, n_arcs=1200, seed=0)
     return prob, arcs_settings(), genomic_step(anchors), arcs
+
+
+def pattern_ceiling(key="arc_pattern_ceiling_moves_per_s"):
+    """Moves/s the move kernel's own access pattern sustains with no
+    arithmetic, from the committed bench.py --probe summary, or None."""
+    try:
+        for line in open(PROBE_FILE):
+            if line.startswith(key + ":"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def measured_traffic_per_move():
@@ -208,6 +221,8 @@ def smooth_leg(ctx, with_cpu=True, cpu_seconds=8.0):
         "kernel_moves_per_s": st["moves"] / kms, "solve_s": wall,
         "final_energy": st["energy_resummed"],
         "bytes_per_move": bpm, "achieved_GBps": bpm * st["moves"] / kms / 1e9,
+        "frac_of_pattern_ceiling": (None if pattern_ceiling("smooth_pattern_ceiling_moves_per_s") is None
+                                    else st["moves"] / kms / pattern_ceiling("smooth_pattern_ceiling_moves_per_s")),
         "bound": "VALU and HBM together (~500 VALU instructions per 64-chain move; "
                  "3.8 KB window read per 64-chain move)"}
     if with_cpu:
@@ -407,6 +422,39 @@ def served(ctx, seconds=3.0, rate=200.0, seed=0, plain_sample=24):
                    "ensemble 1-16 solves of 64 chains per request"}
 
 
+def probe() -> int:
+    """Sustained bandwidth of the move kernels' own access patterns
+    (pmmc_probe_move_pattern), the ceiling their HBM figures are read
+    against besides the nominal 8 TB/s: the arc move (7 random beads of
+    768 B read, one written back by no lane, about half of the lanes, every
+    lane) over a 1.25 GiB buffer like the 400-anchor workload's positions, at
+    16 and 64 wavefronts per CU; the smooth move (one random 5-bead window,
+    3,840 B; one lane writes one bead back in every other move, like a 1.4%
+    acceptance rate) over 8 GiB, at 20 per CU; a streaming
+    read; LDS read bandwidth."""
+    from pmmc_amd import Context
+    out = {"probe": "move patterns", "runs": []}
+    with Context(0) as ctx:
+        cus = ctx.cu_count
+        for waves in (16 * cus, 64 * cus):
+            for mode in (0, 1, 2):
+                r = ctx.probe_move_pattern(5 << 28, waves, 1024, 7, 1, mode)
+                out["runs"].append({"pattern": "arc move: 7 beads read", "write": ["none", "half lanes", "all lanes"][mode],
+                                    "waves": waves, **r})
+        # footprints that fit the 256 MB Infinity Cache
+        for mib in (64, 256):
+            r = ctx.probe_move_pattern(mib << 20, 16 * cus, 1024, 7, 1, 1)
+            out["runs"].append({"pattern": f"arc move: 7 beads read, {mib} MiB buffer",
+                                "write": "half lanes", "waves": 16 * cus, **r})
+        for mode in (0, 3):
+            r = ctx.probe_move_pattern(8 << 30, 20 * cus, 1024, 1, 5, mode)
+            out["runs"].append({"pattern": "smooth move: 5-bead window read",
+                                "write": {0: "none", 3: "one lane, every other move"}[mode],
+                                "waves": 20 * cus, **r})
+    print(json.dumps(out))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -417,6 +465,8 @@ def main(argv=None):
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-levels", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="only measure the move kernels' access-pattern bandwidth ceilings")
     ap.add_argument("--only", choices=sorted(LEGS),
                     help="run one level workload alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -425,6 +475,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.copy_calibration:
         return copy_calibration()
+    if a.probe:
+        return probe()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": a.gpus}))
@@ -494,6 +546,13 @@ def main(argv=None):
                                    "anchors; traffic = measured HBM bytes per launch (rocprofv3 "
                                    "FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-kernel calibrated)"},
     }
+    ceil = pattern_ceiling()
+    if ceil:
+        # the same random bead reads and write-backs with no arithmetic
+        # (bench.py --probe, profiles/probe_move_patterns.txt)
+        line["pattern_ceiling"] = {"moves_per_s": ceil, "frac": moves / (kern_ms * 1e-3) / ceil,
+                                   "pattern": "7 random 768-byte bead reads + one bead written "
+                                              "back by half of the lanes, per move"}
     line.update(extra)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(prob, s, step)

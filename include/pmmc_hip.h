@@ -27,6 +27,7 @@
  * proposals):  pmmc_delta_energy.
  * Batch / serve forms (no reference counterpart; the batched and served
  * annealing forms):  pmmc_anneal_batch, pmmc_server_*.
+ * Diagnostics (no reference counterpart): pmmc_probe_move_pattern.
  *
  * Errors: every entry point returns pmmc_status; nothing calls exit().  The
  * message of the last failure is available from pmmc_last_error(ctx).
@@ -251,6 +252,32 @@ typedef struct {
 pmmc_status pmmc_heatmap_sweep(pmmc_ctx *ctx, const pmmc_heatmap_problem *p,
                                const pmmc_sweep_schedule *s, uint64_t seed, uint32_t segment,
                                float *pos_inout, pmmc_anneal_stats *stats);
+
+/* ---------------- bandwidth probe (diagnostics) ---------------- */
+/* No reference counterpart: measures what the GPU sustains for the move
+ * kernels' own access patterns, the ceiling their throughput is read against
+ * besides the nominal HBM peak.  Per move of a 64-lane wavefront: `reads`
+ * random windows of `chunk_beads` consecutive beads (768 B per bead: three
+ * 256-byte SoA rows; chunk_beads 1 with reads <= 8 -- the arc move -- or 5
+ * with reads 1 -- the smooth move), then the first window's first bead is
+ * written back by no lane (write_mode 0), about half of the lanes (1), every
+ * lane (2) or one lane in every other move (3); two moves in flight per
+ * wavefront; `waves` wavefronts of
+ * `moves_per_wave` (even) moves over a buffer of `bytes` of context-owned
+ * device memory.  Also a streaming read of that buffer and a conflict-free
+ * ds_read_b128 loop over 64 KB of LDS per workgroup. */
+typedef struct {
+  double moves_per_s;       /* lane-moves per second of the pattern        */
+  double read_gbps;         /* bytes of the pattern read per second / 1e9  */
+  double write_gbps;        /* bytes of the pattern written / s / 1e9      */
+  double stream_read_gbps;  /* streaming read of the buffer                */
+  double lds_gbps;          /* aggregate LDS read bandwidth                */
+  double ms;                /* duration of the pattern launch              */
+} pmmc_probe_result;
+
+pmmc_status pmmc_probe_move_pattern(pmmc_ctx *ctx, size_t bytes, int32_t waves,
+                                    int32_t moves_per_wave, int32_t reads, int32_t chunk_beads,
+                                    int32_t write_mode, pmmc_probe_result *out);
 
 /* ---------------- per-move ΔE test hook ---------------- */
 /* Evaluate ΔE (FP32, the kernels' arithmetic) for n_prop injected proposals

@@ -73,6 +73,14 @@ class Run(C.Structure):
                 ("n_chains", C.c_int32)]
 
 
+class ProbeResult(C.Structure):
+    _fields_ = [("moves_per_s", C.c_double), ("read_gbps", C.c_double), ("write_gbps", C.c_double),
+                ("stream_read_gbps", C.c_double), ("lds_gbps", C.c_double), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Stats(C.Structure):
     _fields_ = [("energy", C.c_double), ("energy_resummed", C.c_double),
                 ("best_energy", C.c_double), ("moves", C.c_int64), ("accepted", C.c_int64),
@@ -194,6 +202,8 @@ SIGNATURES = {
     "pmmc_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_uint64)]),
     "pmmc_server_step": (C.c_int, [C.c_void_p, i32p, i32p, i32p]),
     "pmmc_server_done": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "pmmc_probe_move_pattern": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int32, C.c_int32, C.c_int32,
+                                          C.c_int32, C.c_int32, C.POINTER(ProbeResult)]),
     "pmmc_spline_count": (C.c_int64, [C.c_int64, C.c_int64, C.c_int32]),
     "pmmc_spline_positions": (C.c_int, [C.c_void_p, C.c_int32, i32p, f32p, C.c_int32, C.c_int32,
                                         C.c_int64, C.c_int32, C.c_int64, f32p]),

@@ -170,6 +170,18 @@ class Context:
             out.ctypes.data_as(A.f32p)))
         return out
 
+    def probe_move_pattern(self, bytes_: int, waves: int, moves_per_wave: int, reads: int,
+                           chunk_beads: int, write_mode: int) -> dict:
+        """pmmc_probe_move_pattern: sustained bandwidth of the move kernels'
+        access pattern (random 768-byte bead reads, one bead written back),
+        plus a streaming read and the LDS read bandwidth."""
+        r = A.ProbeResult()
+        self._check(self.lib.pmmc_probe_move_pattern(self.h, int(bytes_), int(waves),
+                                                      int(moves_per_wave), int(reads),
+                                                      int(chunk_beads), int(write_mode),
+                                                      C.byref(r)))
+        return r.as_dict()
+
     @property
     def cu_count(self) -> int:
         return self.lib.pmmc_ctx_cu_count(self.h)

@@ -41,7 +41,7 @@ STRUCTS = {
     "pmmc_level_params": A.LevelParams, "pmmc_chromosome": A.Chromosome,
     "pmmc_segment_region": A.SegmentRegion, "pmmc_cluster": A.Cluster,
     "pmmc_level_report": A.LevelReport, "pmmc_arcs_level_opts": A.ArcsLevelOpts,
-    "pmmc_heatmap_level_opts": A.HeatmapLevelOpts,
+    "pmmc_heatmap_level_opts": A.HeatmapLevelOpts, "pmmc_probe_result": A.ProbeResult,
 }
 
 
