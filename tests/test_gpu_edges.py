@@ -156,3 +156,17 @@ def test_degenerate_problems(ctx):
     with pytest.raises(PmmcError):
         ctx.anneal(Heatmap(pos=np.zeros((0, 3), np.float32), dist=np.zeros((0, 0), np.float32)),
                    _sched(1, 1, 1.0), seed=1, n_chains=64)
+
+
+def test_probe_move_pattern(ctx):
+    # the bandwidth probe: argument checks, and a small run returns sane rates
+    r = ctx.probe_move_pattern(64 << 20, 256, 64, 7, 1, 1)
+    assert r["moves_per_s"] > 0 and r["read_gbps"] > 0 and r["write_gbps"] > 0
+    assert 0 < r["read_gbps"] < 20000 and 0 < r["stream_read_gbps"] < 20000 and r["lds_gbps"] > 0
+    r = ctx.probe_move_pattern(64 << 20, 256, 64, 1, 5, 0)
+    assert r["write_gbps"] == 0 and r["read_gbps"] > 0
+    for bad in [(64 << 20, 0, 64, 7, 1, 1), (64 << 20, 256, 63, 7, 1, 1), (64 << 20, 256, 64, 9, 1, 1),
+                (64 << 20, 256, 64, 2, 5, 0), (64 << 20, 256, 64, 7, 3, 0), (64 << 20, 256, 64, 7, 1, 4),
+                (1000, 256, 64, 7, 1, 1)]:
+        with pytest.raises(PmmcError):
+            ctx.probe_move_pattern(*bad)
