@@ -20,6 +20,8 @@
  *                           ParallelMarkArcs.cu:150-243
  *   pmmc_set_gpu_seed    <- void setGpuSeed(unsigned) platform.h:106,
  *                           ParallelMonteCarloHeatmap.cu:37
+ *   pmmc_singleton_heatmap <- LooperSolver::createSingletonHeatmap
+ *                           LooperSolver.hpp:2517-2749 (records parsed)
  *   pmmc_ctx_create/destroy/reserve <- GpuResourceCache::init/cleanup and its
  *                           grow-only buffers, LooperSolver.hpp:54-106,
  *                           ParallelMonteCarloArcs.cu:84-213
@@ -252,6 +254,58 @@ typedef struct {
 pmmc_status pmmc_heatmap_sweep(pmmc_ctx *ctx, const pmmc_heatmap_problem *p,
                                const pmmc_sweep_schedule *s, uint64_t seed, uint32_t segment,
                                float *pos_inout, pmmc_anneal_stats *stats);
+
+/* ---------------- singleton heatmap (heatmap input construction) ---------------- */
+/* <- LooperSolver::createSingletonHeatmap (LooperSolver.hpp:2517-2749): the
+ * contact heatmap between the clusters of the current level, from singleton
+ * PET records.  The reference reads the records from text files
+ * (:2590-2626); here they come parsed, each chromosome given as an index into
+ * the level's chromosome list (< 0 or >= n_chr: a chromosome that is not
+ * processed, counted in bad_chr).  Bins per chromosome: 0, the midpoints of
+ * consecutive clusters ((end_i + start_i+1) / 2, integer), 1e9; a position
+ * falls in the first bin whose closed range holds it (:2652-2663); records
+ * whose start1 or end2 lies outside [region_start, region_end] are skipped
+ * when region_end > 0 (:2633-2637); negative start1 / start2 count as 0;
+ * records with both ends in one bin are counted in `diagonal` and skipped.
+ * Scores are summed as 64-bit integers on the device (the reference sums them
+ * in float: the same value while a cell stays below 2^24).  Then, as the
+ * reference: each long-range arc adds (float)(long_scale * score^long_power)
+ * twice to cell [bin of start][bin of end] (:2701-2737, as written); the
+ * first and last break of each chromosome become its lowest start and
+ * highest end among the added records (:2740-2746); bin lengths exclude the
+ * chromosome's centromere (getGenomicLengthExcludingCentromere, :5916-5935)
+ * and are in Mb; the upper triangle divided by len_i * len_j is mirrored
+ * (:2773-2779); the band |i - j| < diag is cleared (Heatmap::clearDiagonal).
+ * A record or long arc outside every bin is an error here (the reference
+ * exits, :2665-2668, or indexes bin -1). */
+typedef struct {
+  int32_t n_chr;
+  const int32_t *cluster_offsets;  /* [n_chr + 1]: clusters of chromosome c   */
+  const int32_t *cluster_start;    /* genomic start / end of the level's       */
+  const int32_t *cluster_end;      /* clusters, chromosome by chromosome       */
+  int64_t n_records;
+  const int32_t *chr1, *start1, *end1;  /* first end of each record           */
+  const int32_t *chr2, *start2, *end2;  /* second end                          */
+  const int32_t *score;
+  int32_t region_start, region_end;
+  const int32_t *centromere;   /* [3 * n_chr]: has, start, end per chromosome
+                                  (the first region of the chromosome), or
+                                  NULL: no centromere data                   */
+  int32_t n_long;              /* long-range arcs, each chromosome's in the  */
+  const int32_t *long_chr;     /* reference's order                          */
+  const int32_t *long_start, *long_end, *long_score;
+  float long_scale, long_power;  /* Settings::longPETClustersEffectScale / Power */
+  int32_t diag;
+} pmmc_singleton_input;
+
+typedef struct {
+  int64_t added, diagonal, bad_chr;  /* records added, on one bin, on other chromosomes */
+  double kernel_ms;                  /* device time of the binning launches          */
+} pmmc_singleton_counts;
+
+/* heat: N x N row-major, N = cluster_offsets[n_chr]; counts nullable. */
+pmmc_status pmmc_singleton_heatmap(pmmc_ctx *ctx, const pmmc_singleton_input *in, float *heat,
+                                   pmmc_singleton_counts *counts);
 
 /* ---------------- bandwidth probe (diagnostics) ---------------- */
 /* No reference counterpart: measures what the GPU sustains for the move

@@ -26,6 +26,12 @@
  *   pmmc_freq_to_distance   LooperSolver::freqToDistance :3949-3967
  *   pmmc_distance_heatmap   LooperSolver::createDistanceHeatmap :3042-3085,
  *                           freqToDistanceHeatmap(Inter) :3934-3942
+ *   pmmc_normalize_heatmap  LooperSolver::normalizeHeatmap :2996-3040
+ *   pmmc_normalize_heatmap_diagonal_total
+ *                           LooperSolver::normalizeHeatmapDiagonalTotal
+ *                           :3146-3172
+ *   pmmc_normalize_heatmap_inter
+ *                           LooperSolver::normalizeHeatmapInter :3106-3144
  *
  * Errors: the message of the last failure on the calling thread is returned
  * by pmmc_host_last_error() (the reference prints and exits instead).
@@ -95,6 +101,24 @@ double pmmc_genomic_length_to_distance(int32_t length, const pmmc_level_params *
 pmmc_status pmmc_distance_heatmap(const float *freq, int32_t n, int32_t diagonal_size,
                                   int32_t inter, const pmmc_level_params *p, float *out,
                                   float *avg_out);
+
+/* Frequency-heatmap normalisations the level drivers apply between
+ * createSingletonHeatmap (pmmc_singleton_heatmap, pmmc_hip.h) and
+ * createDistanceHeatmap (LooperSolver.hpp:1095-1121, :1203-1215).  Float
+ * arithmetic and summation order of the reference. */
+/* Rows scaled to the average row sum (a row summing to <= 1e-12 becomes 0),
+ * then each pair set to the mean of its two triangles.  in, out: n*n
+ * row-major (may alias). */
+pmmc_status pmmc_normalize_heatmap(const float *in, int32_t n, float *out);
+/* Scaled in place so that the first non-zero diagonal (Heatmap::
+ * getDiagonalSize) averages val (FP64 average); unchanged when that average
+ * is not above 1e-12. */
+pmmc_status pmmc_normalize_heatmap_diagonal_total(float *heat, int32_t n, float val);
+/* Every cell times scale, then the intra-chromosome blocks (chromosome c has
+ * sizes[c] consecutive bins) divided by it again; unchanged for one
+ * chromosome. */
+pmmc_status pmmc_normalize_heatmap_inter(float *heat, int32_t n, int32_t n_chr,
+                                         const int32_t *sizes, float scale);
 
 /* ---- cluster tree ---------------------------------------------------------- */
 

@@ -571,3 +571,199 @@ def heatmap_level(t, level, dist, diag, p, o, seed, segment, run_job):
         cl[c]["pos"] = [F(x) for x in r["pos"][i]]
     t["_candidates"] = {0: [(s, active, rr) for s, rr in results]}
     return dict(n_blocks=1, n_jobs=restarts, score_anchor=score, avg_dist=avg_dist)
+
+
+# ---------------------------------------------------------------------------
+# heatmap input construction (upstream of createDistanceHeatmap)
+# ---------------------------------------------------------------------------
+MAX_POSITION = 2000000000  # LooperSolver.hpp:2554
+LAST_BREAK = 1000000000    # :2536
+
+
+def singleton_breaks(clusters):
+    """Bin boundaries of one chromosome (LooperSolver.hpp:2527-2536): 0, the
+    midpoints between consecutive clusters of the level, 1e9."""
+    return [0] + [c_div(clusters[i][1] + clusters[i + 1][0], 2)
+                  for i in range(len(clusters) - 1)] + [LAST_BREAK]
+
+
+def _first_bin(breaks, pos):
+    """Linear scan of the reference (:2652-2657): the first i with
+    breaks[i] <= pos <= breaks[i+1], or -1."""
+    for i in range(len(breaks) - 1):
+        if breaks[i] <= pos <= breaks[i + 1]:
+            return i
+    return -1
+
+
+def _first_bins(breaks, pos):
+    """_first_bin for an array of positions (numpy, same first-match rule)."""
+    b = np.asarray(breaks, np.int64)
+    pos = np.asarray(pos, np.int64)
+    out = np.full(pos.shape, -1, np.int64)
+    for i in range(len(b) - 2, -1, -1):   # the lowest matching i is written last
+        hit = (b[i] <= pos) & (pos <= b[i + 1])
+        out[hit] = i
+    return out
+
+
+def length_excluding_centromere(cent, st, end):
+    """LooperSolver::getGenomicLengthExcludingCentromere (:5916-5935), cent =
+    the chromosome's first centromere region (start, end) or None."""
+    if cent is None:
+        return end - st + 1
+    cs, ce = cent
+    if cs > end or ce < st:
+        return end - st + 1
+    if cs < st:
+        if ce < end:
+            return end - ce + 1
+        return 0
+    if ce < end:
+        return end - st - (ce - cs + 1) + 1
+    return cs - st
+
+
+def singleton_heatmap(chrs, clusters, records, region=(0, 0), centromeres=(), long_arcs=(),
+                      long_scale=10.0, long_power=2.0, diag=1):
+    """LooperSolver::createSingletonHeatmap (:2517-2749) on parsed records.
+    chrs: chromosome names in order; clusters: per chromosome the level's
+    (start, end) in level order; records: (chr1, sta, stb, chr2, enda, endb,
+    score); centromeres: (chr, start, end) regions (the first per chromosome
+    counts); long_arcs: (chr, start, end, score).  Returns (heat [N,N]
+    float32, counts dict).  Float sums in record order, as the reference."""
+    breaks, start_ind, cur = {}, {}, 0
+    for c, cl in zip(chrs, clusters):
+        breaks[c] = singleton_breaks(cl)
+        start_ind[c] = cur
+        cur += len(breaks[c]) - 1
+    n = cur
+    h = np.zeros((n, n), np.float32)
+    cmax = {c: 0 for c in chrs}
+    cmin = {c: MAX_POSITION for c in chrs}
+    sel = region[1] > 0
+    cnt = dict(added=0, diagonal=0, bad_chr=0)
+    rec = list(records)
+    idx = {c: k for k, c in enumerate(chrs)}
+    c1 = np.array([idx.get(r[0], -1) for r in rec], np.int64)
+    c2 = np.array([idx.get(r[3], -1) for r in rec], np.int64)
+    sta, stb, enda, endb, sc = (np.array([r[k] for r in rec], np.int64) for k in (1, 2, 4, 5, 6))
+    good = (c1 >= 0) & (c2 >= 0)
+    cnt["bad_chr"] = int((~good).sum())
+    if sel:
+        good &= (sta >= region[0]) & (sta <= region[1]) & (endb >= region[0]) & (endb <= region[1])
+    sta = np.where(sta < 0, 0, sta)
+    enda = np.where(enda < 0, 0, enda)
+    st = np.full(len(rec), -1, np.int64)
+    en = np.full(len(rec), -1, np.int64)
+    for c, k in idx.items():
+        m = good & (c1 == k)
+        st[m] = _first_bins(breaks[c], sta[m]) + np.where(_first_bins(breaks[c], sta[m]) >= 0, start_ind[c], 0)
+        m = good & (c2 == k)
+        en[m] = _first_bins(breaks[c], enda[m]) + np.where(_first_bins(breaks[c], enda[m]) >= 0, start_ind[c], 0)
+    if np.any(good & ((st < 0) | (en < 0))):
+        raise ReferenceError_("non matching arc (the reference exits)")
+    dg = good & (st == en)
+    cnt["diagonal"] = int(dg.sum())
+    ok = good & (st != en)
+    cnt["added"] = int(ok.sum())
+    for c, k in idx.items():
+        for mc, lo, hi in ((c1 == k, sta, stb), (c2 == k, enda, endb)):
+            m = ok & mc
+            if m.any():
+                cmax[c] = max(cmax[c], int(hi[m].max()))
+                cmin[c] = min(cmin[c], int(lo[m].min()))
+    # float accumulation in record order (np.add.at is unbuffered and ordered)
+    np.add.at(h, (st[ok], en[ok]), sc[ok].astype(np.float32))
+    np.add.at(h, (en[ok], st[ok]), sc[ok].astype(np.float32))
+    for c in chrs:
+        for (ac, a, b, s) in long_arcs:
+            if ac != c:
+                continue
+            i, j = _first_bin(breaks[c], a), _first_bin(breaks[c], b)
+            if i < 0 or j < 0:
+                raise ReferenceError_("long arc outside the bins (the reference indexes -1)")
+            i += start_ind[c]
+            j += start_ind[c]
+            if i == j:
+                continue
+            val = F(float(F(long_scale)) * math.pow(float(s), float(F(long_power))))
+            h[i, j] = F(h[i, j] + val)   # twice into [st][end] (:2735-2736)
+            h[i, j] = F(h[i, j] + val)
+    cent = {}
+    for (c, s, e) in centromeres:
+        cent.setdefault(c, (s, e))
+    lens = []
+    for c in chrs:
+        b = breaks[c]
+        b[0] = cmin[c]
+        b[-1] = cmax[c]
+        for i in range(len(b) - 1):
+            d = length_excluding_centromere(cent.get(c) if centromeres else None, b[i], b[i + 1])
+            lens.append(F(F(d) / F(1e6)))
+    lens = np.array(lens, np.float32)
+    with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+        iu = np.triu_indices(n, 1)
+        v = h[iu] / (lens[iu[0]] * lens[iu[1]])
+    out = np.zeros_like(h)
+    out[iu] = v
+    out[(iu[1], iu[0])] = v
+    for i in range(n):
+        for p in range(diag):
+            if i + p < n:
+                out[i, i + p] = 0.0
+                out[i + p, i] = 0.0
+    return out, cnt
+
+
+def normalize_heatmap(heat):
+    """LooperSolver::normalizeHeatmap (:2996-3040): every row scaled to the
+    average row sum (rows summing to <= 1e-12 become 0), then symmetrised as
+    the mean of the two triangles.  Sequential float sums."""
+    v = np.asarray(heat, np.float32)
+    n = v.shape[0]
+    rows = np.array([np.add.accumulate(v[i], dtype=np.float32)[-1] if n else F(0) for i in range(n)],
+                    np.float32)
+    expected = np.add.accumulate(rows, dtype=np.float32)[-1] if n else F(0)
+    expected = F(expected / F(n))
+    with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+        mn = np.where(rows > F(1e-12), expected / rows, F(0)).astype(np.float32)
+        h = (v * mn[:, None]).astype(np.float32)
+    iu = np.triu_indices(n, 1)
+    s = ((h[iu] + h[(iu[1], iu[0])]) / F(2)).astype(np.float32)
+    h[iu] = s
+    h[(iu[1], iu[0])] = s
+    return h
+
+
+def normalize_heatmap_diagonal_total(heat, val=1.0):
+    """LooperSolver::normalizeHeatmapDiagonalTotal (:3146-3172): scale so that
+    the first non-zero diagonal averages `val` (FP64 average); unchanged when
+    that average is not above 1e-12."""
+    h = np.array(heat, np.float32)
+    n = h.shape[0]
+    w = diagonal_size(h)
+    avg = 0.0
+    for i in range(n - w):
+        avg += float(h[i, i + w])
+    avg /= (n - w)
+    if not avg > 1e-12:
+        return h
+    mn = F(float(F(val)) / avg)
+    return (h * mn).astype(np.float32)
+
+
+def normalize_heatmap_inter(heat, sizes, scale):
+    """LooperSolver::normalizeHeatmapInter (:3106-3144): every cell times
+    `scale`, then the intra-chromosome blocks divided by it again (float);
+    unchanged for one chromosome."""
+    h = np.array(heat, np.float32)
+    if len(sizes) <= 1:
+        return h
+    s = F(scale)
+    h = (h * s).astype(np.float32)
+    o = 0
+    for k in sizes:
+        h[o:o + k, o:o + k] = (h[o:o + k, o:o + k] / s).astype(np.float32)
+        o += k
+    return h

@@ -24,6 +24,10 @@
 //   metrics   MetricsFramework::pearsonCorrelation, distanceCorrelation,
 //             structuralSimilarity (MetricsFramework.hpp:238-479) and
 //             Chromosome::calcRMSD -> fast_rmsd (Chromosome.hpp:613-634)
+//   singleton tree, then LooperSolver::createSingletonHeatmap (:2517-2749)
+//             on records written to a temporary singleton file
+//   normalize normalizeHeatmap (:2996-3040), normalizeHeatmapDiagonalTotal
+//             (:3146-3172), normalizeHeatmapInter (:3106-3144)
 //
 // The reference keeps these members private; this driver is a white-box test,
 // so it lifts the access specifiers before including the reference header.
@@ -32,6 +36,7 @@
 #define protected public
 #include <LooperSolver.hpp>
 #include <MetricsFramework.hpp>
+#include <AppCommon.hpp>  // defines g_cancel_requested (read by createSingletonHeatmap)
 #undef private
 #undef protected
 #include <Static.hpp>
@@ -389,6 +394,92 @@ static void mode_metrics() {
   printf("\n@@metrics %a %a %a %a\n", pear, dcor, scc, rmsd);
 }
 
+// singleton: tree (read_tree), then level diag long_scale long_power
+//   region_start region_end, n_centromeres "chr start end", n_long
+//   "chr start end score", n_records "chr1 sta stb chr2 enda endb score"
+//   (written to a temporary singleton file), then
+//   LooperSolver::createSingletonHeatmap (LooperSolver.hpp:2517-2749).
+static void mode_singleton(LooperSolver &s) {
+  const int level = geti(), diag = geti();
+  Settings::longPETClustersEffectScale = getf();
+  Settings::longPETClustersEffectPower = getf();
+  const int rs = geti(), re = geti();
+  s.selected_region = BedRegion("sel", rs, re);
+  s.selected_region.start = rs;
+  s.selected_region.end = re;
+  const int n_cent = geti();
+  for (int i = 0; i < n_cent; ++i) {
+    std::string chr = tok();
+    int st = geti(), en = geti();
+    s.centromeres.regions.push_back(BedRegion(chr, st, en));
+  }
+  const int n_long = geti();
+  for (int i = 0; i < n_long; ++i) {
+    std::string chr = tok();
+    int st = geti(), en = geti(), sc = geti();
+    s.arcs.long_arcs[chr].push_back(InteractionArc(st, en, sc));
+  }
+  const int n_rec = geti();
+  char path[] = "/tmp/pmmc_singletons_XXXXXX";
+  const int fd = mkstemp(path);
+  FILE *f = fdopen(fd, "w");
+  for (int i = 0; i < n_rec; ++i) {
+    std::string c1 = tok();
+    int sta = geti(), stb = geti();
+    std::string c2 = tok();
+    int enda = geti(), endb = geti(), sc = geti();
+    fprintf(f, "%s %d %d %s %d %d %d\n", c1.c_str(), sta, stb, c2.c_str(), enda, endb, sc);
+  }
+  fclose(f);
+  Settings::dataSplitSingletonFilesByChr = false;
+  s.arcs_singletons = {path};
+  s.arcs_singletons_inter.clear();
+  s.setLevel(level);
+  fflush(stdout);
+  Heatmap h = s.createSingletonHeatmap(diag);
+  unlink(path);
+  printf("\n@@singleton %zu\n", h.size);
+  for (size_t i = 0; i < h.size; ++i) {
+    for (size_t j = 0; j < h.size; ++j) printf("%a ", h.v[i][j]);
+    printf("\n");
+  }
+}
+
+// normalize: n_chr, bins per chromosome, val scale, then the n x n heatmap;
+// prints normalizeHeatmap (LooperSolver.hpp:2996-3040), then
+// normalizeHeatmapDiagonalTotal (:3146-3172) and normalizeHeatmapInter
+// (:3106-3144) each applied to a copy of the input.
+static void mode_normalize(LooperSolver &s) {
+  const int n_chr = geti();
+  int n = 0;
+  s.chrs.clear();
+  for (int c = 0; c < n_chr; ++c) {
+    const std::string chr = "chr" + std::to_string(c + 1);
+    const int k = geti();
+    s.chrs.push_back(chr);
+    s.current_level[chr] = std::vector<int>(k, 0);
+    n += k;
+  }
+  const float val = getf(), scale = getf();
+  Heatmap h;
+  h.init(n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) h.v[i][j] = getf();
+  fflush(stdout);
+  Heatmap a = s.normalizeHeatmap(h);
+  Heatmap b = h;
+  s.normalizeHeatmapDiagonalTotal(b, val);
+  Heatmap c = h;
+  s.normalizeHeatmapInter(c, scale);
+  printf("\n@@normalize %d\n", n);
+  for (Heatmap *m : {&a, &b, &c}) {
+    for (int i = 0; i < n; ++i) {
+      for (int j = 0; j < n; ++j) printf("%a ", m->v[i][j]);
+      printf("\n");
+    }
+  }
+}
+
 int main() {
   Settings settings;
   settings.init();
@@ -431,6 +522,11 @@ int main() {
     mode_time_anneal(s);
   } else if (mode == "metrics") {
     mode_metrics();
+  } else if (mode == "singleton") {
+    read_tree(s);
+    mode_singleton(s);
+  } else if (mode == "normalize") {
+    mode_normalize(s);
   } else {
     fprintf(stderr, "unknown mode %s\n", mode.c_str());
     return 2;

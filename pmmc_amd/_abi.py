@@ -73,6 +73,24 @@ class Run(C.Structure):
                 ("n_chains", C.c_int32)]
 
 
+class SingletonInput(C.Structure):
+    _fields_ = [("n_chr", C.c_int32), ("cluster_offsets", i32p), ("cluster_start", i32p),
+                ("cluster_end", i32p), ("n_records", C.c_int64), ("chr1", i32p), ("start1", i32p),
+                ("end1", i32p), ("chr2", i32p), ("start2", i32p), ("end2", i32p), ("score", i32p),
+                ("region_start", C.c_int32), ("region_end", C.c_int32), ("centromere", i32p),
+                ("n_long", C.c_int32), ("long_chr", i32p), ("long_start", i32p), ("long_end", i32p),
+                ("long_score", i32p), ("long_scale", C.c_float), ("long_power", C.c_float),
+                ("diag", C.c_int32)]
+
+
+class SingletonCounts(C.Structure):
+    _fields_ = [("added", C.c_int64), ("diagonal", C.c_int64), ("bad_chr", C.c_int64),
+                ("kernel_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class ProbeResult(C.Structure):
     _fields_ = [("moves_per_s", C.c_double), ("read_gbps", C.c_double), ("write_gbps", C.c_double),
                 ("stream_read_gbps", C.c_double), ("lds_gbps", C.c_double), ("ms", C.c_double)]
@@ -202,6 +220,8 @@ SIGNATURES = {
     "pmmc_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_uint64)]),
     "pmmc_server_step": (C.c_int, [C.c_void_p, i32p, i32p, i32p]),
     "pmmc_server_done": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "pmmc_singleton_heatmap": (C.c_int, [C.c_void_p, C.POINTER(SingletonInput), f32p,
+                                         C.POINTER(SingletonCounts)]),
     "pmmc_probe_move_pattern": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int32, C.c_int32, C.c_int32,
                                           C.c_int32, C.c_int32, C.POINTER(ProbeResult)]),
     "pmmc_spline_count": (C.c_int64, [C.c_int64, C.c_int64, C.c_int32]),
@@ -212,6 +232,9 @@ SIGNATURES = {
     "pmmc_host_last_error": (C.c_char_p, []),
     "pmmc_freq_to_distance": (C.c_float, [C.c_int32, C.POINTER(LevelParams)]),
     "pmmc_genomic_length_to_distance": (C.c_double, [C.c_int32, C.POINTER(LevelParams)]),
+    "pmmc_normalize_heatmap": (C.c_int, [f32p, C.c_int32, f32p]),
+    "pmmc_normalize_heatmap_diagonal_total": (C.c_int, [f32p, C.c_int32, C.c_float]),
+    "pmmc_normalize_heatmap_inter": (C.c_int, [f32p, C.c_int32, C.c_int32, i32p, C.c_float]),
     "pmmc_distance_heatmap": (C.c_int, [f32p, C.c_int32, C.c_int32, C.c_int32,
                                         C.POINTER(LevelParams), f32p, f32p]),
     "pmmc_tree_create": (C.c_int, [C.c_int32, C.POINTER(Chromosome), C.c_int32,

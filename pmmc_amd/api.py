@@ -170,6 +170,59 @@ class Context:
             out.ctypes.data_as(A.f32p)))
         return out
 
+    def singleton_heatmap(self, chrs, clusters, records, region=(0, 0), centromeres=(),
+                          long_arcs=(), long_scale=10.0, long_power=2.0, diag=1):
+        """pmmc_singleton_heatmap: contact heatmap of a level's clusters from
+        singleton records.  chrs: chromosome names in order; clusters: per
+        chromosome the level's (start, end) in level order; records: either
+        (chr1, start1, end1, chr2, start2, end2, score) tuples with chromosome
+        names, or a dict of int32 arrays chr1/start1/end1/chr2/start2/end2/score
+        with chromosome indices (-1: not processed); centromeres: (chr, start,
+        end), the first per chromosome counts; long_arcs: (chr, start, end,
+        score).  Returns (heat [N, N] float32, counts dict)."""
+        idx = {c: k for k, c in enumerate(chrs)}
+        off = np.zeros(len(chrs) + 1, np.int32)
+        for k, cl in enumerate(clusters):
+            off[k + 1] = off[k] + len(cl)
+        cs = np.array([c[0] for cl in clusters for c in cl], np.int32)
+        ce = np.array([c[1] for cl in clusters for c in cl], np.int32)
+        if isinstance(records, dict):
+            rec = {k: np.ascontiguousarray(records[k], np.int32)
+                   for k in ("chr1", "start1", "end1", "chr2", "start2", "end2", "score")}
+        else:
+            rl = list(records)
+            rec = {"chr1": np.array([idx.get(r[0], -1) for r in rl], np.int32),
+                   "start1": np.array([r[1] for r in rl], np.int32),
+                   "end1": np.array([r[2] for r in rl], np.int32),
+                   "chr2": np.array([idx.get(r[3], -1) for r in rl], np.int32),
+                   "start2": np.array([r[4] for r in rl], np.int32),
+                   "end2": np.array([r[5] for r in rl], np.int32),
+                   "score": np.array([r[6] for r in rl], np.int32)}
+        cent = None
+        if centromeres:
+            cent = np.zeros(3 * len(chrs), np.int32)
+            for c, a, b in centromeres:
+                k = idx.get(c, -1)
+                if k >= 0 and cent[3 * k] == 0:
+                    cent[3 * k:3 * k + 3] = (1, a, b)
+        la = list(long_arcs)
+        lchr = np.array([idx.get(a[0], -1) for a in la], np.int32)
+        lst = np.array([a[1] for a in la], np.int32)
+        len_ = np.array([a[2] for a in la], np.int32)
+        lsc = np.array([a[3] for a in la], np.int32)
+        n = int(off[-1])
+        inp = A.SingletonInput(
+            len(chrs), _ptr(off, A.i32p), _ptr(cs, A.i32p), _ptr(ce, A.i32p), len(rec["chr1"]),
+            *(_ptr(rec[k], A.i32p) for k in ("chr1", "start1", "end1", "chr2", "start2", "end2", "score")),
+            int(region[0]), int(region[1]), _ptr(cent, A.i32p), len(la), _ptr(lchr, A.i32p),
+            _ptr(lst, A.i32p), _ptr(len_, A.i32p), _ptr(lsc, A.i32p), float(long_scale),
+            float(long_power), int(diag))
+        heat = np.empty((n, n), np.float32)
+        cnt = A.SingletonCounts()
+        self._check(self.lib.pmmc_singleton_heatmap(self.h, C.byref(inp), _ptr(heat, A.f32p),
+                                                    C.byref(cnt)))
+        return heat, cnt.as_dict()
+
     def probe_move_pattern(self, bytes_: int, waves: int, moves_per_wave: int, reads: int,
                            chunk_beads: int, write_mode: int) -> dict:
         """pmmc_probe_move_pattern: sustained bandwidth of the move kernels'

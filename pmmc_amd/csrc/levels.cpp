@@ -148,6 +148,71 @@ extern "C" pmmc_status pmmc_distance_heatmap(const float *freq, int32_t n, int32
   return PMMC_OK;
 }
 
+// LooperSolver::normalizeHeatmap (LooperSolver.hpp:2996-3040)
+extern "C" pmmc_status pmmc_normalize_heatmap(const float *in, int32_t n, float *out) {
+  if (!in || !out || n < 0)
+    return fail("pmmc_normalize_heatmap: null pointer or negative size", PMMC_ERR_INVALID);
+  std::vector<float> rows(n);
+  float expected = 0.0f;
+  for (int i = 0; i < n; ++i) {
+    float sum = 0.0f;
+    for (int j = 0; j < n; ++j) sum += in[(size_t)i * n + j];
+    rows[i] = sum;
+    expected += sum;
+  }
+  expected /= n;
+  std::vector<float> h((size_t)n * n);
+  for (int i = 0; i < n; ++i) {
+    const float mn = rows[i] > 1e-12f ? expected / rows[i] : 0.0f;
+    for (int j = 0; j < n; ++j) h[(size_t)i * n + j] = in[(size_t)i * n + j] * mn;
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      const float v = (h[(size_t)i * n + j] + h[(size_t)j * n + i]) / 2.0f;
+      h[(size_t)i * n + j] = v;
+      h[(size_t)j * n + i] = v;
+    }
+  std::copy(h.begin(), h.end(), out);
+  return PMMC_OK;
+}
+
+// LooperSolver::normalizeHeatmapDiagonalTotal (LooperSolver.hpp:3146-3172)
+extern "C" pmmc_status pmmc_normalize_heatmap_diagonal_total(float *heat, int32_t n, float val) {
+  if (!heat || n < 0)
+    return fail("pmmc_normalize_heatmap_diagonal_total: null pointer or negative size",
+                PMMC_ERR_INVALID);
+  const int w = diagonal_size(heat, n);
+  double avg = 0.0;
+  for (int i = 0; i + w < n; ++i) avg += heat[(size_t)i * n + i + w];
+  avg /= static_cast<double>(n - w);
+  if (!(avg > 1e-12)) return PMMC_OK;
+  const float mn = static_cast<float>(val / avg);
+  for (size_t k = 0; k < (size_t)n * n; ++k) heat[k] *= mn;
+  return PMMC_OK;
+}
+
+// LooperSolver::normalizeHeatmapInter (LooperSolver.hpp:3106-3144)
+extern "C" pmmc_status pmmc_normalize_heatmap_inter(float *heat, int32_t n, int32_t n_chr,
+                                                    const int32_t *sizes, float scale) {
+  if (!heat || n < 0 || n_chr < 0 || (n_chr > 0 && !sizes))
+    return fail("pmmc_normalize_heatmap_inter: null pointer or negative size", PMMC_ERR_INVALID);
+  int64_t total = 0;
+  for (int c = 0; c < n_chr; ++c) {
+    if (sizes[c] < 0) return fail("pmmc_normalize_heatmap_inter: negative size", PMMC_ERR_INVALID);
+    total += sizes[c];
+  }
+  if (total != n) return fail("pmmc_normalize_heatmap_inter: sizes do not sum to n", PMMC_ERR_INVALID);
+  if (n_chr <= 1) return PMMC_OK;
+  for (size_t k = 0; k < (size_t)n * n; ++k) heat[k] *= scale;
+  int o = 0;
+  for (int c = 0; c < n_chr; ++c) {
+    for (int i = o; i < o + sizes[c]; ++i)
+      for (int j = o; j < o + sizes[c]; ++j) heat[(size_t)i * n + j] /= scale;
+    o += sizes[c];
+  }
+  return PMMC_OK;
+}
+
 // ---------------------------------------------------------------- tree
 
 namespace {

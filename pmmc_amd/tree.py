@@ -60,6 +60,36 @@ def distance_heatmap(freq, diagonal_size: int, inter: bool, p: A.LevelParams):
     return out, np.float32(avg.value)
 
 
+def normalize_heatmap(heat):
+    """pmmc_normalize_heatmap (LooperSolver::normalizeHeatmap)."""
+    h = np.ascontiguousarray(heat, np.float32)
+    out = np.empty_like(h)
+    _check(_lib().pmmc_normalize_heatmap(h.ctypes.data_as(_f32p), h.shape[0],
+                                         out.ctypes.data_as(_f32p)), "pmmc_normalize_heatmap")
+    return out
+
+
+def normalize_heatmap_diagonal_total(heat, val=1.0):
+    """pmmc_normalize_heatmap_diagonal_total (normalizeHeatmapDiagonalTotal)."""
+    h = np.array(heat, np.float32)
+    _check(_lib().pmmc_normalize_heatmap_diagonal_total(h.ctypes.data_as(_f32p), h.shape[0],
+                                                        float(val)),
+           "pmmc_normalize_heatmap_diagonal_total")
+    return h
+
+
+def normalize_heatmap_inter(heat, sizes, scale):
+    """pmmc_normalize_heatmap_inter (normalizeHeatmapInter); sizes: bins per
+    chromosome."""
+    h = np.array(heat, np.float32)
+    sz = np.ascontiguousarray(sizes, np.int32)
+    _check(_lib().pmmc_normalize_heatmap_inter(h.ctypes.data_as(_f32p), h.shape[0], len(sz),
+                                               sz.ctypes.data_as(C.POINTER(C.c_int32)),
+                                               float(scale)),
+           "pmmc_normalize_heatmap_inter")
+    return h
+
+
 def _i32(a):
     return np.ascontiguousarray(np.asarray(a, np.int64).astype(np.int32))
 

@@ -42,6 +42,7 @@ STRUCTS = {
     "pmmc_segment_region": A.SegmentRegion, "pmmc_cluster": A.Cluster,
     "pmmc_level_report": A.LevelReport, "pmmc_arcs_level_opts": A.ArcsLevelOpts,
     "pmmc_heatmap_level_opts": A.HeatmapLevelOpts, "pmmc_probe_result": A.ProbeResult,
+    "pmmc_singleton_input": A.SingletonInput, "pmmc_singleton_counts": A.SingletonCounts,
 }
 
 
