@@ -266,8 +266,63 @@ def heatmap_sweep_leg(ctx, with_cpu=True, cpu_seconds=8.0):
     return out
 
 
+def singleton_leg(ctx, with_cpu=True, cpu_seconds=8.0):
+    """createSingletonHeatmap on the GPU (pmmc_singleton_heatmap): 20 M
+    singleton records (70% intra-chromosome) binned onto the anchor level of
+    3 chromosomes x 200 anchors (600 bins: sums in the device matrix) and onto
+    12 segment-like bins (sums in per-workgroup LDS histograms).  Kernel time
+    from HIP events around the binning launch; end to end includes the
+    records' upload and the host's O(bins^2) float steps."""
+    import numpy as np
+    del cpu_seconds
+    rng = np.random.default_rng(11)
+    out = {}
+    for name, per_chr in (("anchor_600_bins", 200), ("segment_12_bins", 4)):
+        chrs, clusters = [], []
+        for c in range(3):
+            gaps = rng.integers(2000, 30000, size=per_chr)
+            lens = rng.integers(500, 8000, size=per_chr)
+            starts = 50_000 + np.cumsum(gaps + lens) - lens
+            chrs.append(f"chr{c + 1}")
+            clusters.append([(int(a), int(a + b)) for a, b in zip(starts, lens)])
+        n = 20_000_000
+        hi = np.array([cl[-1][1] + 20_000 for cl in clusters], np.int64)
+        c1 = rng.integers(0, 3, size=n)
+        c2 = np.where(rng.random(n) < 0.7, c1, rng.integers(0, 3, size=n))
+        s1 = (rng.random(n) * hi[c1]).astype(np.int64)
+        s2 = (rng.random(n) * hi[c2]).astype(np.int64)
+        rec = {k: v.astype(np.int32) for k, v in dict(
+            chr1=c1, start1=s1, end1=s1 + 150, chr2=c2, start2=s2, end2=s2 + 150,
+            score=rng.integers(1, 5, size=n)).items()}
+        ctx.singleton_heatmap(chrs, clusters, rec)
+        t0 = time.perf_counter()
+        _, cnt = ctx.singleton_heatmap(chrs, clusters, rec)
+        wall = time.perf_counter() - t0
+        ks = cnt["kernel_ms"] * 1e-3
+        leg = {"workload": f"singleton records binned onto {3 * per_chr} bins of 3 chromosomes",
+               "records": n, "kernel_records_per_s": n / ks, "records_per_s": n / wall,
+               "kernel_ms": cnt["kernel_ms"], "added": cnt["added"],
+               "bytes_per_record": 28, "achieved_GBps": 28 * n / ks / 1e9,
+               "frac_of_hbm_peak": 28 * n / ks / 1e9 / HBM_PEAK_GBS,
+               "bound": ("atomics on the device matrix" if per_chr > 20 else
+                         "LDS histogram atomics")}
+        if with_cpu and name == "anchor_600_bins":
+            from oracle import refdriver
+            if refdriver.available():
+                m = 2_000_000
+                sub = {k: v[:m] for k, v in rec.items()}
+                secs, cnt_ref = refdriver.singleton_time(chrs, clusters, sub)
+                leg["cpu_reference"] = {
+                    "value": cnt_ref / secs, "unit": "records/s", "cores": 1, "kind": "reference",
+                    "sample": f"LooperSolver::createSingletonHeatmap on {cnt_ref} of the records "
+                              "(the same bins), one thread, reading them from a singleton text "
+                              "file as the reference does"}
+        out[name] = leg
+    return out
+
+
 LEGS = {"heatmap_64": heatmap_leg, "heatmap_64_sweep": heatmap_sweep_leg,
-        "smooth_19951": smooth_leg}
+        "smooth_19951": smooth_leg, "singleton_20M": singleton_leg}
 
 
 def levels(ctx, with_cpu=True, cpu_seconds=8.0):

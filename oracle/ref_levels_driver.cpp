@@ -26,6 +26,7 @@
 //             Chromosome::calcRMSD -> fast_rmsd (Chromosome.hpp:613-634)
 //   singleton tree, then LooperSolver::createSingletonHeatmap (:2517-2749)
 //             on records written to a temporary singleton file
+//   singleton_clusters  the same on clusters given directly, timed
 //   normalize normalizeHeatmap (:2996-3040), normalizeHeatmapDiagonalTotal
 //             (:3146-3172), normalizeHeatmapInter (:3106-3144)
 //
@@ -399,8 +400,38 @@ static void mode_metrics() {
 //   "chr start end score", n_records "chr1 sta stb chr2 enda endb score"
 //   (written to a temporary singleton file), then
 //   LooperSolver::createSingletonHeatmap (LooperSolver.hpp:2517-2749).
+static void singleton_run(LooperSolver &s, int level, int diag, bool use_level);
+
 static void mode_singleton(LooperSolver &s) {
   const int level = geti(), diag = geti();
+  singleton_run(s, level, diag, true);
+}
+
+// singleton_clusters: n_chr, per chromosome "name n" and n "start end" of
+// the level's clusters (no tree), then the singleton input from diag on;
+// also prints the seconds spent in createSingletonHeatmap (@@singleton_time).
+static void mode_singleton_clusters(LooperSolver &s) {
+  const int n_chr = geti();
+  s.chrs.clear();
+  for (int c = 0; c < n_chr; ++c) {
+    const std::string chr = tok();
+    const int n = geti();
+    s.chrs.push_back(chr);
+    std::vector<int> ids;
+    for (int i = 0; i < n; ++i) {
+      Cluster k;
+      k.start = geti();
+      k.end = geti();
+      ids.push_back(static_cast<int>(s.clusters.size()));
+      s.clusters.push_back(k);
+    }
+    s.current_level[chr] = ids;
+  }
+  const int diag = geti();
+  singleton_run(s, -1, diag, false);
+}
+
+static void singleton_run(LooperSolver &s, int level, int diag, bool use_level) {
   Settings::longPETClustersEffectScale = getf();
   Settings::longPETClustersEffectPower = getf();
   const int rs = geti(), re = geti();
@@ -434,10 +465,13 @@ static void mode_singleton(LooperSolver &s) {
   Settings::dataSplitSingletonFilesByChr = false;
   s.arcs_singletons = {path};
   s.arcs_singletons_inter.clear();
-  s.setLevel(level);
+  if (use_level) s.setLevel(level);
   fflush(stdout);
+  const auto t0 = std::chrono::steady_clock::now();
   Heatmap h = s.createSingletonHeatmap(diag);
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   unlink(path);
+  printf("\n@@singleton_time %.6f %d\n", secs, n_rec);
   printf("\n@@singleton %zu\n", h.size);
   for (size_t i = 0; i < h.size; ++i) {
     for (size_t j = 0; j < h.size; ++j) printf("%a ", h.v[i][j]);
@@ -525,6 +559,8 @@ int main() {
   } else if (mode == "singleton") {
     read_tree(s);
     mode_singleton(s);
+  } else if (mode == "singleton_clusters") {
+    mode_singleton_clusters(s);
   } else if (mode == "normalize") {
     mode_normalize(s);
   } else {

@@ -3,9 +3,10 @@
 oracle/_ref/levels_ref is the reference's LooperSolver compiled in place from
 its sources by oracle/Makefile (target `ref`; only where the reference
 sources exist, the binary then travels with the tree).  This module formats
-its `anneal` / `time_anneal` input and reads its output.  Used by
-tests/golden/make_anneal_fixtures.py (fixtures) and bench.py's cpu_baseline
-leg (the reference's CPU annealing timed on the host's cores).
+its `anneal` / `time_anneal` / `singleton_clusters` input and reads its
+output.  Used by tests/golden/make_anneal_fixtures.py (fixtures) and
+bench.py's CPU legs (the reference's CPU annealing, and its singleton
+heatmap construction, timed on the host's cores).
 """
 from __future__ import annotations
 
@@ -89,3 +90,27 @@ def time_anneal(level: int, prob, step: float, settings: dict, seconds: float, t
         moves += int(m)
         secs = max(secs, float(s))
     return moves, secs, runs
+
+
+def singleton_time(chrs, clusters, rec, diag=1, long_scale=10.0, long_power=2.0) -> tuple[float, int]:
+    """LooperSolver::createSingletonHeatmap run once by the reference on the
+    level's clusters and records (dict of int32 arrays chr1/start1/end1/chr2/
+    start2/end2/score, chromosome indices, -1: a chromosome not processed),
+    single-threaded as the reference runs it, reading them from a singleton
+    text file as it does.  Returns (seconds inside the call, records)."""
+    names = list(chrs) + ["chrUn"]
+    parts = [f"singleton_clusters {HEADER}\n{len(chrs)}\n"]
+    for c, cl in zip(chrs, clusters):
+        parts.append(f"{c} {len(cl)}\n" + "".join(f"{a} {b}\n" for a, b in cl))
+    parts.append(f"{diag}\n{float(np.float32(long_scale))} {float(np.float32(long_power))}\n0 0\n0\n0\n")
+    n = len(rec["chr1"])
+    cols = [np.asarray(rec[k]) for k in ("chr1", "start1", "end1", "chr2", "start2", "end2", "score")]
+    c1 = np.array(names, dtype=object)[cols[0]]
+    c2 = np.array(names, dtype=object)[cols[3]]
+    lines = [f"{a} {b} {c} {d} {e} {f} {g}" for a, b, c, d, e, f, g in
+             zip(c1, cols[1].tolist(), cols[2].tolist(), c2, cols[4].tolist(), cols[5].tolist(),
+                 cols[6].tolist())]
+    parts.append(f"{n}\n" + "\n".join(lines) + "\n")
+    out = run("".join(parts), "@@singleton_time")
+    _, secs, cnt = out[0].split()
+    return float(secs), int(cnt)

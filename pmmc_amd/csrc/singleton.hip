@@ -7,7 +7,7 @@
 // scores as 64-bit integers -- in a per-workgroup LDS histogram when the whole
 // N x N matrix fits, straight into the device matrix otherwise -- and keeps
 // the counters and the chromosomes' extreme positions with one atomic per
-// wavefront or workgroup.  The O(N^2) remainder (long-range arcs, bin
+// workgroup.  The O(N^2) remainder (long-range arcs, bin
 // lengths, normalisation, diagonal) is the reference's float arithmetic, in
 // its order, on the host.
 #include <hip/hip_runtime.h>
@@ -28,6 +28,7 @@ namespace {
 constexpr int kSingletonThreads = 256;
 constexpr int kSingletonMaxChr = 64;             // per-workgroup min/max slots
 constexpr int kSingletonLdsCells = 6144;         // 48 KB of 64-bit cells
+constexpr int kSingletonLdsTable = 8192;         // 32 KB of breaks and tables
 constexpr int32_t kLastBreak = 1000000000;       // LooperSolver.hpp:2536
 constexpr int32_t kMaxPosition = 2000000000;     // :2554
 
@@ -65,15 +66,15 @@ __device__ __forceinline__ int find_bin(const int32_t *b, int nb, bool sorted, i
   return -1;
 }
 
-__device__ __forceinline__ void wave_count(unsigned long long *c, bool pred) {
-  const uint64_t m = __ballot(pred);
-  if (m && (threadIdx.x & 63) == __builtin_ctzll(m)) atomicAdd(c, static_cast<unsigned long long>(__popcll(m)));
-}
-
-template <bool LDS>
-__global__ __launch_bounds__(kSingletonThreads) void k_singleton_bin(SingletonArgs a) {
+// LDS: score sums in a per-workgroup histogram (the whole matrix fits);
+// TBL: the breaks and per-chromosome tables copied to LDS (tbl_n ints: the
+// bin searches' dependent reads then cost LDS, not L2, latency).
+template <bool LDS, bool TBL>
+__global__ __launch_bounds__(kSingletonThreads) void k_singleton_bin(SingletonArgs a, int tbl_n) {
   __shared__ unsigned long long hist[LDS ? kSingletonLdsCells : 1];
   __shared__ int32_t smin[kSingletonMaxChr], smax[kSingletonMaxChr];
+  __shared__ unsigned long long scount[4];
+  extern __shared__ int32_t stbl[];
   const int nn = a.n * a.n;
   if constexpr (LDS)
     for (int k = threadIdx.x; k < nn; k += blockDim.x) hist[k] = 0;
@@ -81,16 +82,27 @@ __global__ __launch_bounds__(kSingletonThreads) void k_singleton_bin(SingletonAr
     smin[c] = kMaxPosition;
     smax[c] = 0;
   }
+  if (threadIdx.x < 4) scount[threadIdx.x] = 0;
+  // the tables are contiguous: breaks, break_off, bins, bin0, sorted
+  const int32_t *breaks = a.breaks, *break_off = a.break_off, *bins = a.bins, *bin0 = a.bin0,
+                *sorted = a.sorted;
+  if constexpr (TBL) {
+    for (int k = threadIdx.x; k < tbl_n; k += blockDim.x) stbl[k] = a.breaks[k];
+    breaks = stbl;
+    break_off = stbl + (a.break_off - a.breaks);
+    bins = break_off + a.n_chr;
+    bin0 = bins + a.n_chr;
+    sorted = bin0 + a.n_chr;
+  }
   __syncthreads();
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  const int64_t first = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  // every thread runs the same number of rounds, so the per-wave counters'
-  // ballots see the whole wavefront
-  const int64_t rounds = (a.n_records + stride - 1) / stride;
-  for (int64_t r = 0; r < rounds; ++r) {
-    const int64_t i = first + r * stride;
+  // counters per thread, summed per workgroup at the end: one global atomic
+  // per counter and workgroup (per-record atomics on four addresses serialise)
+  uint32_t n_added = 0, n_diag = 0, n_bad = 0, n_unmatched = 0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n_records;
+       i += stride) {
     bool bad = false, diag = false, added = false, unmatched = false;
-    if (i < a.n_records) {
+    {
       const int c1 = a.chr1[i], c2 = a.chr2[i];
       if (c1 < 0 || c1 >= a.n_chr || c2 < 0 || c2 >= a.n_chr) {
         bad = true;
@@ -105,40 +117,46 @@ __global__ __launch_bounds__(kSingletonThreads) void k_singleton_bin(SingletonAr
         if (in_region) {
           s1 = s1 < 0 ? 0 : s1;
           s2 = s2 < 0 ? 0 : s2;
-          const int b1 = find_bin(a.breaks + a.break_off[c1], a.bins[c1], a.sorted[c1] != 0, s1);
-          const int b2 = find_bin(a.breaks + a.break_off[c2], a.bins[c2], a.sorted[c2] != 0, s2);
+          const int b1 = find_bin(breaks + break_off[c1], bins[c1], sorted[c1] != 0, s1);
+          const int b2 = find_bin(breaks + break_off[c2], bins[c2], sorted[c2] != 0, s2);
           if (b1 < 0 || b2 < 0) {
             unmatched = true;
           } else {
-            const int st = a.bin0[c1] + b1, en = a.bin0[c2] + b2;
+            const int st = bin0[c1] + b1, en = bin0[c2] + b2;
             if (st == en) {
               diag = true;
             } else {
               added = true;
-              atomicMin(&smin[c1], s1);
-              atomicMax(&smax[c1], e1);
-              atomicMin(&smin[c2], s2);
-              atomicMax(&smax[c2], e2);
+              // the extremes change rarely: atomics only when improving
+              if (s1 < smin[c1]) atomicMin(&smin[c1], s1);
+              if (e1 > smax[c1]) atomicMax(&smax[c1], e1);
+              if (s2 < smin[c2]) atomicMin(&smin[c2], s2);
+              if (e2 > smax[c2]) atomicMax(&smax[c2], e2);
               const unsigned long long sc =
                   static_cast<unsigned long long>(static_cast<int64_t>(a.score[i]));
-              if constexpr (LDS) {
-                atomicAdd(&hist[st * a.n + en], sc);
-                atomicAdd(&hist[en * a.n + st], sc);
-              } else {
-                atomicAdd(&a.cells[static_cast<size_t>(st) * a.n + en], sc);
-                atomicAdd(&a.cells[static_cast<size_t>(en) * a.n + st], sc);
-              }
+              // [st][en] and [en][st] receive the same sums: one add to the
+              // upper-triangle cell, mirrored on the host
+              const int lo = st < en ? st : en, hi = st < en ? en : st;
+              if constexpr (LDS)
+                atomicAdd(&hist[lo * a.n + hi], sc);
+              else
+                atomicAdd(&a.cells[static_cast<size_t>(lo) * a.n + hi], sc);
             }
           }
         }
       }
     }
-    wave_count(&a.counts[0], added);
-    wave_count(&a.counts[1], diag);
-    wave_count(&a.counts[2], bad);
-    wave_count(&a.counts[3], unmatched);
+    n_added += added;
+    n_diag += diag;
+    n_bad += bad;
+    n_unmatched += unmatched;
   }
+  if (n_added) atomicAdd(&scount[0], static_cast<unsigned long long>(n_added));
+  if (n_diag) atomicAdd(&scount[1], static_cast<unsigned long long>(n_diag));
+  if (n_bad) atomicAdd(&scount[2], static_cast<unsigned long long>(n_bad));
+  if (n_unmatched) atomicAdd(&scount[3], static_cast<unsigned long long>(n_unmatched));
   __syncthreads();
+  if (threadIdx.x < 4 && scount[threadIdx.x]) atomicAdd(&a.counts[threadIdx.x], scount[threadIdx.x]);
   if constexpr (LDS)
     for (int k = threadIdx.x; k < nn; k += blockDim.x)
       if (hist[k]) atomicAdd(&a.cells[k], hist[k]);
@@ -268,6 +286,9 @@ extern "C" pmmc_status pmmc_singleton_heatmap(pmmc_ctx *ctx, const pmmc_singleto
   double kernel_ms = 0.0;
   const int32_t *h_rec[7] = {in->chr1, in->start1, in->end1, in->chr2, in->start2, in->end2, in->score};
   const bool lds = nn <= static_cast<size_t>(kSingletonLdsCells);
+  const int tbl_n = static_cast<int>(tbl);
+  const bool tbl_lds = tbl <= static_cast<size_t>(kSingletonLdsTable);
+  const size_t dyn = tbl_lds ? tbl * 4 : 0;
   for (int64_t r0 = 0; r0 < in->n_records; r0 += chunk) {
     const int64_t m = std::min<int64_t>(chunk, in->n_records - r0);
     for (int k = 0; k < 7; ++k)
@@ -285,10 +306,14 @@ extern "C" pmmc_status pmmc_singleton_heatmap(pmmc_ctx *ctx, const pmmc_singleto
     const unsigned grid = static_cast<unsigned>(std::max<int64_t>(
         1, std::min<int64_t>(static_cast<int64_t>(ctx->cu_count) * 8, (m + per_group - 1) / per_group)));
     PMMC_HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-    if (lds)
-      hipLaunchKernelGGL(k_singleton_bin<true>, dim3(grid), dim3(kSingletonThreads), 0, st, a);
+    if (lds && tbl_lds)
+      hipLaunchKernelGGL((k_singleton_bin<true, true>), dim3(grid), dim3(kSingletonThreads), dyn, st, a, tbl_n);
+    else if (lds)
+      hipLaunchKernelGGL((k_singleton_bin<true, false>), dim3(grid), dim3(kSingletonThreads), 0, st, a, tbl_n);
+    else if (tbl_lds)
+      hipLaunchKernelGGL((k_singleton_bin<false, true>), dim3(grid), dim3(kSingletonThreads), dyn, st, a, tbl_n);
     else
-      hipLaunchKernelGGL(k_singleton_bin<false>, dim3(grid), dim3(kSingletonThreads), 0, st, a);
+      hipLaunchKernelGGL((k_singleton_bin<false, false>), dim3(grid), dim3(kSingletonThreads), 0, st, a, tbl_n);
     PMMC_HIP_TRY(ctx, hipGetLastError());
     PMMC_HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
     // the chunk's host arrays stay in use until its copies ran
@@ -307,8 +332,15 @@ extern "C" pmmc_status pmmc_singleton_heatmap(pmmc_ctx *ctx, const pmmc_singleto
   if (h_counts[3] != 0)
     return fail(ctx, PMMC_ERR_INVALID, "singleton: a record position lies outside every bin");
   // ---- host: the reference's float steps, in its order
-  std::vector<float> h(nn);
-  for (size_t k = 0; k < nn; ++k) h[k] = static_cast<float>(static_cast<int64_t>(cells[k]));
+  // the device summed each pair into its upper-triangle cell; both cells of
+  // the reference's symmetric sums get it
+  std::vector<float> h(nn, 0.0f);
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      const float v = static_cast<float>(static_cast<int64_t>(cells[static_cast<size_t>(i) * n + j]));
+      h[static_cast<size_t>(i) * n + j] = v;
+      h[static_cast<size_t>(j) * n + i] = v;
+    }
   // long-range arcs, chromosome by chromosome (LooperSolver.hpp:2701-2737)
   for (int c = 0; c < n_chr; ++c)
     for (int k = 0; k < in->n_long; ++k) {
