@@ -5,8 +5,9 @@ chain counts (the last 64-lane ensemble partly empty), a bead with more arc
 entries than one fetched chunk (the pipelined arc kernel loads further
 chunks at evaluation), smooth chains of 3-5 beads (the 5-bead window clipped
 at both ends), heatmap sizes at the kernels' boundaries (2, 33 = first
-register-resident size, 65 and 129 = weight rows past 64), and degenerate
-problems (every bead fixed, a single bead, no beads)."""
+register-resident size, 65 and 129 = weight rows past 64), degenerate
+problems (every bead fixed, a single bead, no beads) and the workloads'
+maximum sizes."""
 import numpy as np
 import pytest
 
@@ -170,3 +171,23 @@ def test_probe_move_pattern(ctx):
                 (1000, 256, 64, 7, 1, 1)]:
         with pytest.raises(PmmcError):
             ctx.probe_move_pattern(*bad)
+
+
+@pytest.mark.parametrize("case", ["arcs_2000", "smooth_19951", "heatmap_512"])
+def test_maximum_sizes(ctx, case):
+    # the largest problems of the workloads: 2,000-anchor arc segments (the
+    # batched problems' upper end), the 19,951-bead smooth segment, 512-bead
+    # heatmaps (the served mix's largest)
+    if case == "arcs_2000":
+        prob, level, step = synth.arcs_problem(n=2000, n_arcs=6000, seed=9), 2, 0.4
+    elif case == "smooth_19951":
+        prob, level, step = synth.config_case(3, 0)[0], 3, 0.5
+    else:
+        prob, level, step = synth.heatmap_problem(n=512, seed=9)[0], 1, 1.0
+    print(f"maximum size {case}", flush=True)
+    sched = _sched(level, 1, step, max_moves=4096)
+    pos, st, res, best = _run_pair(ctx, prob, sched)
+    if level == 3:
+        _assert_smooth_traj(prob, pos, st, res, best)
+    else:
+        _assert_traj(prob, pos, st, res, best)
