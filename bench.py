@@ -129,7 +129,7 @@ def cpu_reference(level, prob, step, settings_dict, seconds, sample_moves=0):
         sd.update(steps=sample_moves, improvement=0.0, min_successes=2**31 - 1)
     moves, secs, runs = R.time_anneal(level, prob, step, sd, seconds, threads)
     return {"value": moves / secs, "unit": "moves/s", "cores": threads, "kind": "reference",
-            "runs": runs, "seconds": secs,
+            "per_core": moves / secs / threads, "runs": runs, "seconds": secs,
             "sample": (f"{runs} runs of {sample_moves} moves" if sample_moves > 0 else
                        f"{runs} complete anneals") + f" on {threads} host threads"}
 
@@ -161,6 +161,7 @@ def cpu_baseline(prob, s, step, budget_s=15.0):
     _, stats, secs = P.anneal_many(sched, 1, 0, 0, chains, threads)
     moves = sum(x["moves"] for x in stats)
     return {"value": moves / secs, "unit": "moves/s", "cores": threads, "kind": "port",
+            "per_core": moves / secs / threads,
             "sample": f"{chains} chains x {moves_per_chain} moves of the oracle's restatement "
                       f"of LooperSolver::MonteCarloArcs (dense O(n) local scans), "
                       f"{secs:.1f} s on {threads} host threads"}
