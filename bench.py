@@ -584,7 +584,7 @@ def main(argv=None):
         "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "plain arc-level SA, one 2 Mb segment: 400 anchors, 1200 "
-                               "PET-cluster arcs (configs[1]), GPU-kernel schedule of "
+                               "PET-cluster arcs, GPU-kernel schedule of "
                                "default.ini [simulation_arcs]",
                    "n_anchors": 400, "n_arcs": int(len(arcs)), "chains_per_solve": chains,
                    "step_size": step},

@@ -6,7 +6,9 @@
 // headers (g++, no CUDA), and calls libpmmc_hip.so through include/pmmc_hip.h.
 // Each member flattens the solver state exactly as the .cu host code did,
 // calls the C ABI, writes the positions back and returns the value the
-// reference returned.  See INTEGRATION.md for the build lines.
+// reference returned.  Build it from the reference's tree with
+//   g++ -std=c++17 -O2 -Isrc/include -Isrc/thirdparty -I<repo>/include -c reference_glue.cpp
+// in place of the four .cu files, and link libpmmc_hip.so.
 #include <LooperSolver.hpp>
 
 #include <ctime>
