@@ -104,7 +104,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     if constexpr (ASYM) lw2[i] = in ? P.W2[i] : 0.f;
   }
   __syncthreads();
-  float *g = it.pos + lane;
+  // global, not generic, address space: flat loads and stores would also
+  // count on lgkmcnt, so every wait for a weight-row LDS read would wait for
+  // the memory copy's traffic too
+  float PMMC_AS_GLOBAL *g = (float PMMC_AS_GLOBAL *)(it.pos) + lane;
   float X[NB], Y[NB], Z[NB];
 #pragma unroll
   for (int j = 0; j < NB; ++j) {

@@ -37,10 +37,13 @@ def distance_heatmap(freq: np.ndarray, diagonal: int = 1, scale: float = 25.0,
     iu = np.triu_indices(n)
     v = f[iu]
     band = np.abs(iu[0] - iu[1]) < diagonal
-    val = np.where(v < 1e-6, np.float32(0.0),
-                   np.where(band, np.float32(-1.0),
-                            (np.float64(scale) * np.power(v.astype(np.float64), power))
-                            .astype(np.float32)))
+    # np.where evaluates both branches: zero counts give inf in the power
+    # branch, which the first condition masks
+    with np.errstate(divide="ignore"):
+        val = np.where(v < 1e-6, np.float32(0.0),
+                       np.where(band, np.float32(-1.0),
+                                (np.float64(scale) * np.power(v.astype(np.float64), power))
+                                .astype(np.float32)))
     h[iu] = val
     h.T[iu] = val
     # getAvg: from the first diagonal holding a value > 1e-6
