@@ -175,7 +175,7 @@ def heatmap_leg(ctx, with_cpu=True, cpu_seconds=8.0):
     from pmmc_amd import metrics, schedule_preset, settings, synth
     prob, step, truth = synth.config_case(1, 0)
     sched = schedule_preset(1, 0, settings(**synth.CONFIG_SETTINGS[1]), step)
-    chains = 131072  # one round of the register-resident kernel (8 ensembles per CU)
+    chains = 98304  # one round of the lane-pair register kernel (6 ensembles per CU)
     warm = schedule_preset(1, 0, settings(**synth.CONFIG_SETTINGS[1]), step)
     warm.max_moves = 1024
     ctx.anneal(prob, warm, seed=1, n_chains=chains)
@@ -188,7 +188,8 @@ def heatmap_leg(ctx, with_cpu=True, cpu_seconds=8.0):
          "kernel_moves_per_s": st["moves"] / (st["kernel_ms"] * 1e-3), "solve_s": wall,
          "final_energy": st["energy_resummed"],
          "partner_evaluations_per_s": 2 * 63 * st["moves"] / (st["kernel_ms"] * 1e-3),
-         "bound": "issue latency (positions of 33-64 beads held in 192 VGPRs: 2 waves per SIMD; "
+         "bound": "issue latency (a chain's 33-64 bead positions held by a lane pair, 96 VGPRs "
+                  "each: 3 waves per SIMD; "
                   "12 B read + 12 B x acceptance written per chain-move)"}
     h.update(metrics.compare(pos, truth))
     if with_cpu:

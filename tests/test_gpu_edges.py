@@ -96,6 +96,18 @@ def test_ragged_chain_counts(ctx, level, n_chains):
         _assert_traj(prob, pos, st, res, best)
 
 
+@pytest.mark.parametrize("n_chains", [100, 20])
+def test_heatmap_lane_pair_ragged(ctx, n_chains):
+    # 48 beads run on the lane-pair register kernel (one 128-lane workgroup
+    # per 64-chain ensemble); at 20 chains the workgroup's second wave holds
+    # only padding lanes and leaves after the weight-table barrier
+    prob, _ = synth.heatmap_problem(n=48, seed=4)
+    print(f"heatmap lane-pair ragged n_chains={n_chains}", flush=True)
+    pos, st, res, best = _run_pair(ctx, prob, _sched(1, 1, 1.0), n_chains=n_chains)
+    assert st["n_chains"] == n_chains
+    _assert_traj(prob, pos, st, res, best)
+
+
 def _hub_arcs(n=80, hub=5, degree=40, seed=8):
     rng = np.random.default_rng(seed)
     pos = (synth.random_walk(n, rng, step=0.5)

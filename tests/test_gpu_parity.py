@@ -107,8 +107,9 @@ def _assert_traj(pos, st, res, best, total_moves_rel=0.0):
     assert st["moves"] == sum(r[1]["moves"] for r in res)
 
 
-# 33 <= n <= 64 runs the register-resident kernel (symmetric and asymmetric
-# weight tables); n <= 32 and n = 100 the partner loop over memory.
+# 33 <= n <= 64 runs the register-resident kernels (symmetric weight tables:
+# the lane-pair form; asymmetric: one lane per chain); n <= 32 and n = 100
+# the partner loop over memory.
 @pytest.mark.parametrize("n,asym,radius,use2d,preset", [
     (32, False, 0.0, 0, 0), (32, False, 0.0, 0, 1), (64, False, 0.0, 0, 1), (50, True, 0.0, 0, 1),
     (40, False, 6.0, 1, 1), (20, True, 5.0, 0, 0), (100, False, 0.0, 0, 1)])
