@@ -41,40 +41,46 @@ __device__ __forceinline__ float hreg_delta(const float *lw, const float *lw2, c
                                             float px, float py, float pz, float nx, float ny,
                                             float nz) {
   const float4 *w1 = reinterpret_cast<const float4 *>(lw + p * NB);
-  float acc0 = 0.f, acc1 = 0.f;
+  const float4 *w2 = reinterpret_cast<const float4 *>(lw2 + p * NB);
+  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
+  // One pass: both distances of a partner serve the W terms (acc0 / acc1 by
+  // partner parity) and, for asymmetric tables, the W2 term (acc2, every
+  // partner in increasing order); each sum keeps its own order.
 #pragma unroll
   for (int q = 0; q < NB; q += 4) {
     const float4 w = w1[q / 4];
     const float wq[4] = {w.x, w.y, w.z, w.w};
+    float wq2[4] = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (ASYM) {
+      const float4 v = w2[q / 4];
+      wq2[0] = v.x;
+      wq2[1] = v.y;
+      wq2[2] = v.z;
+      wq2[3] = v.w;
+    }
 #pragma unroll
     for (int u = 0; u < 4; u += 2) {
       const int a = q + u, b = q + u + 1;
-      const float a0 = fmaf(dist3(px, py, pz, X[a], Y[a], Z[a]), wq[u], -1.f);
-      const float a1 = fmaf(dist3(nx, ny, nz, X[a], Y[a], Z[a]), wq[u], -1.f);
-      const float b0 = fmaf(dist3(px, py, pz, X[b], Y[b], Z[b]), wq[u + 1], -1.f);
-      const float b1 = fmaf(dist3(nx, ny, nz, X[b], Y[b], Z[b]), wq[u + 1], -1.f);
+      const float da0 = dist3(px, py, pz, X[a], Y[a], Z[a]);
+      const float da1 = dist3(nx, ny, nz, X[a], Y[a], Z[a]);
+      const float db0 = dist3(px, py, pz, X[b], Y[b], Z[b]);
+      const float db1 = dist3(nx, ny, nz, X[b], Y[b], Z[b]);
+      const float a0 = fmaf(da0, wq[u], -1.f), a1 = fmaf(da1, wq[u], -1.f);
+      const float b0 = fmaf(db0, wq[u + 1], -1.f), b1 = fmaf(db1, wq[u + 1], -1.f);
       acc0 = fmaf(a1 - a0, a1 + a0, acc0);
       acc1 = fmaf(b1 - b0, b1 + b0, acc1);
+      if constexpr (ASYM) {
+        const float c0 = fmaf(da0, wq2[u], -1.f), c1 = fmaf(da1, wq2[u], -1.f);
+        acc2 = fmaf(c1 - c0, c1 + c0, acc2);
+        const float e0 = fmaf(db0, wq2[u + 1], -1.f), e1 = fmaf(db1, wq2[u + 1], -1.f);
+        acc2 = fmaf(e1 - e0, e1 + e0, acc2);
+      }
     }
   }
   const float d = acc0 + acc1;
   if constexpr (!ASYM) {
     return 2.f * d;
   } else {
-    const float4 *w2 = reinterpret_cast<const float4 *>(lw2 + p * NB);
-    float acc2 = 0.f;
-#pragma unroll
-    for (int q = 0; q < NB; q += 4) {
-      const float4 w = w2[q / 4];
-      const float wq[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int a = q + u;
-        const float a0 = fmaf(dist3(px, py, pz, X[a], Y[a], Z[a]), wq[u], -1.f);
-        const float a1 = fmaf(dist3(nx, ny, nz, X[a], Y[a], Z[a]), wq[u], -1.f);
-        acc2 = fmaf(a1 - a0, a1 + a0, acc2);
-      }
-    }
     return d + acc2;
   }
 }
