@@ -282,7 +282,9 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
   const DevEns it = ens[blockIdx.x];
   const DevProblem &P = probs[it.problem];
   const int n = P.n;
-  // the weight table first: no wave leaves before the barrier
+  // the weight table first: no wave leaves before the barrier.  W rows are
+  // 64 floats (P.ldw == 64: hreg_halves admits only such problems), zero
+  // past column n; rows n..63 of the LDS table are zero.
   for (int i = threadIdx.x; i < 64 * kPairRow; i += 128) {
     const int row = i / kPairRow, c = i % kPairRow;
     const int col = c < 32 ? 2 * c : (c >= kPairOdd ? 2 * (c - kPairOdd) + 1 : -1);
