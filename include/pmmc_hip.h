@@ -212,6 +212,16 @@ int pmmc_ctx_cu_count(const pmmc_ctx *ctx);
 /* setGpuSeed: default seed used when a run passes seed == 0. */
 void pmmc_set_gpu_seed(pmmc_ctx *ctx, uint64_t seed);
 
+/* Chains an annealing call runs for a problem of `level` with n beads when
+ * run.n_chains = 0 (no reference counterpart: the reference sizes its launch
+ * from the SM count, ParallelMonteCarloArcs.cu:700-751): whole rounds of the
+ * level kernel's resident wavefronts (64 chains each) per compute unit, at
+ * least 16 wavefronts per CU, capped so that the chains' positions take at
+ * most a third of the device's HBM.  symmetric (heatmap level): whether the
+ * heatmap's weight tables are symmetric, which selects the kernel for 33..64
+ * beads.  Returns 0 for a null context, an unknown level or n < 1. */
+int32_t pmmc_default_chains(const pmmc_ctx *ctx, int32_t level, int32_t n, int32_t symmetric);
+
 /* ---------------- plain (one problem per call) ---------------- */
 /* Anneal n_chains chains from problem->pos, write the positions of the chain
  * with the lowest final running energy (ties: lowest chain id) to pos_out

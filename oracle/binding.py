@@ -106,7 +106,10 @@ def load():
                                       C.POINTER(OrcStats), C.c_int, C.c_int, C.c_double]
         L.orc_heatmap_sweep.restype = C.c_int
         L.orc_heatmap_sweep.argtypes = [C.POINTER(OrcHeatmap), C.POINTER(OrcSweep), C.c_uint64,
-                                        C.c_uint32, f32p, C.POINTER(OrcStats)]
+                                        C.c_uint32, f32p, C.POINTER(OrcStats), C.c_double, u8p,
+                                        C.POINTER(C.c_double)]
+        L.orc_sweep_argmin.restype = C.c_int
+        L.orc_sweep_argmin.argtypes = [C.POINTER(C.c_double), C.c_int]
         L.orc_mark_arcs.argtypes = [i32p, i32p, C.c_int32, i32p, i32p, C.c_int32, i32p, i32p]
         L.orc_group_arcs.restype = C.c_int
         L.orc_group_arcs.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, i32p, i32p, i32p, i32p,
@@ -222,6 +225,29 @@ class Problem:
                                  n_chains, pick_shift, _p(self.pos0, f32p), _p(out, f32p), stats,
                                  n_threads, full_rescore, float(tie_rel))
         return out.reshape(n_chains, self.n, 3), [s.as_dict() for s in stats], secs
+
+
+def sweep(prob: "Problem", sched: OrcSweep, seed, segment, pos=None, tie_rel=0.0):
+    """orc_heatmap_sweep: returns (positions [n,3], stats dict with
+    'tie_beads' [n] bool and 'e_trace' [iterations + 1])."""
+    L = load()
+    x = (prob.pos0 if pos is None else np.ascontiguousarray(pos, np.float32).reshape(-1)).copy()
+    st = OrcStats()
+    tb = np.zeros(prob.n, np.uint8)
+    tr = np.zeros(int(sched.max_iters) + 1, np.float64)
+    it = L.orc_heatmap_sweep(C.byref(prob.c), C.byref(sched), seed, segment, _p(x, f32p),
+                             C.byref(st), float(tie_rel), _p(tb, u8p),
+                             tr.ctypes.data_as(C.POINTER(C.c_double)))
+    d = st.as_dict()
+    d["tie_beads"] = tb.astype(bool)
+    d["e_trace"] = tr[:it + 1].copy()
+    return x.reshape(-1, 3), d
+
+
+def sweep_argmin(scores):
+    L = load()
+    a = np.ascontiguousarray(scores, np.float64)
+    return L.orc_sweep_argmin(a.ctypes.data_as(C.POINTER(C.c_double)), a.size)
 
 
 def schedule_from(s) -> OrcSchedule:

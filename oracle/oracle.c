@@ -568,10 +568,25 @@ double orc_anneal_many(int level, const void *problem, const orc_schedule *s,
 /* heatmap sweep: the reference GPU heatmap kernel's algorithm          */
 /* (ParallelMonteCarloHeatmap.cu:217-363) with races replaced by a     */
 /* synchronous (Jacobi) sweep over a snapshot of the positions.        */
-/* PINNED BY: parity unpinned against the reference GPU kernel (not    */
-/* runnable here); internally checked by the invariants in             */
-/* test_oracle.py::test_sweep_* (never increases a bead's local score). */
+/* PINNED BY (tests/test_oracle_sweep.py): an independent Python       */
+/* enumeration of the sweep's definition on tiny cases (n <= 4,        */
+/* proposers <= 2, inner <= 4, several iterations, fixed beads,        */
+/* sphere); the invariants the definition fixes (a bead moves only to  */
+/* a point of strictly lower snapshot-local score, fixed beads never   */
+/* move, at T0 = 0 no proposer accepts an uphill move); the argmin's   */
+/* tie rule (orc_sweep_argmin, hand cases); the stop rule's truth      */
+/* table over the energy trace (misses never reset).  Parity against   */
+/* the reference GPU kernel itself stays unpinned (CUDA, racy).        */
 /* ================================================================== */
+
+/* lowest score, ties to the lowest index (proposer index = replica * 64 +
+ * lane, so this is "lowest lane, then lowest replica") */
+int orc_sweep_argmin(const double *sc, int n) {
+  int b = 0;
+  for (int r = 1; r < n; ++r)
+    if (sc[r] < sc[b]) b = r;
+  return b;
+}
 
 /* local score with the GPU skip threshold, bead b at x, against snapshot */
 static double sweep_local(const orc_heatmap *P, const float *snap, int b, const float *x) {
@@ -587,13 +602,25 @@ static double sweep_local(const orc_heatmap *P, const float *snap, int b, const 
   return err;
 }
 
+/* near-tie bound of a comparison between two FP32-evaluated scores */
+static inline int near(double a, double b, double rel) {
+  return fabs(a - b) <= rel * (fabs(a) + fabs(b)) + ORC_TIE_ABS;
+}
+
 int orc_heatmap_sweep(const orc_heatmap *P, const orc_sweep_schedule *s, uint64_t seed,
-                      uint32_t segment, float *pos, orc_stats *st) {
+                      uint32_t segment, float *pos, orc_stats *st, double tie_rel,
+                      uint8_t *tie_beads, double *e_trace) {
   int n = P->n;
   memset(st, 0, sizeof(*st));
-  float *snap = (float *)malloc(sizeof(float) * 3 * (size_t)n);
-  double *sprev = (double *)malloc(sizeof(double) * (size_t)n);
+  st->first_near_tie = -1;
+  if (tie_beads) memset(tie_beads, 0, (size_t)n);
+  float *snap = (float *)malloc(sizeof(float) * 3 * (size_t)(n > 0 ? n : 1));
+  double *sprev = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  double *psc = (double *)malloc(sizeof(double) * (size_t)(s->proposers > 0 ? s->proposers : 1));
+  float *px = (float *)malloc(sizeof(float) * 3 * (size_t)(s->proposers > 0 ? s->proposers : 1));
+  uint8_t *itie = (uint8_t *)malloc((size_t)(n > 0 ? n : 1));
   double E = orc_heatmap_full(P, pos);
+  if (e_trace) e_trace[0] = E;
   for (int b = 0; b < n; ++b) sprev[b] = E; /* first comparison against the total (Q3) */
   double T = s->T0, ms = E;
   float step = s->step;
@@ -603,14 +630,13 @@ int orc_heatmap_sweep(const orc_heatmap *P, const orc_sweep_schedule *s, uint64_
   while (reason == 0) {
     if (it >= s->max_iters) { reason = 3; break; }
     memcpy(snap, pos, sizeof(float) * 3 * (size_t)n);
+    memset(itie, 0, (size_t)n);
+    int any_tie = 0;
     for (int b = 0; b < n; ++b) {
       if (P->fixed && P->fixed[b]) continue;
-      double best_sc = INFINITY;
-      float best_x[3] = {snap[3 * b], snap[3 * b + 1], snap[3 * b + 2]};
       for (int r = 0; r < s->proposers; ++r) {
         float x[3] = {snap[3 * b], snap[3 * b + 1], snap[3 * b + 2]};
         double sp = sprev[b];
-        double sc = sp;
         for (int k = 0; k < s->inner; ++k) {
           uint32_t r0[4];
           uint32_t mv = (uint32_t)it * (uint32_t)s->inner + (uint32_t)k;
@@ -621,42 +647,71 @@ int orc_heatmap_sweep(const orc_heatmap *P, const orc_sweep_schedule *s, uint64_
           d[1] = (2.0f * u01(r0[1]) - 1.0f) * step;
           d[2] = s->use2d ? 0.0f : (2.0f * u01(r0[2]) - 1.0f) * step;
           float y[3] = {x[0] + d[0], x[1] + d[1], x[2] + d[2]};
+          if (s->use2d) y[2] = x[2];
           double sn = sweep_local(P, snap, b, y) + orc_sphere_penalty(y, s->sphere_radius);
           int ok = sn <= sp;
+          int tie = tie_rel > 0.0 && near(sn, sp, tie_rel);
           if (!ok && T > 0.0) {
-            double tp = s->tjs * exp(-s->tjc * (sn / sp) / T);
-            ok = (double)u01(r0[3]) < tp;
+            const double arg = s->tjc * (sn / sp) / T;
+            double tp = s->tjs * exp(-arg);
+            const double U = (double)u01(r0[3]);
+            ok = U < tp;
+            /* the FP32 ratio carries both scores' errors */
+            if (tie_rel > 0.0) tie = tie || fabs(U - tp) <= tie_rel * (1.0 + 2.0 * fabs(arg)) * tp;
           }
+          if (tie) itie[b] = 1;
           ++moves;
           if (ok) {
             x[0] = y[0]; x[1] = y[1]; x[2] = y[2];
             sp = sn;
             ++acc;
           }
-          sc = sp;
         }
-        if (sc < best_sc) { /* ties: lowest proposer index wins */
-          best_sc = sc;
-          best_x[0] = x[0]; best_x[1] = x[1]; best_x[2] = x[2];
-        }
+        psc[r] = sp;
+        px[3 * r] = x[0]; px[3 * r + 1] = x[1]; px[3 * r + 2] = x[2];
       }
+      const int br = orc_sweep_argmin(psc, s->proposers);
+      if (tie_rel > 0.0)
+        for (int r = 0; r < s->proposers; ++r)
+          if (r != br && near(psc[r], psc[br], tie_rel)) itie[b] = 1;
+      const float *best_x = px + 3 * br;
       double lc = sweep_local(P, snap, b, best_x);
       double lg = sweep_local(P, snap, b, snap + 3 * b);
+      if (tie_rel > 0.0 && fabs(lc - lg) <= 1e-12 * (lc + lg)) itie[b] = 1;
       if (lc < lg) { pos[3 * b] = best_x[0]; pos[3 * b + 1] = best_x[1]; pos[3 * b + 2] = best_x[2]; }
-      sprev[b] = best_sc;
+      sprev[b] = psc[br];
+      any_tie |= itie[b];
     }
     T *= s->dt;
     step *= s->step_decay;
     ++it;
     E = orc_heatmap_full(P, pos);
+    if (e_trace) e_trace[it] = E;
+    /* the stop test compares FP64 sums formed in another order */
+    const int stop_tie = tie_rel > 0.0 &&
+                         (fabs(E - s->improvement * ms) <= 1e-12 * fabs(E) ||
+                          fabs(E - s->small_energy) <= 1e-12 * fabs(E));
+    if (any_tie || stop_tie) {
+      ++st->near_ties;
+      if (st->first_near_tie < 0) {
+        st->first_near_tie = it - 1;
+        if (tie_beads) {
+          if (stop_tie) memset(tie_beads, 1, (size_t)n);
+          else memcpy(tie_beads, itie, (size_t)n);
+        }
+      }
+    }
     if (!isfinite(E)) { reason = 6; break; }
-    if (E > s->improvement * ms) ++misses;
+    if (E > s->improvement * ms) ++misses;   /* never reset (Q4) */
     if (E < s->small_energy) { reason = 2; break; }
     if (misses >= s->fails_threshold) { reason = 1; break; }
     ms = E;
   }
   free(snap);
   free(sprev);
+  free(psc);
+  free(px);
+  free(itie);
   st->energy = E;
   st->best_energy = E;
   st->T = T;

@@ -197,8 +197,20 @@ typedef struct {
   int32_t use2d;
 } orc_sweep_schedule;
 
+/* tie_rel > 0 counts near ties: decisions that an FP32 evaluation of the
+ * scores with relative error tie_rel could take the other way (a proposal's
+ * downhill or Metropolis test, the argmin over proposers), and FP64 ties of
+ * the candidate-vs-snapshot test and of the stop test.  st->near_ties counts
+ * iterations with a near tie, st->first_near_tie is the first one (0-based,
+ * -1 if none); tie_beads [n] (nullable) flags the beads with a near tie in
+ * that iteration (every bead for a stop-test tie).  e_trace (nullable,
+ * max_iters + 1 entries): the full score before the first iteration and
+ * after each one.  Returns the number of iterations. */
 int orc_heatmap_sweep(const orc_heatmap *P, const orc_sweep_schedule *s, uint64_t seed,
-                      uint32_t segment, float *pos, orc_stats *st);
+                      uint32_t segment, float *pos, orc_stats *st, double tie_rel,
+                      uint8_t *tie_beads, double *e_trace);
+/* index of the lowest of n scores, ties to the lowest index */
+int orc_sweep_argmin(const double *scores, int n);
 
 /* ================= arc marking ================= */
 /* ParallelMarkArcs.cu:26-64 / InteractionArcs.hpp:226-235: last anchor j with

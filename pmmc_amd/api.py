@@ -242,6 +242,14 @@ class Context:
     def set_gpu_seed(self, seed: int):
         self.lib.pmmc_set_gpu_seed(self.h, seed)
 
+    def default_chains(self, level: int, n: int, symmetric: bool = True) -> int:
+        """pmmc_default_chains: chains a call with n_chains = 0 runs."""
+        return int(self.lib.pmmc_default_chains(self.h, int(level), int(n), int(bool(symmetric))))
+
+    def reserve(self, nbytes: int):
+        """pmmc_ctx_reserve: grow the context's device arena to nbytes."""
+        self._check(self.lib.pmmc_ctx_reserve(self.h, C.c_size_t(int(nbytes))))
+
     # ---- plain ----
     def anneal(self, prob, sched: A.Schedule, seed=1, segment=0, chain0=0, n_chains=0):
         """Anneal one problem; returns (positions [n,3], stats dict)."""

@@ -64,6 +64,18 @@ extern "C" void pmmc_set_gpu_seed(pmmc_ctx *c, uint64_t seed) {
   if (c) c->default_seed = seed;
 }
 
+extern "C" int32_t pmmc_default_chains(const pmmc_ctx *c, int32_t level, int32_t n,
+                                       int32_t symmetric) {
+  if (!c || n < 1 || level < PMMC_LEVEL_HEATMAP || level > PMMC_LEVEL_SMOOTH) return 0;
+  HostProblem h;
+  h.level = level;
+  h.n = n;
+  h.symmetric = symmetric != 0;
+  // the row stride prep_heatmap gives the weight tables (selects the kernel)
+  h.ldw = level == PMMC_LEVEL_HEATMAP ? (n <= 32 ? 32 : (n <= 64 ? 64 : n)) : 0;
+  return default_chains(c, h);
+}
+
 // ------------------------------------------------------------------ //
 // schedule presets: the reference annealers as pmmc_schedule
 // ------------------------------------------------------------------ //
@@ -219,21 +231,14 @@ static pmmc_status prep_heatmap(pmmc_ctx *ctx, const pmmc_heatmap_problem *p, Ho
     if (std::abs(i - m) < p->diagonal) return false;
     return p->dist[static_cast<size_t>(i) * n + m] >= thr;
   };
-  h.symmetric = true;
-  for (int i = 0; i < n && h.symmetric; ++i)
-    for (int j = i + 1; j < n; ++j)
-      if (std::memcmp(&p->dist[static_cast<size_t>(i) * n + j], &p->dist[static_cast<size_t>(j) * n + i],
-                      sizeof(float)) != 0) {
-        h.symmetric = false;
-        break;
-      }
   // rows padded with zeros to 32 / 64 entries for small heatmaps, so the
   // register-resident kernel reads whole rows with wide scalar loads
   h.ldw = n <= 32 ? 32 : (n <= 64 ? 64 : n);
   const size_t nl = static_cast<size_t>(n) * h.ldw;
   h.W.assign(nl, 0.f);
   h.Hc.assign(nl, 0.f);
-  if (!h.symmetric) h.W2.assign(nl, 0.f);
+  std::vector<float> Hc2(nl, 0.f);
+  h.W2.assign(nl, 0.f);
   for (int q = 0; q < n; ++q)
     for (int j = 0; j < n; ++j) {
       if (valid(j, q)) {
@@ -241,9 +246,21 @@ static pmmc_status prep_heatmap(pmmc_ctx *ctx, const pmmc_heatmap_problem *p, Ho
         h.W[static_cast<size_t>(q) * h.ldw + j] = 1.0f / hv;
         h.Hc[static_cast<size_t>(q) * h.ldw + j] = hv;
       }
-      if (!h.symmetric && valid(q, j))
-        h.W2[static_cast<size_t>(q) * h.ldw + j] = 1.0f / p->dist[static_cast<size_t>(q) * n + j];
+      if (valid(q, j)) {
+        const float hv = p->dist[static_cast<size_t>(q) * n + j];
+        h.W2[static_cast<size_t>(q) * h.ldw + j] = 1.0f / hv;
+        Hc2[static_cast<size_t>(q) * h.ldw + j] = hv;
+      }
     }
+  // The kernels' symmetric form (ΔE = 2 * the row-p terms, re-sums over i < j
+  // doubled) is exact only when the term of partner j in local(p) equals the
+  // term of p in local(j) for every pair.  That depends on the targets AND
+  // on the validity rule (diagonal band, skip threshold, chromosome blocks:
+  // boundaries that do not partition [0, n) give some beads the whole range
+  // and others one block), so it is decided on the built tables.
+  h.symmetric = std::memcmp(h.W.data(), h.W2.data(), sizeof(float) * nl) == 0 &&
+                std::memcmp(h.Hc.data(), Hc2.data(), sizeof(float) * nl) == 0;
+  if (h.symmetric) std::vector<float>().swap(h.W2);
   h.pos0.assign(p->pos, p->pos + 3 * static_cast<size_t>(n));
   h.fixed.assign(n, 0);
   if (p->fixed) h.fixed.assign(p->fixed, p->fixed + n);

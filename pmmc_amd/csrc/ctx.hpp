@@ -55,6 +55,7 @@ inline pmmc_status arena_reserve(pmmc_ctx *c, size_t bytes) {
   size_t cap = bytes + (bytes >> 3) + (1u << 20);
   if (hipMalloc(&c->arena, cap) != hipSuccess) {
     c->arena = nullptr;
+    (void)hipGetLastError();  // a later launch check must not see this failure
     return fail(c, PMMC_ERR_NOMEM, "device arena allocation failed");
   }
   c->arena_cap = cap;

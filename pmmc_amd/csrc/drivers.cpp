@@ -83,15 +83,14 @@ struct StageJob {
   pmmc_job job{};
 };
 
-// chains per job: the library's default pool (16 wavefronts per compute
-// unit) shared among the jobs, capped at 32 GiB of chain positions
-int32_t chains_for(pmmc_ctx *ctx, int32_t requested, size_t n_jobs, int32_t max_n) {
+// chains per job: the library's default pool for the stage's largest
+// problem (pmmc_default_chains: whole rounds of the kernel's resident
+// wavefronts, at least 16 per compute unit, positions within a third of the
+// device's HBM) shared among the stage's jobs, at least one wavefront each
+int32_t chains_for(pmmc_ctx *ctx, int32_t requested, size_t n_jobs, int32_t level, int32_t max_n) {
   if (requested > 0) return requested;
-  const int cus = pmmc_ctx_cu_count(ctx) > 0 ? pmmc_ctx_cu_count(ctx) : 256;
-  size_t ens = (static_cast<size_t>(16) * cus) / (n_jobs ? n_jobs : 1);
-  const size_t per_ens = static_cast<size_t>(max_n > 0 ? max_n : 1) * 3 * 64 * sizeof(float);
-  const size_t cap = ((size_t(32) << 30) / per_ens) / (n_jobs ? n_jobs : 1);
-  if (ens > cap) ens = cap;
+  const int32_t pool = pmmc_default_chains(ctx, level, max_n > 0 ? max_n : 1, 1);
+  size_t ens = (static_cast<size_t>(pool > 0 ? pool : 64) / 64) / (n_jobs ? n_jobs : 1);
   if (ens < 1) ens = 1;
   return static_cast<int32_t>(ens) * 64;
 }
@@ -202,7 +201,7 @@ extern "C" pmmc_status pmmc_arcs_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t chr,
   size_t n_jobs_a = 0;
   for (const Block &b : blocks)
     if (t->cl[b.cluster].children.size() > 1) n_jobs_a += (size_t)p->restarts_anchor;
-  const int32_t chains_a = chains_for(ctx, o->chains_per_restart, n_jobs_a, max_n);
+  const int32_t chains_a = chains_for(ctx, o->chains_per_restart, n_jobs_a, PMMC_LEVEL_ARCS, max_n);
   for (size_t bi = 0; bi < blocks.size(); ++bi) {
     const Block &b = blocks[bi];
     const std::vector<int32_t> active = t->cl[b.cluster].children;
@@ -274,7 +273,7 @@ extern "C" pmmc_status pmmc_arcs_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t chr,
     max_n = std::max<int32_t>(max_n, (int32_t)t->cl[b.cluster].children.size());
     if (t->cl[b.cluster].children.size() > 1) n_jobs_s += (size_t)p->restarts_subanchor;
   }
-  const int32_t chains_s = chains_for(ctx, o->chains_per_restart, n_jobs_s, max_n);
+  const int32_t chains_s = chains_for(ctx, o->chains_per_restart, n_jobs_s, PMMC_LEVEL_SMOOTH, max_n);
   for (size_t bi = 0; bi < blocks.size(); ++bi) {
     const Block &b = blocks[bi];
     const std::vector<int32_t> active = t->cl[b.cluster].children;
@@ -379,7 +378,7 @@ extern "C" pmmc_status pmmc_heatmap_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t l
   pmmc_schedule sched{};
   pmmc_status st = pmmc_schedule_preset(PMMC_LEVEL_HEATMAP, o->preset, &o->heatmap, avg_dist, &sched);
   if (st != PMMC_OK) return fail("pmmc_heatmap_level: schedule preset", st);
-  const int32_t chains = chains_for(ctx, o->chains_per_restart, (size_t)restarts, n);
+  const int32_t chains = chains_for(ctx, o->chains_per_restart, (size_t)restarts, PMMC_LEVEL_HEATMAP, n);
   std::vector<std::unique_ptr<StageJob>> jobs;
   for (int32_t k = 0; k < restarts; ++k) {
     auto j = std::make_unique<StageJob>();

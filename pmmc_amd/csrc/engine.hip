@@ -606,10 +606,14 @@ static constexpr int kStageMaxBytes = PMMC_STAGE_MAX_BYTES;
 bool stage_ok(int n) { return static_cast<size_t>(n) * 3 * kLanes * sizeof(float) <= kStageMaxBytes; }
 
 int resident_ensembles_per_cu(const HostProblem &h) {
-  static std::atomic<int> cache[16];
+  // keyed by everything that selects the kernel: level, staging, the
+  // register-resident form and, for it, the table symmetry (the lane-pair
+  // kernel and the one-lane-per-chain kernel differ in occupancy)
+  static std::atomic<int> cache[32];
   const bool hreg = !stage_ok(h.n) && hreg_halves(h) > 0;
   const bool stage = stage_ok(h.n);
-  const int key = (h.level & 3) * 4 + (hreg ? 1 : 0) + (stage ? 2 : 0);
+  const bool asym = hreg && !h.symmetric;
+  const int key = (h.level & 3) * 8 + (hreg ? 1 : 0) + (stage ? 2 : 0) + (asym ? 4 : 0);
   const int known = cache[key].load(std::memory_order_relaxed);
   if (known > 0) return known;
   int blocks = 0;
@@ -953,21 +957,28 @@ pmmc_status run_problems(pmmc_ctx *ctx, std::vector<HostProblem> &hp, std::vecto
     hipError_t err = batch_step(ctx, bd, INT_MAX);
     if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("anneal: ") + hipGetErrorString(err));
   } else {
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
+    // the captured step loop; destroyed on every return path
+    struct GraphGuard {
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      ~GraphGuard() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+      }
+    } gg;
     if (use_graph) {
       if (steps_per_graph <= 0) steps_per_graph = 8;
       PMMC_HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       hipError_t cerr = hipSuccess;
       for (int i = 0; i < steps_per_graph && cerr == hipSuccess; ++i) cerr = batch_step(ctx, bd, blocks_per_launch);
-      hipError_t eerr = hipStreamEndCapture(s, &graph);
+      hipError_t eerr = hipStreamEndCapture(s, &gg.graph);
       if (cerr != hipSuccess || eerr != hipSuccess)
         return fail(ctx, PMMC_ERR_HIP, "graph capture failed");
-      PMMC_HIP_TRY(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      PMMC_HIP_TRY(ctx, hipGraphInstantiate(&gg.exec, gg.graph, nullptr, nullptr, 0));
     }
     for (;;) {
       hipError_t err;
-      if (use_graph) err = hipGraphLaunch(exec, s);
+      if (use_graph) err = hipGraphLaunch(gg.exec, s);
       else err = batch_step(ctx, bd, blocks_per_launch);
       if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("step: ") + hipGetErrorString(err));
       int active = 0;
@@ -975,8 +986,6 @@ pmmc_status run_problems(pmmc_ctx *ctx, std::vector<HostProblem> &hp, std::vecto
       if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("count: ") + hipGetErrorString(err));
       if (active == 0) break;
     }
-    if (exec) hipGraphExecDestroy(exec);
-    if (graph) hipGraphDestroy(graph);
   }
   PMMC_HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
   PMMC_HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));

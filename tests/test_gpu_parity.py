@@ -237,23 +237,52 @@ def test_batch_equals_plain(ctx):
 
 
 # ---------------- heatmap sweep (reference GPU heatmap algorithm) ----------------
-def test_sweep_matches_oracle(ctx):
+# The GPU scores a proposal in FP32 (reciprocal weights, hardware square
+# root, two interleaved accumulators) and keeps each bead's reference score
+# as a float; the oracle scores in FP64.  A decision can differ only when the
+# oracle finds it within TIE_SWEEP (relative) of its threshold: the
+# proposal's downhill or Metropolis test, the argmin over proposers, or (in
+# FP64 on both sides) the candidate-vs-snapshot and stop tests.
+TIE_SWEEP = 2e-6
+
+
+def _sweep_scheds(iters, inner=4, proposers=64):
     from pmmc_amd import _abi as A
-    prob, _ = synth.heatmap_problem(n=24, seed=6, skip_below=1e-3)
-    ss = A.SweepSchedule(5.0, 0.9999, 0.99, 1e-4, 50.0, 20.0, 0.75, 0.95, 0.0, 64, 64, 3, 3, 0)
-    pos, st = ctx.heatmap_sweep(prob, ss, seed=9, segment=1)
+    args = (5.0, 0.9999, 0.99, 1e-4, 50.0, 20.0, 0.75, 0.95, 0.0, inner, proposers, 3, iters, 0)
+    return A.SweepSchedule(*args), O.OrcSweep(*args)
+
+
+@pytest.mark.parametrize("n,seed", [(24, 9), (24, 3), (40, 5)])
+def test_sweep_matches_oracle(ctx, n, seed):
+    prob, _ = synth.heatmap_problem(n=n, seed=6, skip_below=1e-3)
     P = O.Problem(prob)
-    osw = O.OrcSweep(5.0, 0.9999, 0.99, 1e-4, 50.0, 20.0, 0.75, 0.95, 0.0, 64, 64, 3, 3, 0)
-    x = P.pos0.copy()
-    ost = O.OrcStats()
-    O.load().orc_heatmap_sweep(O.C.byref(P.c), O.C.byref(osw), 9, 1, x.ctypes.data_as(O.f32p),
-                               O.C.byref(ost))
-    x = x.reshape(-1, 3)
-    same = np.mean(np.all(pos == x, axis=1))
-    assert same >= 0.9, same
-    assert st["energy"] == pytest.approx(ost.energy, rel=1e-2)
-    assert st["milestones"] == ost.milestones
-    assert st["energy"] < P.total()          # the sweep lowers the full score
+    checked = 0
+    for k in range(1, 6):
+        ss, osw = _sweep_scheds(k)
+        print(f"sweep n={n} seed={seed} iterations={k}", flush=True)
+        pos, st = ctx.heatmap_sweep(prob, ss, seed=seed, segment=1)
+        x, ost = O.sweep(P, osw, seed, 1, tie_rel=TIE_SWEEP)
+        first = ost["first_near_tie"]
+        if first < 0:
+            # no decision of the k iterations was near its threshold: the
+            # runs are the same, bit for bit
+            assert np.array_equal(pos, x), k
+            assert st["milestones"] == ost["milestones"] and st["stop_reason"] == ost["stop_reason"]
+            assert st["energy"] == pytest.approx(ost["energy"], rel=1e-12)
+            checked += n
+        elif first == k - 1:
+            # the first near tie is in the last iteration: only the beads it
+            # touched may differ, every other bead is bit-exact
+            diff = np.any(pos != x, axis=1)
+            assert not np.any(diff & ~ost["tie_beads"]), (k, np.nonzero(diff & ~ost["tie_beads"]))
+            assert ost["tie_beads"].sum() <= n // 4
+            checked += int((~ost["tie_beads"]).sum())
+        else:
+            # an earlier near tie may have changed the snapshot of later
+            # iterations: the runs agree statistically only
+            assert st["energy"] == pytest.approx(ost["energy"], rel=2e-2)
+        assert st["energy"] < P.total()          # the sweep lowers the full score
+    assert checked >= n - n // 4      # most beads compared bit for bit
 
 
 # ---------------- served form ----------------
