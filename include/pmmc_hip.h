@@ -238,6 +238,23 @@ pmmc_status pmmc_smooth_anneal(pmmc_ctx *ctx, const pmmc_smooth_problem *p,
                                const pmmc_schedule *s, const pmmc_run *run,
                                float *pos_out, pmmc_anneal_stats *stats);
 
+/* ---------------- plain, allocation-free ---------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the plain annealing call of `level` (same problem
+ * structs, kernels, results and write-back as pmmc_heatmap_anneal /
+ * pmmc_arcs_anneal / pmmc_smooth_anneal, bit for bit) with every device
+ * buffer carved from the library's static per-device scratch array of
+ * 64 MiB; it calls no hipMalloc, hipFree or other allocating routine.
+ * run->n_chains = 0 takes the library default reduced to the most whole
+ * wavefronts (64 chains) that fit; a problem or an explicit chain count that
+ * needs more than 64 MiB is rejected with PMMC_ERR_NOMEM (the message gives
+ * the bytes needed).  Every call on one GPU shares that GPU's array, so calls
+ * that overlap on one GPU (other threads or contexts) race. */
+pmmc_status pmmc_anneal_noalloc(pmmc_ctx *ctx, int32_t level, const void *problem,
+                                const pmmc_schedule *s, const pmmc_run *run, float *pos_out,
+                                pmmc_anneal_stats *stats);
+
 /* ---------------- heatmap sweep (reference GPU heatmap kernel) ---------------- */
 /* Each iteration: every non-fixed bead b gets `proposers` independent
  * proposers; proposer r runs `inner` Metropolis proposals on bead b against a

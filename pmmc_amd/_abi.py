@@ -206,6 +206,8 @@ SIGNATURES = {
                                    C.POINTER(Run), f32p, C.POINTER(Stats)]),
     "pmmc_smooth_anneal": (C.c_int, [C.c_void_p, C.POINTER(SmoothProblem), C.POINTER(Schedule),
                                      C.POINTER(Run), f32p, C.POINTER(Stats)]),
+    "pmmc_anneal_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(Schedule),
+                                      C.POINTER(Run), f32p, C.POINTER(Stats)]),
     "pmmc_heatmap_sweep": (C.c_int, [C.c_void_p, C.POINTER(HeatmapProblem),
                                      C.POINTER(SweepSchedule), C.c_uint64, C.c_uint32, f32p,
                                      C.POINTER(Stats)]),

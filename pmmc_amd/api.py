@@ -264,6 +264,19 @@ class Context:
                        out.ctypes.data_as(A.f32p), C.byref(stats)))
         return out, stats.as_dict()
 
+    def anneal_noalloc(self, prob, sched: A.Schedule, seed=1, segment=0, chain0=0, n_chains=0):
+        """pmmc_anneal_noalloc: as anneal(), with no device allocation (the
+        library's static 64 MiB scratch per device)."""
+        cp = prob.c()
+        n = cp.n
+        out = _f32(prob.pos).reshape(n, 3).copy()
+        stats = A.Stats()
+        run = A.Run(seed, segment, chain0, n_chains)
+        self._check(self.lib.pmmc_anneal_noalloc(self.h, prob.level, C.cast(C.pointer(cp), C.c_void_p),
+                                                 C.byref(sched), C.byref(run),
+                                                 out.ctypes.data_as(A.f32p), C.byref(stats)))
+        return out, stats.as_dict()
+
     def delta_energy(self, prob, beads, disp, sphere_radius=0.0):
         cp = prob.c()
         b = _i32(beads)

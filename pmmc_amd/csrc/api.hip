@@ -34,6 +34,9 @@ extern "C" pmmc_status pmmc_ctx_create(int device, pmmc_ctx **out) {
     delete c;
     return PMMC_ERR_HIP;
   }
+  // the allocation-free entry point's scratch (device current); a failure
+  // only disables that entry point, which then reports it
+  if (scratch_init(c) != PMMC_OK) c->scratch_error = c->last_error;
   *out = c;
   return PMMC_OK;
 }

@@ -25,6 +25,12 @@ struct pmmc_ctx {
   size_t arena_cap = 0;
   size_t arena_top = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // the device's static scratch array (pmmc_anneal_noalloc): its device
+  // address, looked up when the context is created, or null with the reason
+  // in scratch_error
+  char *scratch = nullptr;
+  size_t scratch_cap = 0;
+  std::string scratch_error;
 };
 
 namespace pmmc {
@@ -61,6 +67,10 @@ inline pmmc_status arena_reserve(pmmc_ctx *c, size_t bytes) {
   c->arena_cap = cap;
   return PMMC_OK;
 }
+
+// Look up the device address of the static scratch array of ctx->device
+// (the device must be current); sets ctx->scratch / scratch_cap.
+pmmc_status scratch_init(pmmc_ctx *ctx);
 
 // Bump allocator over a reserved span (256-byte aligned).
 struct Carver {

@@ -829,7 +829,8 @@ void fill_problem(Carver &cv, hipStream_t s, const HostProblem &h, int index, De
   }
 }
 
-pmmc_status batch_setup(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &bd) {
+// Device bytes batch_setup carves for these problems.
+size_t batch_bytes(const std::vector<HostProblem> &hp) {
   Sizer z;
   z.add<DevProblem>(hp.size());
   z.add<DevResult>(hp.size());
@@ -840,9 +841,27 @@ pmmc_status batch_setup(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &b
     n_ens += (h.n_chains + kLanes - 1) / kLanes;
   }
   z.add<DevEns>(n_ens);
-  pmmc_status ps = arena_reserve(ctx, z.top + 4096);
-  if (ps != PMMC_OK) return ps;
-  Carver cv{ctx->arena, 0, ctx->arena_cap};
+  return z.top + 4096;
+}
+
+// Carves the batch from `span` when it has a base (the caller's device
+// memory, nothing is allocated), else from the context's arena.
+pmmc_status batch_setup(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &bd, DevSpan span) {
+  size_t n_ens = 0;
+  for (auto &h : hp) n_ens += (h.n_chains + kLanes - 1) / kLanes;
+  const size_t need = batch_bytes(hp);
+  if (span.base) {
+    if (need > span.cap)
+      return fail(ctx, PMMC_ERR_NOMEM, "the problem needs " + std::to_string(need) +
+                                           " bytes of device memory; the span holds " +
+                                           std::to_string(span.cap));
+  } else {
+    pmmc_status ps = arena_reserve(ctx, need);
+    if (ps != PMMC_OK) return ps;
+    span.base = ctx->arena;
+    span.cap = ctx->arena_cap;
+  }
+  Carver cv{span.base, 0, span.cap};
   hipStream_t s = ctx->stream;
   hipError_t err = hipSuccess;
   bd.probs = cv.take<DevProblem>(hp.size());
@@ -947,9 +966,9 @@ pmmc_status batch_finish(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &
 // a single launch per level group that runs every chain to its end.
 pmmc_status run_problems(pmmc_ctx *ctx, std::vector<HostProblem> &hp, std::vector<float *> &outs,
                          std::vector<pmmc_anneal_stats *> &stats, int blocks_per_launch,
-                         bool use_graph, int steps_per_graph) {
+                         bool use_graph, int steps_per_graph, DevSpan span) {
   BatchDev bd;
-  pmmc_status ps = batch_setup(ctx, hp, bd);
+  pmmc_status ps = batch_setup(ctx, hp, bd, span);
   if (ps != PMMC_OK) return ps;
   hipStream_t s = ctx->stream;
   PMMC_HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
@@ -1120,7 +1139,7 @@ extern "C" pmmc_status pmmc_delta_energy(pmmc_ctx *ctx, int level, const void *p
   for (int i = 0; i < n_prop; ++i)
     if (beads[i] < 0 || beads[i] >= hp[0].n) return fail(ctx, PMMC_ERR_INVALID, "delta: bead out of range");
   BatchDev bd;
-  ps = batch_setup(ctx, hp, bd);
+  ps = batch_setup(ctx, hp, bd, DevSpan{});
   if (ps != PMMC_OK) return ps;
   hipStream_t st = ctx->stream;
   // proposals live after the batch in a second span of the arena; grow it by

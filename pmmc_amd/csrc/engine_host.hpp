@@ -73,6 +73,21 @@ struct BatchDev {
   int n_ens_total = 0;
 };
 
+// Device memory a batch is carved from: a caller-given span (base non-null;
+// nothing is allocated) or, when base is null, the context's arena.
+struct DevSpan {
+  char *base = nullptr;
+  size_t cap = 0;
+};
+// device bytes a batch of these problems needs
+size_t batch_bytes(const std::vector<HostProblem> &hp);
+// Run prepared problems to completion.  blocks_per_launch <= 0: one launch
+// per launch group runs every chain to its end; else step launches
+// (replayed from a captured hipGraph when use_graph) until every chain stops.
+pmmc_status run_problems(pmmc_ctx *ctx, std::vector<HostProblem> &hp, std::vector<float *> &outs,
+                         std::vector<pmmc_anneal_stats *> &stats, int blocks_per_launch,
+                         bool use_graph, int steps_per_graph, DevSpan span = DevSpan{});
+
 bool stage_ok(int n);
 size_t problem_bytes(const HostProblem &h);
 void fill_problem(Carver &cv, hipStream_t s, const HostProblem &h, int index, DevResult *result,
