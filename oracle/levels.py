@@ -23,6 +23,10 @@ powf/pow/exp are libm's, through ctypes):
 PINNED BY tests/golden/levels_ref.json, written by the reference's own code
 (oracle/_ref/levels_ref), and by the invariants in tests/test_levels.py
 (children partition parents, genomic monotonicity, base ranges, counts).
+The driver loops arcs_level / heatmap_level: the product's drivers match them
+bit for bit (tests/test_gpu_levels.py) and match the reference's own drivers,
+run through the reference-side glue, bit for bit at zero restart noise and
+within a 16-seed band with noise (tests/test_gpu_ref_drivers.py).
 """
 from __future__ import annotations
 
@@ -435,17 +439,45 @@ def restart_noise(seed, segment, bead, amp, use2d):
     return out
 
 
-def _offer(best, b, score, payload):
-    """best-of rule score < best || best < 0 (LooperSolver.hpp:1392-1398, :4457-4463)"""
+def _offer(best, b, score, payload, gpu_member=True):
+    """best-of rule score < best || best < 0 (LooperSolver.hpp:1392-1398,
+    :4457-4463) on the value the annealing member returns: a float from the
+    GPU members, a double from the CPU annealer"""
+    if gpu_member:
+        score = float(np.float32(score))
     cur = best.get(b)
     if cur is None or score < cur[0] or cur[0] < 0.0:
         best[b] = (score, payload)
 
 
+def call_segments(sizes, restarts_anchor, restarts_subanchor, loop_density, segment):
+    """Philox segments of the anchor and sub-anchor restarts of each block, in
+    the order the reference driver issues its GPU calls (LooperSolver.hpp:
+    4027-4138; blocks of at most one child make none; sub-anchor problems
+    below 4 beads run the CPU annealer, :4431-4437, and take segments from
+    2^30 on).  sizes: children per block before densification."""
+    call, cpu, out = 0, 0, []
+    for n in sizes:
+        if n <= 1:
+            out.append((0, 0))
+            continue
+        a = segment + call
+        call += restarts_anchor
+        if n + (n - 1) * loop_density >= 4:
+            sm = segment + call
+            call += restarts_subanchor
+        else:
+            sm = segment + (1 << 30) + cpu
+            cpu += restarts_subanchor
+        out.append((a, sm))
+    return out
+
+
 def arcs_level(t, chr_name, p, o, seed, segment, run_job):
     """reconstructClustersArcsDistances' block loop (LooperSolver.hpp:3978-4140)
     over reconstructClusterArcsDistances (:4303-4487), in the library's
-    batched order: every block's anchor restarts, then densification, then
+    batched order (job segments numbered in the reference's call order):
+    every block's anchor restarts, then densification, then
     every block's sub-anchor restarts.  p: level parameters (dict, the keys of
     pmmc_level_params), o: dict with arcs_sched(step) / smooth_sched(step,
     n_beads) schedule factories, springs, chains and densify.  run_job(level,
@@ -461,7 +493,8 @@ def arcs_level(t, chr_name, p, o, seed, segment, run_job):
             if cl[ib]["level"] == 3:
                 blocks.append((name, ib))
     rep = dict(n_blocks=len(blocks), n_jobs=0, score_anchor=0.0, score_smooth=0.0)
-    next_job = 0
+    segs = call_segments([len(cl[ib]["children"]) for _, ib in blocks], p["restarts_anchor"],
+                         p["restarts_subanchor"], p["loop_density"], segment)
     best = {}
     for bi, (name, ib) in enumerate(blocks):
         active = list(cl[ib]["children"])
@@ -475,9 +508,8 @@ def arcs_level(t, chr_name, p, o, seed, segment, run_job):
         for i, c in enumerate(active[:-1]):
             cl[c]["dist_to_next"] = float(dtn[i])
         sched = o["arcs_sched"](noise)
-        for _ in range(p["restarts_anchor"]):
-            seg = segment + next_job
-            next_job += 1
+        for k in range(p["restarts_anchor"]):
+            seg = segs[bi][0] + k
             pos = np.array([[F(cl[c]["pos"][a] + restart_noise(seed, seg, i, p["noise_anchor_small"],
                                                                p["use2d"])[a]) for a in range(3)]
                             for i, c in enumerate(active)], np.float32)
@@ -505,9 +537,8 @@ def arcs_level(t, chr_name, p, o, seed, segment, run_job):
         for i, c in enumerate(active[:-1]):
             cl[c]["dist_to_next"] = float(dtn[i])
         sched = o["smooth_sched"](noise, len(active))
-        for _ in range(p["restarts_subanchor"]):
-            seg = segment + next_job
-            next_job += 1
+        for k in range(p["restarts_subanchor"]):
+            seg = segs[bi][1] + k
             pos = []
             for i, c in enumerate(active):
                 d = [F(0), F(0), F(0)] if cl[c]["is_fixed"] else restart_noise(seed, seg, i, noise, p["use2d"])
@@ -520,7 +551,7 @@ def arcs_level(t, chr_name, p, o, seed, segment, run_job):
             r = run_job(3, prob, sched, seed, seg, o["chains"])
             rep["n_jobs"] += 1
             cand.setdefault(bi, []).append((r["resummed"], active, r))
-            _offer(best, bi, r["resummed"], (active, r))
+            _offer(best, bi, r["resummed"], (active, r), len(active) >= 4)
     for bi, (score, (active, r)) in best.items():
         for i, c in enumerate(active):
             if not cl[c]["is_fixed"]:

@@ -29,6 +29,18 @@
 //   singleton_clusters  the same on clusters given directly, timed
 //   normalize normalizeHeatmap (:2996-3040), normalizeHeatmapDiagonalTotal
 //             (:3146-3172), normalizeHeatmapInter (:3106-3144)
+//   drive_arcs
+//             tree, then the per-block body of reconstructClustersArcsDistances
+//             (:4027-4138) for one chromosome: children at the block's
+//             position, calcAnchorExpectedDistancesHeatmap,
+//             reconstructClusterArcsDistances(ib, i, false) (:4303-4487),
+//             densifyActiveRegion, reconstructClusterArcsDistances(ib, i, true);
+//             the annealing calls go through the product's reference-side
+//             glue to libpmmc_hip.so (a GPU is needed)
+//   drive_heatmap
+//             tree, then reconstructClustersHeatmapSingleLevel(level)
+//             (:1258-1422) on a given distance heatmap, with the chromosome
+//             boundaries its caller sets (:1219-1229), through the glue
 //
 // The reference keeps these members private; this driver is a white-box test,
 // so it lifts the access specifiers before including the reference header.
@@ -41,6 +53,8 @@
 #undef private
 #undef protected
 #include <Static.hpp>
+
+#include "../pmmc_amd/integration/reference_glue.h"
 
 static std::string tok() {
   std::string s;
@@ -514,6 +528,170 @@ static void mode_normalize(LooperSolver &s) {
   }
 }
 
+// Annealing settings of one [simulation_*] section: maxT dt tjs tjc
+// improvement min_successes steps.
+struct AnnealSettings {
+  float maxT, dt, tjs, tjc, impr;
+  int min_succ, steps;
+};
+
+static AnnealSettings read_anneal_settings() {
+  AnnealSettings a;
+  a.maxT = getf();
+  a.dt = getf();
+  a.tjs = getf();
+  a.tjc = getf();
+  a.impr = getf();
+  a.min_succ = geti();
+  a.steps = geti();
+  return a;
+}
+
+// harness seed, first Philox segment and chains per call of the glue
+static void read_run() {
+  const unsigned seed = static_cast<unsigned>(std::stoul(tok()));
+  const uint32_t segment = static_cast<uint32_t>(std::stoul(tok()));
+  const int chains = geti();
+  Settings::gpuSeed = seed;
+  setGpuSeed(seed);
+  pmmc_glue_set_segment(segment);
+  pmmc_glue_set_chains(chains);
+  srand(seed);
+}
+
+static void print_clusters(LooperSolver &s) {
+  printf("clusters %zu\n", s.clusters.size());
+  for (size_t i = 0; i < s.clusters.size(); ++i) {
+    const Cluster &c = s.clusters[i];
+    printf("%a %a %a %d %a %zu", c.pos.x, c.pos.y, c.pos.z, (int)c.is_fixed,
+           static_cast<double>(c.dist_to_next), c.children.size());
+    for (int ch : c.children) printf(" %d", ch);
+    printf("\n");
+  }
+}
+
+// drive_arcs: tree, chromosome name, then
+//   loop_density noise_anchor noise_subanchor noise_anchor_small
+//   restarts_anchor restarts_subanchor use2d milestone_fails,
+//   [simulation_arcs] and [simulation_arcs_smooth] settings,
+//   springs: ks_arcs kq_arcs ks kq ka w_dist w_angle,
+//   seed segment chains, then the positions of the chromosome's
+//   interaction-block clusters (level list 2, in order)
+static void mode_drive_arcs(LooperSolver &s) {
+  const std::string chr = tok();
+  Settings::loopDensity = geti();
+  Settings::noiseCoefficientLevelAnchor = getf();
+  Settings::noiseCoefficientLevelSubanchor = getf();
+  Settings::noiseAnchorSmall = getf();
+  Settings::simulationStepsLevelAnchor = geti();
+  Settings::simulationStepsLevelSubanchor = geti();
+  Settings::use2D = geti() != 0;
+  Settings::milestoneFailsThreshold = geti();
+  const AnnealSettings a = read_anneal_settings(), m = read_anneal_settings();
+  Settings::maxTemp = a.maxT;
+  Settings::dtTemp = a.dt;
+  Settings::tempJumpScale = a.tjs;
+  Settings::tempJumpCoef = a.tjc;
+  Settings::MCstopConditionImprovement = a.impr;
+  Settings::MCstopConditionMinSuccesses = a.min_succ;
+  Settings::MCstopConditionSteps = a.steps;
+  Settings::maxTempSmooth = m.maxT;
+  Settings::dtTempSmooth = m.dt;
+  Settings::tempJumpScaleSmooth = m.tjs;
+  Settings::tempJumpCoefSmooth = m.tjc;
+  Settings::MCstopConditionImprovementSmooth = m.impr;
+  Settings::MCstopConditionMinSuccessesSmooth = m.min_succ;
+  Settings::MCstopConditionStepsSmooth = m.steps;
+  Settings::springConstantStretchArcs = getf();
+  Settings::springConstantSqueezeArcs = getf();
+  Settings::springConstantStretch = getf();
+  Settings::springConstantSqueeze = getf();
+  Settings::springAngularConstant = getf();
+  Settings::weightDistSmooth = getf();
+  Settings::weightAngleSmooth = getf();
+  Settings::useSubanchorHeatmap = false;
+  Settings::useAnchorHeatmap = false;
+  Settings::useCTCFMotifOrientation = false;
+  read_run();
+  s.setLevel(LVL_INTERACTION_BLOCK);
+  std::vector<int> ibs = s.current_level[chr];
+  for (int ib : ibs) {
+    float x = getf(), y = getf(), z = getf();
+    s.clusters[ib].pos.set(x, y, z);
+  }
+  s.current_chr = chr;
+  s.gpu_cache.init();
+  fflush(stdout);
+  for (size_t i = 0; i < ibs.size(); ++i) {
+    const int ib = ibs[i];
+    s.active_region.assign(s.clusters[ib].children.begin(), s.clusters[ib].children.end());
+    for (int c : s.active_region) s.clusters[c].pos = s.clusters[ib].pos;
+    s.calcAnchorExpectedDistancesHeatmap();
+    s.reconstructClusterArcsDistances(ib, static_cast<int>(i), false);
+    s.densifyActiveRegion(ib, true);
+    s.reconstructClusterArcsDistances(ib, static_cast<int>(i), true, false);
+  }
+  printf("\n@@drive %d %zu\n", s.total_mc_steps, ibs.size());
+  for (int ib : ibs) printf("%d ", ib);
+  printf("\n");
+  print_clusters(s);
+}
+
+// drive_heatmap: tree, then level (0 or 1), noise_chr noise_segment
+//   restarts_chr restarts_segment use2d milestone_fails, [simulation_heatmap]
+//   settings, seed segment chains, diag n and the n x n distance heatmap of
+//   the level's clusters, then the positions of the chromosome clusters
+//   (level list 0, one per chromosome)
+static void mode_drive_heatmap(LooperSolver &s) {
+  const int level = geti();
+  Settings::noiseCoefficientLevelChr = getf();
+  Settings::noiseCoefficientLevelSegment = getf();
+  Settings::simulationStepsLevelChr = geti();
+  Settings::simulationStepsLevelSegment = geti();
+  Settings::use2D = geti() != 0;
+  Settings::milestoneFailsThreshold = geti();
+  const AnnealSettings h = read_anneal_settings();
+  Settings::maxTempHeatmap = h.maxT;
+  Settings::dtTempHeatmap = h.dt;
+  Settings::tempJumpScaleHeatmap = h.tjs;
+  Settings::tempJumpCoefHeatmap = h.tjc;
+  Settings::MCstopConditionImprovementHeatmap = h.impr;
+  Settings::MCstopConditionMinSuccessesHeatmap = h.min_succ;
+  Settings::MCstopConditionStepsHeatmap = h.steps;
+  Settings::useDensity = false;
+  Settings::useTelomerePositions = false;
+  Settings::initMethod = "random";
+  read_run();
+  const int diag = geti(), n = geti();
+  s.heatmap_dist.init(n);
+  s.heatmap_dist.diagonal_size = diag;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) s.heatmap_dist.v[i][j] = getf();
+  s.setLevel(LVL_CHROMOSOME);
+  for (const std::string &chr : s.chrs) {
+    const int root = s.current_level[chr][0];
+    float x = getf(), y = getf(), z = getf();
+    s.clusters[root].pos.set(x, y, z);
+  }
+  s.heatmap_chromosome_boundaries.clear();
+  if (level == LVL_SEGMENT && s.chrs.size() > 1) {
+    s.setLevel(LVL_SEGMENT);
+    int p = 0;
+    s.heatmap_chromosome_boundaries.push_back(p);
+    for (const std::string &chr : s.chrs) {
+      p += static_cast<int>(s.current_level[chr].size());
+      s.heatmap_chromosome_boundaries.push_back(p);
+    }
+  }
+  s.gpu_cache.init();
+  fflush(stdout);
+  s.reconstructClustersHeatmapSingleLevel(level);
+  printf("\n@@drive %d %zu\n", s.total_mc_steps, s.active_region.size());
+  for (int c : s.active_region) printf("%d ", c);
+  printf("\n");
+  print_clusters(s);
+}
+
 int main() {
   Settings settings;
   settings.init();
@@ -563,6 +741,12 @@ int main() {
     mode_singleton_clusters(s);
   } else if (mode == "normalize") {
     mode_normalize(s);
+  } else if (mode == "drive_arcs") {
+    read_tree(s);
+    mode_drive_arcs(s);
+  } else if (mode == "drive_heatmap") {
+    read_tree(s);
+    mode_drive_heatmap(s);
   } else {
     fprintf(stderr, "unknown mode %s\n", mode.c_str());
     return 2;

@@ -113,13 +113,17 @@ pmmc_status run_stage(pmmc_ctx *ctx, std::vector<std::unique_ptr<StageJob>> &job
   return PMMC_OK;
 }
 
-// the reference's best-of rule: score < best || best < 0 (first on ties)
+// the reference's best-of rule: score < best || best < 0 (first on ties), on
+// the value the annealing member returns: a float from the GPU members
+// (LooperSolver.hpp:1392-1402, :4457-4467: `double score = <float member>()`),
+// a double from the CPU annealer (MonteCarloArcsSmooth)
 struct Best {
   double score = -1.0;
   int job = -1;
-  void offer(double s, int j) {
-    if (s < score || score < 0.0) {
-      score = s;
+  void offer(double s, int j, bool gpu_member = true) {
+    const double f = gpu_member ? static_cast<double>(static_cast<float>(s)) : s;
+    if (f < score || score < 0.0) {
+      score = f;
       job = j;
     }
   }
@@ -192,7 +196,31 @@ extern "C" pmmc_status pmmc_arcs_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t chr,
       if (t->cl[ib].level == 3) blocks.push_back({c, ib});
   }
   rep.n_blocks = (int32_t)blocks.size();
-  uint32_t next_job = 0;
+  // Philox segments in the order the reference driver issues its GPU calls
+  // (:4027-4138: per block, its anchor restarts, then densification and its
+  // sub-anchor restarts; a block of at most one child makes no call, and a
+  // sub-anchor problem of fewer than 4 beads runs the CPU annealer there,
+  // :4431-4437), so a run matches the reference's driver over the
+  // reference-side glue, whose per-call segment counter starts at `segment`.
+  // Jobs that are not GPU calls there take segments from 2^30 on.
+  std::vector<uint32_t> aseg(blocks.size(), 0), sseg(blocks.size(), 0);
+  {
+    uint32_t call = 0, cpu = 0;
+    for (size_t bi = 0; bi < blocks.size(); ++bi) {
+      const size_t na = t->cl[blocks[bi].cluster].children.size();
+      if (na <= 1) continue;
+      aseg[bi] = segment + call;
+      call += (uint32_t)p->restarts_anchor;
+      const size_t ns = na + (na - 1) * (size_t)p->loop_density;  // after densification
+      if (ns >= 4) {
+        sseg[bi] = segment + call;
+        call += (uint32_t)p->restarts_subanchor;
+      } else {
+        sseg[bi] = segment + (1u << 30) + cpu;
+        cpu += (uint32_t)p->restarts_subanchor;
+      }
+    }
+  }
 
   // ---- anchor stage: every restart of every block with >= 2 children
   std::vector<std::unique_ptr<StageJob>> jobs;
@@ -218,7 +246,7 @@ extern "C" pmmc_status pmmc_arcs_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t chr,
     if (st != PMMC_OK) return fail("pmmc_arcs_level: arc schedule preset", st);
     for (int32_t k = 0; k < p->restarts_anchor; ++k) {
       auto j = std::make_unique<StageJob>();
-      const uint32_t jseg = segment + next_job++;
+      const uint32_t jseg = aseg[bi] + (uint32_t)k;
       j->level = PMMC_LEVEL_ARCS;
       j->block = (int32_t)bi;
       j->beads = active;
@@ -287,7 +315,7 @@ extern "C" pmmc_status pmmc_arcs_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t chr,
     if (st != PMMC_OK) return fail("pmmc_arcs_level: smooth schedule preset", st);
     for (int32_t k = 0; k < p->restarts_subanchor; ++k) {
       auto j = std::make_unique<StageJob>();
-      const uint32_t jseg = segment + next_job++;
+      const uint32_t jseg = sseg[bi] + (uint32_t)k;
       j->level = PMMC_LEVEL_SMOOTH;
       j->block = (int32_t)bi;
       j->beads = active;
@@ -321,7 +349,8 @@ extern "C" pmmc_status pmmc_arcs_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t chr,
   {
     std::vector<Best> best(blocks.size());
     for (size_t ji = 0; ji < jobs.size(); ++ji)
-      best[jobs[ji]->block].offer(jobs[ji]->st.energy_resummed, (int)ji);
+      best[jobs[ji]->block].offer(jobs[ji]->st.energy_resummed, (int)ji,
+                                  jobs[ji]->beads.size() >= 4);
     for (size_t bi = 0; bi < blocks.size(); ++bi) {
       if (best[bi].job < 0) continue;
       const StageJob &j = *jobs[best[bi].job];

@@ -9,6 +9,14 @@
 // reference returned.  Build it from the reference's tree with
 //   g++ -std=c++17 -O2 -Isrc/include -Isrc/thirdparty -I<repo>/include -c reference_glue.cpp
 // in place of the four .cu files, and link libpmmc_hip.so.
+//
+// Two knobs a build or a harness may set (the reference's own code calls
+// neither; declared in reference_glue.h): the chains per call
+// (pmmc_glue_set_chains, 0 = the library default) and the Philox segment of
+// the next call (pmmc_glue_set_segment; each call takes the next one).
+// With PMMC_GLUE_TRACE=<file> in the environment every annealing call
+// appends its flattened inputs, the positions it wrote back into
+// clusters[], its return value and total_mc_steps to that file.
 #include <LooperSolver.hpp>
 
 #include <ctime>
@@ -17,12 +25,67 @@
 #include <vector>
 
 #include "pmmc_hip.h"
+#include "reference_glue.h"
 
 namespace {
 
 pmmc_ctx *g_ctx = nullptr;       // one context per process (device 0)
 unsigned int g_seed = 0;         // setGpuSeed
 uint32_t g_call = 0;             // distinct Philox segment per call
+int32_t g_chains = 0;            // chains per call (0: library default)
+
+// ---- optional call trace (PMMC_GLUE_TRACE) ----
+FILE *trace_file() {
+  static FILE *f = nullptr;
+  static bool tried = false;
+  if (!tried) {
+    tried = true;
+    const char *path = std::getenv("PMMC_GLUE_TRACE");
+    if (path && *path) f = std::fopen(path, "a");
+  }
+  return f;
+}
+
+void trace_floats(FILE *f, const char *tag, const float *v, size_t n) {
+  std::fprintf(f, "%s %zu", tag, n);
+  for (size_t i = 0; i < n; ++i) std::fprintf(f, " %a", v[i]);
+  std::fprintf(f, "\n");
+}
+
+void trace_ints(FILE *f, const char *tag, const int32_t *v, size_t n) {
+  std::fprintf(f, "%s %zu", tag, n);
+  for (size_t i = 0; i < n; ++i) std::fprintf(f, " %d", v[i]);
+  std::fprintf(f, "\n");
+}
+
+// header and inputs of one call
+void trace_call(int level, const pmmc_run &run, int n, float step, long long steps_before,
+                const std::vector<float> &pos, const std::vector<uint8_t> &fixed) {
+  FILE *f = trace_file();
+  if (!f) return;
+  std::fprintf(f, "@@call %d %llu %u %d %d %a %lld\n", level,
+               static_cast<unsigned long long>(run.seed), run.segment, run.n_chains, n,
+               static_cast<double>(step), steps_before);
+  trace_floats(f, "pos", pos.data(), pos.size());
+  std::vector<int32_t> fx(fixed.begin(), fixed.end());
+  trace_ints(f, "fixed", fx.data(), fx.size());
+}
+
+// what the call left in clusters[] (xyz of the active region) and returned
+void trace_result(const std::vector<Cluster> &cl, const std::vector<int> &active, float ret,
+                  long long steps_after) {
+  FILE *f = trace_file();
+  if (!f) return;
+  std::vector<float> out;
+  for (int c : active) {
+    out.push_back(cl[c].pos.x);
+    out.push_back(cl[c].pos.y);
+    out.push_back(cl[c].pos.z);
+  }
+  trace_floats(f, "out", out.data(), out.size());
+  std::fprintf(f, "ret %a %lld\n", static_cast<double>(ret), steps_after);
+  std::fflush(f);
+}
 
 pmmc_ctx *ctx_or_die() {
   if (!g_ctx) {
@@ -70,6 +133,9 @@ void setGpuSeed(unsigned int seed) {
   if (g_ctx) pmmc_set_gpu_seed(g_ctx, seed);
 }
 
+void pmmc_glue_set_chains(int32_t chains) { g_chains = chains > 0 ? chains : 0; }
+void pmmc_glue_set_segment(uint32_t segment) { g_call = segment; }
+
 // GpuResourceCache (LooperSolver.hpp:54-106): device memory now lives in the
 // pmmc_ctx arena; the cache only records that the device is ready.
 void GpuResourceCache::init() {
@@ -113,7 +179,14 @@ float LooperSolver::ParallelMonteCarloHeatmap(float step_size) {
       Settings::MCstopConditionMinSuccessesHeatmap, Settings::MCstopConditionStepsHeatmap);
   pmmc_schedule s;
   check(pmmc_schedule_preset(PMMC_LEVEL_HEATMAP, PMMC_PRESET_GPU, &ls, step_size, &s), "preset");
-  const pmmc_run run = {run_seed(), g_call++, 0, 0};
+  const pmmc_run run = {run_seed(), g_call++, 0, g_chains};
+  if (FILE *f = trace_file()) {
+    trace_call(1, run, n, step_size, static_cast<long long>(total_mc_steps), pos, fixed);
+    trace_floats(f, "mat", dist.data(), dist.size());
+    std::vector<int32_t> bnd(heatmap_chromosome_boundaries.begin(), heatmap_chromosome_boundaries.end());
+    std::fprintf(f, "diag %d\n", p.diagonal);
+    trace_ints(f, "bnd", bnd.data(), bnd.size());
+  }
   pmmc_anneal_stats st;
   check(pmmc_heatmap_anneal(ctx_or_die(), &p, &s, &run, pos.data(), &st), "heatmap anneal");
   for (int i = 0; i < n; ++i) {
@@ -123,7 +196,9 @@ float LooperSolver::ParallelMonteCarloHeatmap(float step_size) {
     c.pos.z = pos[3 * i + 2];
   }
   total_mc_steps += static_cast<int>(st.best_chain_moves);
-  return static_cast<float>(st.energy_resummed);  // calcScoreHeatmapActiveRegion()
+  const float ret = static_cast<float>(st.energy_resummed);  // calcScoreHeatmapActiveRegion()
+  trace_result(clusters, active_region, ret, static_cast<long long>(total_mc_steps));
+  return ret;
 }
 
 // ParallelMonteCarloArcs.cu:680-944
@@ -167,7 +242,13 @@ float LooperSolver::parallelMonteCarloArcs(float step_size) {
       Settings::MCstopConditionSteps);
   pmmc_schedule s;
   check(pmmc_schedule_preset(PMMC_LEVEL_ARCS, PMMC_PRESET_GPU, &ls, step_size, &s), "preset");
-  const pmmc_run run = {run_seed(), g_call++, 0, 0};
+  const pmmc_run run = {run_seed(), g_call++, 0, g_chains};
+  if (FILE *f = trace_file()) {
+    trace_call(2, run, n, step_size, static_cast<long long>(total_mc_steps), pos, fixed);
+    trace_floats(f, "mat", ed.data(), ed.size());
+    trace_ints(f, "loops", lp.data(), lp.size());
+    trace_floats(f, "loopk", lq.data(), lq.size());
+  }
   pmmc_anneal_stats st;
   check(pmmc_arcs_anneal(ctx_or_die(), &p, &s, &run, pos.data(), &st), "arcs anneal");
   for (int i = 0; i < n; ++i) {
@@ -180,7 +261,9 @@ float LooperSolver::parallelMonteCarloArcs(float step_size) {
   if (sim_logger)
     sim_logger->logAcceptanceRatio("arcs_gpu", static_cast<int>(st.best_chain_moves),
                                    static_cast<int>(st.best_chain_accepted));
-  return static_cast<float>(st.energy);  // the best chain's running score
+  const float ret = static_cast<float>(st.energy);  // the best chain's running score
+  trace_result(clusters, active_region, ret, static_cast<long long>(total_mc_steps));
+  return ret;
 }
 
 // ParallelMonteCarloSmooth.cu:357-506
@@ -213,7 +296,11 @@ float LooperSolver::ParallelMonteCarloSmooth(float step_size, unsigned long long
       Settings::MCstopConditionMinSuccessesSmooth, Settings::MCstopConditionStepsSmooth);
   pmmc_schedule s;
   check(pmmc_schedule_preset(PMMC_LEVEL_SMOOTH, PMMC_PRESET_GPU, &ls, step_size, &s), "preset");
-  const pmmc_run run = {seed != 0 ? seed : run_seed(), g_call++, 0, 0};
+  const pmmc_run run = {seed != 0 ? seed : run_seed(), g_call++, 0, g_chains};
+  if (FILE *f = trace_file()) {
+    trace_call(3, run, n, step_size, static_cast<long long>(total_mc_steps), pos, fixed);
+    trace_floats(f, "mat", dtn.data(), dtn.size());
+  }
   pmmc_anneal_stats st;
   check(pmmc_smooth_anneal(ctx_or_die(), &p, &s, &run, pos.data(), &st), "smooth anneal");
   for (int i = 0; i < n; ++i) {
@@ -227,7 +314,9 @@ float LooperSolver::ParallelMonteCarloSmooth(float step_size, unsigned long long
   if (sim_logger)
     sim_logger->logAcceptanceRatio("smooth_gpu", static_cast<int>(st.best_chain_moves),
                                    static_cast<int>(st.best_chain_accepted));
-  return static_cast<float>(st.energy_resummed);  // calcScoreStructureSmooth(true, true)
+  const float ret = static_cast<float>(st.energy_resummed);  // calcScoreStructureSmooth(true, true)
+  trace_result(clusters, active_region, ret, static_cast<long long>(total_mc_steps));
+  return ret;
 }
 
 // ParallelMarkArcs.cu:69-244: all chromosomes in one call, grouping state
