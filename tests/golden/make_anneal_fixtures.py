@@ -18,7 +18,9 @@ distanceCorrelation, structuralSimilarity, Chromosome::calcRMSD) on seeded
 random structures and maps, for the metric restatements to be pinned against.
 
 Run from the repository root after `make -C oracle ref`:
-    python tests/golden/make_anneal_fixtures.py
+    python tests/golden/make_anneal_fixtures.py              # everything
+    python tests/golden/make_anneal_fixtures.py --levels 3   # one level of
+        anneal_ref.json, the other levels and metrics_ref.json kept
 """
 from __future__ import annotations
 
@@ -38,9 +40,10 @@ from oracle import refdriver as R  # noqa: E402
 from pmmc_amd import synth  # noqa: E402
 
 SEEDS = list(range(16))
-# smooth instance of the band test: 21 anchors x 49 beads between them = 1,001
-# beads (the 20,000-bead instance needs ~39 M moves per chain at this schedule)
-SMOOTH_ANCHORS, SMOOTH_ADD = 21, 49
+# smooth instance of the band test: the 19,951-bead segment of the smooth
+# workload, 400 fixed anchors with 49 beads between consecutive ones (about
+# 39 M moves per chain at this schedule)
+SMOOTH_ANCHORS, SMOOTH_ADD = 400, 49
 HEADER = R.HEADER
 hx = R.hx
 run_ref = R.run
@@ -98,22 +101,31 @@ def metrics_cases():
     return cases
 
 
-def main() -> int:
+def main(argv=None) -> int:
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--levels", type=int, nargs="*", default=[1, 2, 3])
+    levels = ap.parse_args(argv).levels
     if not R.available():
         print(f"{R.REF} missing: run `make -C oracle ref` (needs the reference sources)")
         return 1
-    out = {"metrics": metrics_cases()}
-    json.dump(out, open(os.path.join(HERE, "metrics_ref.json"), "w"), indent=0)
+    path = os.path.join(HERE, "anneal_ref.json")
+    if sorted(levels) == [1, 2, 3]:
+        out = {"metrics": metrics_cases()}
+        json.dump(out, open(os.path.join(HERE, "metrics_ref.json"), "w"), indent=0)
+        old = {}
+    else:
+        old = json.load(open(path))
     res = {"generator": "tests/golden/make_anneal_fixtures.py",
            "annealers": "LooperSolver::MonteCarloHeatmap / MonteCarloArcs / MonteCarloArcsSmooth "
                         "(reference, compiled in place), glibc rand() seeded with srand(1 + seed)",
            "settings": {str(k): v for k, v in synth.CONFIG_SETTINGS.items()},
            "smooth_anchors": SMOOTH_ANCHORS, "smooth_add": SMOOTH_ADD}
-    jobs = [(lvl, s) for lvl in (1, 2, 3) for s in SEEDS]
+    jobs = [(lvl, s) for lvl in levels for s in SEEDS]
     with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         recs = list(ex.map(lambda t: (t[0], ref_anneal(*t)), jobs))
     for lvl in (1, 2, 3):
-        res[str(lvl)] = [r for lv, r in recs if lv == lvl]
+        res[str(lvl)] = [r for lv, r in recs if lv == lvl] if lvl in levels else old[str(lvl)]
     json.dump(res, open(os.path.join(HERE, "anneal_ref.json"), "w"), indent=1)
     for lvl in (1, 2, 3):
         sc = [r["score"] for r in res[str(lvl)]]

@@ -27,6 +27,6 @@ def test_oracle_annealing_within_band_of_reference(level):
         return job, rec
 
     with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
-        results = dict(ex.map(one, B.jobs(level)))
-    stats = B.compare(level, results)
+        results = dict(ex.map(one, B.jobs(level, B.K_CPU[level])))
+    stats = B.compare(level, results, B.K_CPU[level])
     print(level, stats)

@@ -26,7 +26,7 @@ def _proposals(rng, n, k, scale):
     return beads, disp
 
 
-def _check_delta(ctx, prob, k=2048, scale=0.5, r=0.0, seed=0):
+def _check_delta(ctx, prob, k=2048, scale=0.5, r=0.0, seed=0, label=None):
     rng = np.random.default_rng(seed)
     P = O.Problem(prob)
     n = P.n
@@ -41,8 +41,21 @@ def _check_delta(ctx, prob, k=2048, scale=0.5, r=0.0, seed=0):
         a, b = P.local_pair(pos, p, q, r)
         err = abs(got[i] - want) / (abs(a) + abs(b) + 1e-7 / TOL)
         worst = max(worst, err)
+    print(f"ΔE worst relative error [{label or type(prob).__name__} n={n} r={r}]: {worst:.3e}",
+          flush=True)
     assert worst <= TOL, f"worst relative ΔE error {worst:.3e}"
     return worst
+
+
+# The measured worst ΔE error per level, at each level's workload size, on
+# proposals of the level's own step scale (stated per level; the bar is TOL).
+@pytest.mark.parametrize("level", [1, 2, 3])
+def test_delta_error_per_level_at_workload_size(ctx, level):
+    prob, step, _ = synth.config_case(level, 0)
+    scale = {1: 0.75, 2: 1.0, 3: 1.0}[level] * step
+    worst = _check_delta(ctx, prob, k=4096, scale=scale, seed=level,
+                         label=f"level {level} workload")
+    assert worst <= TOL
 
 
 def test_delta_heatmap(ctx):
