@@ -175,7 +175,9 @@ def heatmap_leg(ctx, with_cpu=True, cpu_seconds=8.0):
     from pmmc_amd import metrics, schedule_preset, settings, synth
     prob, step, truth = synth.config_case(1, 0)
     sched = schedule_preset(1, 0, settings(**synth.CONFIG_SETTINGS[1]), step)
-    chains = 98304  # one round of the lane-pair register kernel (6 ensembles per CU)
+    # 4 chains per slot: one round of the lane-pair register kernel's workers
+    # (6 per CU, 98,304 slots) takes 393,216 chains from the queue
+    chains = 4 * 98304
     warm = schedule_preset(1, 0, settings(**synth.CONFIG_SETTINGS[1]), step)
     warm.max_moves = 1024
     ctx.anneal(prob, warm, seed=1, n_chains=chains)
@@ -189,8 +191,8 @@ def heatmap_leg(ctx, with_cpu=True, cpu_seconds=8.0):
          "final_energy": st["energy_resummed"],
          "partner_evaluations_per_s": 2 * 63 * st["moves"] / (st["kernel_ms"] * 1e-3),
          "bound": "issue latency (a chain's 33-64 bead positions held by a lane pair, 96 VGPRs "
-                  "each: 3 waves per SIMD; "
-                  "12 B read + 12 B x acceptance written per chain-move)"}
+                  "each: 3 waves per SIMD; a slot whose chain stops takes the next chain of the "
+                  "queue; 12 B read + 12 B x acceptance written per chain-move)"}
     h.update(metrics.compare(pos, truth))
     if with_cpu:
         h["cpu_reference"] = cpu_reference(1, prob, step, synth.CONFIG_SETTINGS[1], cpu_seconds,

@@ -74,8 +74,10 @@ typedef enum {
 /* Schedule of one Metropolis chain.  One chain performs blocks of `block`
  * moves.  Move m (0-based, counted) of chain c:
  *   bead p  = free[ pick(B, n_free) ], B = Philox(key, {m>>2, c>>pick_shift, segment,
- *             level<<28|1})[m&3]  (pick_shift = 6: the 64 chains of one wavefront
- *             share bead picks; displacement and acceptance stay per chain)
+ *             level<<28|1})[m&3]  (pick_shift = pmmc_pick_shift(level, n): 6, the
+ *             64 chains of one wavefront share bead picks, or 0 for heatmaps of
+ *             33..64 beads, whose register-resident kernels draw them per
+ *             chain; displacement and acceptance are always per chain)
  *   {dx,dy,dz,U} = Philox(key, {m, c, segment, level<<28|0}), d = (2*u01-1)*step
  *   ΔE = local(p, x_p + d) - local(p, x_p)       (FP32, level semantics)
  *   E' = E + ΔE                                   (FP64 running total)
@@ -115,6 +117,10 @@ typedef struct {
                              milestone (before the stop test); 0: never         */
   int64_t max_moves;      /* hard cap per chain                                 */
 } pmmc_schedule;
+
+/* pick_shift of the bead-pick stream above for a problem of `level` with n
+ * beads: 0 (picks per chain) for heatmaps of 33..64 beads, 6 otherwise. */
+int32_t pmmc_pick_shift(int32_t level, int32_t n);
 
 /* Which reference annealer a preset reproduces. */
 typedef enum { PMMC_PRESET_CPU = 0, PMMC_PRESET_GPU = 1 } pmmc_preset;

@@ -203,9 +203,16 @@ class Problem:
         return (f(C.byref(self.c), _p(x, f32p), p, None, r),
                 f(C.byref(self.c), _p(x, f32p), p, _p(q, f32p), r))
 
-    def anneal(self, sched: OrcSchedule, seed, segment, chain, pick_shift=6, pos=None,
+    def pick_shift(self):
+        """The stream rule: bead picks per chain (0) for heatmaps of 33..64
+        beads, shared by the 64 chains of an ensemble (6) otherwise."""
+        return 0 if self.level == LEVEL_HEATMAP and 32 < self.n <= 64 else 6
+
+    def anneal(self, sched: OrcSchedule, seed, segment, chain, pick_shift=None, pos=None,
                trace=False, full_rescore=0, tie_rel=0.0):
         L = load()
+        if pick_shift is None:
+            pick_shift = self.pick_shift()
         x = (self.pos0 if pos is None else np.ascontiguousarray(pos, np.float32).reshape(-1)).copy()
         st = OrcStats()
         tr = np.zeros(int(min(sched.max_moves + sched.block, 1 << 26)), np.uint8) if trace else None
@@ -216,9 +223,11 @@ class Problem:
             d["trace"] = tr[:d["moves"]].copy()
         return x.reshape(-1, 3), d
 
-    def anneal_many(self, sched, seed, segment, chain0, n_chains, n_threads, pick_shift=6,
+    def anneal_many(self, sched, seed, segment, chain0, n_chains, n_threads, pick_shift=None,
                     full_rescore=0, tie_rel=0.0):
         L = load()
+        if pick_shift is None:
+            pick_shift = self.pick_shift()
         out = np.zeros(n_chains * self.n * 3, np.float32)
         stats = (OrcStats * n_chains)()
         secs = L.orc_anneal_many(self.level, self.ptr(), C.byref(sched), seed, segment, chain0,

@@ -199,6 +199,7 @@ SIGNATURES = {
     "pmmc_last_error": (C.c_char_p, [C.c_void_p]),
     "pmmc_ctx_cu_count": (C.c_int, [C.c_void_p]),
     "pmmc_set_gpu_seed": (None, [C.c_void_p, C.c_uint64]),
+    "pmmc_pick_shift": (C.c_int32, [C.c_int32, C.c_int32]),
     "pmmc_default_chains": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "pmmc_heatmap_anneal": (C.c_int, [C.c_void_p, C.POINTER(HeatmapProblem), C.POINTER(Schedule),
                                       C.POINTER(Run), f32p, C.POINTER(Stats)]),

@@ -51,6 +51,11 @@ def schedule_preset(level: int, preset: int, s: A.LevelSettings, step_size: floa
     return out
 
 
+def pick_shift(level: int, n: int) -> int:
+    """pmmc_pick_shift: chains c and c' share bead picks iff c >> s == c' >> s."""
+    return int(A.load().pmmc_pick_shift(int(level), int(n)))
+
+
 @dataclass
 class Heatmap:
     pos: np.ndarray

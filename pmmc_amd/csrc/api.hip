@@ -1,6 +1,7 @@
 // api.hip -- context lifetime, schedule presets and problem preparation.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -66,6 +67,8 @@ extern "C" int pmmc_ctx_cu_count(const pmmc_ctx *c) { return c ? c->cu_count : 0
 extern "C" void pmmc_set_gpu_seed(pmmc_ctx *c, uint64_t seed) {
   if (c) c->default_seed = seed;
 }
+
+extern "C" int32_t pmmc_pick_shift(int32_t level, int32_t n) { return pick_shift_of(level, n); }
 
 extern "C" int32_t pmmc_default_chains(const pmmc_ctx *c, int32_t level, int32_t n,
                                        int32_t symmetric) {
@@ -358,6 +361,15 @@ pmmc_status prepare_problem(pmmc_ctx *ctx, int level, const void *problem, const
   h.n_chains = run.n_chains > 0 ? run.n_chains : default_chains(ctx, h);
   if (static_cast<uint64_t>(h.chain0) + h.n_chains > 0x7FFFFFFFull)
     return fail(ctx, PMMC_ERR_INVALID, "chain ids exceed 2^31");
+  h.pick_shift = pick_shift_of(h.level, h.n);
+  h.workers = 0;
+  if (h.level == PMMC_LEVEL_HEATMAP && hreg_halves(h) > 0 && !stage_ok(h.n)) {
+    // queued register kernels: at most one resident round of worker
+    // workgroups; their slots take the remaining chains as chains stop
+    const int n_ens = (h.n_chains + kLanes - 1) / kLanes;
+    const int cus = ctx && ctx->cu_count > 0 ? ctx->cu_count : 256;
+    h.workers = std::min(n_ens, std::max(1, resident_ensembles_per_cu(h) * cus));
+  }
   return PMMC_OK;
 }
 

@@ -172,6 +172,12 @@ __device__ void init_problem_body(DevProblem &P, double *red) {
     *P.E0 = E0;
     *P.offset = off;
   }
+  // chain queue: slot s starts on chain s, the queue hands out the rest
+  if (P.workers > 0) {
+    const int slots = P.workers * kLanes;
+    for (int i = threadIdx.x; i < slots; i += blockDim.x) P.slot_chain[i] = i < P.n_chains ? i : -2;
+    if (threadIdx.x == 0) *P.queue = slots;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_init_problem(DevProblem *probs) {
@@ -605,6 +611,10 @@ static constexpr int kStageMaxBytes = PMMC_STAGE_MAX_BYTES;
 
 bool stage_ok(int n) { return static_cast<size_t>(n) * 3 * kLanes * sizeof(float) <= kStageMaxBytes; }
 
+int pick_shift_of(int level, int n) {
+  return (level == PMMC_LEVEL_HEATMAP && n > 32 && n <= 64 && !stage_ok(n)) ? 0 : kPickShift;
+}
+
 int resident_ensembles_per_cu(const HostProblem &h) {
   // keyed by everything that selects the kernel: level, staging, the
   // register-resident form and, for it, the table symmetry (the lane-pair
@@ -754,6 +764,10 @@ static void size_problem(Sizer &z, const HostProblem &h) {
   const int ne = (h.n_chains + kLanes - 1) / kLanes;
   z.add<EnsState>(ne);
   z.add<float>(static_cast<size_t>(ne) * h.pos0.size() * kLanes);
+  if (h.workers > 0) {
+    z.add<int32_t>(1);
+    z.add<int32_t>(static_cast<size_t>(h.workers) * kLanes);
+  }
 }
 
 template <class T>
@@ -815,6 +829,12 @@ void fill_problem(Carver &cv, hipStream_t s, const HostProblem &h, int index, De
   const int ne = (h.n_chains + kLanes - 1) / kLanes;
   EnsState *sts = cv.take<EnsState>(ne);
   float *pos = cv.take<float>(static_cast<size_t>(ne) * h.pos0.size() * kLanes);
+  P.workers = h.workers;
+  P.pick_shift = h.pick_shift;
+  if (h.workers > 0) {
+    P.queue = cv.take<int32_t>(1);
+    P.slot_chain = cv.take<int32_t>(static_cast<size_t>(h.workers) * kLanes);
+  }
   P.ens0 = static_cast<int32_t>(ens.size());
   P.n_ens = ne;
   for (int e = 0; e < ne; ++e) {

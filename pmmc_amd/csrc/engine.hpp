@@ -78,6 +78,13 @@ struct DevProblem {
   int32_t ens0, n_ens, n_chains, pad1;
   float *out_pos;            // [n*3]
   struct DevResult *result;
+  // chain queue of the register-resident heatmap kernels (33..64 beads): the
+  // first `workers` ensembles' workgroups are workers whose 64 slots each run
+  // one chain at a time and take the next chain from *queue when it stops
+  int32_t workers;           // 0: every ensemble's workgroup runs its own chains
+  int32_t pick_shift;        // chains c, c' share bead picks iff c>>s == c'>>s
+  int32_t *queue;            // [1] next chain (relative to chain0) to hand out
+  int32_t *slot_chain;       // [workers*64] chain a slot runs, -1 none, -2 drained
 };
 
 struct DevEns {

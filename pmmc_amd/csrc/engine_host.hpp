@@ -37,7 +37,16 @@ struct HostProblem {
   uint64_t seed = 0;
   uint32_t segment = 0, chain0 = 0;
   int n_chains = 0;
+  // bead-pick groups (pick_shift_of) and, for the queued register kernels,
+  // the worker workgroups (set by prepare_problem)
+  int pick_shift = 6;
+  int workers = 0;
 };
+
+// Bead-pick group of chain c is c >> pick_shift_of(level, n): the 64 chains
+// of an ensemble share picks (6), except heatmaps of 33..64 beads, whose
+// register-resident kernels draw them per chain (0).
+int pick_shift_of(int level, int n);
 
 pmmc_status prepare_problem(pmmc_ctx *ctx, int level, const void *problem, const pmmc_schedule &s,
                             const pmmc_run &run, HostProblem &out);

@@ -22,6 +22,15 @@
 // Two forms: k_anneal_hreg (one lane per chain, 2 waves per SIMD) runs
 // asymmetric tables; k_anneal_hpair (a chain split over a lane pair by bead
 // parity, 3 waves per SIMD) runs symmetric ones, 11% faster on 64 beads.
+//
+// Chains draw their bead picks per chain, so the lanes of a wavefront are
+// independent, and a slot (a lane, or a lane pair) whose chain has stopped
+// takes the next chain of its problem from a queue: with more chains than
+// slots a wavefront no longer idles until its longest chain ends.  The
+// workgroups of the problem's first P.workers ensembles are the workers;
+// the others exit at once.  Chain c keeps its state in ensemble c/64, lane
+// c%64 (the same place as without the queue), so results depend only on the
+// chains, never on which slot ran them.
 #include <hip/hip_runtime.h>
 
 #include "ctx.hpp"
@@ -87,9 +96,59 @@ __device__ __forceinline__ float hreg_delta(const float *lw, const float *lw2, c
 
 // Milestone re-sum over the memory copy of the positions, out of line so that
 // its loops do not share a register allocation with the move loop.
-__device__ __noinline__ double hreg_resum(const DevProblem &P, float *pos, int lane, float r) {
-  const PosView V = global_view(pos, lane);
+// A chain's positions seen through a per-lane pointer to its column of the
+// SoA memory copy: element (bead j, axis k) at q[(3 j + k) * 64].
+struct LaneView {
+  const float PMMC_AS_GLOBAL *q;
+  __device__ __forceinline__ float get(int j, int k) const { return q[(3 * j + k) * kLanes]; }
+};
+
+__device__ __noinline__ double hreg_resum(const DevProblem &P, const float PMMC_AS_GLOBAL *q, float r) {
+  const LaneView V{q};
   return resum_heatmap(P, V, r);
+}
+
+// Bead of a pick index: the free-bead list, or the index itself when no bead
+// is fixed (a uniform branch; the list is then the identity)
+__device__ __forceinline__ int free_bead(const DevProblem &P, bool all_free, uint32_t i) {
+  if (all_free) return static_cast<int>(i);
+  return P.free_list[i];
+}
+
+// Per-chain annealing state (one EnsState lane) held in registers.
+struct ChainRegs {
+  double E, T, ms, best;
+  float step;
+  uint32_t iters, acc;
+  int msucc, misses, miles, status;
+};
+
+__device__ __forceinline__ void chain_load(const EnsState *st, int l, ChainRegs &r) {
+  r.E = st->E[l];
+  r.T = st->T[l];
+  r.ms = st->ms[l];
+  r.best = st->best[l];
+  r.step = st->step[l];
+  r.iters = st->iters[l];
+  r.acc = st->acc[l];
+  r.msucc = st->msucc[l];
+  r.misses = st->misses[l];
+  r.miles = st->milestones[l];
+  r.status = st->status[l];
+}
+
+__device__ __forceinline__ void chain_save(EnsState *st, int l, const ChainRegs &r) {
+  st->E[l] = r.E;
+  st->T[l] = r.T;
+  st->ms[l] = r.ms;
+  st->best[l] = r.best;
+  st->step[l] = r.step;
+  st->iters[l] = r.iters;
+  st->acc[l] = r.acc;
+  st->msucc[l] = r.msucc;
+  st->misses[l] = r.misses;
+  st->milestones[l] = r.miles;
+  st->status[l] = r.status;
 }
 
 // Occupancy target: 2 waves per SIMD (192 position VGPRs of 256).
@@ -99,61 +158,93 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   extern __shared__ float lw[];  // W (and W2), NB*NB floats each
   const DevEns it = ens[blockIdx.x];
   const DevProblem &P = probs[it.problem];
-  const int lane = threadIdx.x;
-  EnsState *__restrict__ st = it.st;
-  int status = st->status[lane];
-  if (__ballot(status == 0) == 0) return;
-
+  const int w = static_cast<int>((it.chain0 - P.chain0) >> 6);  // worker index
+  if (w >= P.workers) return;  // not a worker: its chains are run by the queue
   const int n = P.n;
   constexpr int L = NB;  // == P.ldw (rows padded with zero weights)
   float *lw2 = lw + L * L;
   // W and W2 hold n rows of L floats; rows n..L-1 of the LDS tables are zero
-  for (int i = lane; i < L * L; i += kLanes) {
+  for (int i = threadIdx.x; i < L * L; i += kLanes) {
     const bool in = i < n * L;
     lw[i] = in ? P.W[i] : 0.f;
     if constexpr (ASYM) lw2[i] = in ? P.W2[i] : 0.f;
   }
   __syncthreads();
-  // global, not generic, address space: flat loads and stores would also
-  // count on lgkmcnt, so every wait for a weight-row LDS read would wait for
-  // the memory copy's traffic too
-  float PMMC_AS_GLOBAL *g = (float PMMC_AS_GLOBAL *)(it.pos) + lane;
-  float X[NB], Y[NB], Z[NB];
-#pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    X[j] = j < n ? g[(3 * j + 0) * kLanes] : 0.f;
-    Y[j] = j < n ? g[(3 * j + 1) * kLanes] : 0.f;
-    Z[j] = j < n ? g[(3 * j + 2) * kLanes] : 0.f;
-  }
 
+  const DevEns *pe = ens + (static_cast<int>(blockIdx.x) - w);  // the problem's ensembles
+  const int lane = threadIdx.x;
+  const int slot = w * kLanes + lane;
   const pmmc_schedule S = P.sched;
   const uint64_t seed = P.seed;
   const uint32_t segment = P.segment;
   const int nfree = P.nfree;
   const float r = S.sphere_radius;
+  const int nch = P.n_chains;
+  const bool all_free = nfree == n;
+  const bool aligned4 = (S.block & 3) == 0;
 
-  double E = st->E[lane], T = st->T[lane], ms = st->ms[lane], best = st->best[lane];
-  float step = st->step[lane];
-  uint32_t iters = st->iters[lane], acc = st->acc[lane];
-  int msucc = st->msucc[lane], misses = st->misses[lane], miles = st->milestones[lane];
-  bool active = (status == 0);
-  const uint32_t chain = it.chain0 + static_cast<uint32_t>(lane);
-  const uint32_t group = it.chain0 >> kPickShift;
+  // the slot's chain: memory copy of its positions (global address space,
+  // not generic: flat loads and stores would also count on lgkmcnt, so every
+  // wait for a weight-row LDS read would wait for the copy's traffic too) and
+  // its state; lanes without a chain keep a valid address (their own
+  // ensemble slot) and never store
+  int c = P.slot_chain[slot];
+  bool drained = c == -2, have = false, active = false;
+  float PMMC_AS_GLOBAL *g = (float PMMC_AS_GLOBAL *)(it.pos) + lane;
+  ChainRegs R{};
+  float X[NB], Y[NB], Z[NB];
 
   for (int b = 0; b < max_blocks; ++b) {
-    if (active && static_cast<int64_t>(iters) >= S.max_moves) {
-      status = PMMC_STOP_CAP;
-      active = false;
+    // Every slot ends this loop running a chain, or with the queue drained:
+    // a stopped chain's state goes back and the slot takes the next chain.
+    for (;;) {
+      if (have && active && static_cast<int64_t>(R.iters) >= S.max_moves) {
+        R.status = PMMC_STOP_CAP;
+        active = false;
+      }
+      if (have && !active) {
+        chain_save(pe[c >> 6].st, c & (kLanes - 1), R);
+        have = false;
+        c = -1;
+      }
+      if (have || drained) break;
+      if (c < 0) {
+        const int t = atomicAdd(P.queue, 1);
+        if (t >= nch) {
+          drained = true;
+          break;
+        }
+        c = t;
+      }
+      // start (or resume) chain c
+      const DevEns ce = pe[c >> 6];
+      g = (float PMMC_AS_GLOBAL *)(ce.pos) + (c & (kLanes - 1));
+      chain_load(ce.st, c & (kLanes - 1), R);
+      have = true;
+      active = R.status == 0;
     }
-    const uint64_t am = __ballot(active);
-    if (am == 0) break;
-    const uint32_t m0 = readlane_u32(iters, __builtin_ctzll(am));
+    if (__ballot(active) == 0) break;
+    // the registers take the chain's positions from its memory copy (always
+    // current) at every block: no merge of old and new registers on a refill
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      X[j] = j < n ? g[(3 * j + 0) * kLanes] : 0.f;
+      Y[j] = j < n ? g[(3 * j + 1) * kLanes] : 0.f;
+      Z[j] = j < n ? g[(3 * j + 2) * kLanes] : 0.f;
+    }
+    const uint32_t m0 = R.iters;
+    // bead picks per chain (pick group = chain: the host sets pick_shift 0
+    // for every problem these kernels run)
+    const uint32_t chain = P.chain0 + static_cast<uint32_t>(c < 0 ? 0 : c);
+    const uint32_t group = chain;
     // The moved bead's old position comes from the SoA copy in memory (kept
     // current on every accepted move), read one move ahead: the pick stream
     // does not depend on the chains' state.
     U4 rbn = draw(seed, m0 >> 2, group, segment, PMMC_LEVEL_HEATMAP, kSubBead);
-    int pn = cst(P.free_list)[pick(pick_word(rbn, m0), static_cast<uint32_t>(nfree))];
+    int pn = free_bead(P, all_free, pick(pick_word(rbn, m0), static_cast<uint32_t>(nfree)));
     float qx = g[(3 * pn + 0) * kLanes], qy = g[(3 * pn + 1) * kLanes], qz = g[(3 * pn + 2) * kLanes];
+    double E = R.E, T = R.T;
+    const float step = R.step;
     for (int k = 0; k < S.block; ++k) {
       const uint32_t m = m0 + static_cast<uint32_t>(k);
       const int p = pn;
@@ -161,8 +252,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       const bool more = k + 1 < S.block;
       if (more) {
         const uint32_t mn = m + 1;
-        if ((mn & 3u) == 0) rbn = draw(seed, mn >> 2, group, segment, PMMC_LEVEL_HEATMAP, kSubBead);
-        pn = cst(P.free_list)[pick(pick_word(rbn, mn), static_cast<uint32_t>(nfree))];
+        // a chain's move count is a multiple of the block, so with blocks of a
+        // multiple of 4 moves the redraw is the same in every lane
+        if (aligned4 ? ((k + 1) & 3) == 0 : (mn & 3u) == 0)
+          rbn = draw(seed, mn >> 2, group, segment, PMMC_LEVEL_HEATMAP, kSubBead);
+        pn = free_bead(P, all_free, pick(pick_word(rbn, mn), static_cast<uint32_t>(nfree)));
         qx = g[(3 * pn + 0) * kLanes];
         qy = g[(3 * pn + 1) * kLanes];
         qz = g[(3 * pn + 2) * kLanes];
@@ -194,45 +288,37 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       qz = same ? nz : qz;
       if (ok) {
         E = En;
-        ++msucc;
-        ++acc;
-        best = E < best ? E : best;
+        ++R.msucc;
+        ++R.acc;
+        R.best = E < R.best ? E : R.best;
       }
       if (active && !S.cool_per_block) T *= S.dt;
     }
+    R.E = E;
+    R.T = T;
     bool due = false;
     if (active) {
-      iters += static_cast<uint32_t>(S.block);
-      if (S.cool_per_block) T *= S.dt;
-      step *= S.step_decay;
-      due = (iters % static_cast<uint32_t>(S.milestone_period)) < static_cast<uint32_t>(S.block);
+      R.iters += static_cast<uint32_t>(S.block);
+      if (S.cool_per_block) R.T *= S.dt;
+      R.step *= S.step_decay;
+      due = (R.iters % static_cast<uint32_t>(S.milestone_period)) < static_cast<uint32_t>(S.block);
     }
     if (__ballot(due) != 0) {
-      double Er = E;
-      const bool resum_now = S.resum > 0 && ((miles + 1) % S.resum) == 0;
+      double Er = R.E;
+      const bool resum_now = S.resum > 0 && ((R.miles + 1) % S.resum) == 0;
       if (__ballot(due && resum_now) != 0) {
-        Er = hreg_resum(P, it.pos, lane, r);
+        Er = hreg_resum(P, g, r);
       }
       if (due) {
-        ++miles;
-        E = Er;
-        milestone(S, E, ms, msucc, misses, status, active);
+        ++R.miles;
+        R.E = Er;
+        milestone(S, R.E, R.ms, R.msucc, R.misses, R.status, active);
       }
     }
   }
   // the memory copy of the positions is already current
-  if (status == -1) return;  // padding lane of a partial ensemble
-  st->E[lane] = E;
-  st->T[lane] = T;
-  st->ms[lane] = ms;
-  st->best[lane] = best;
-  st->step[lane] = step;
-  st->iters[lane] = iters;
-  st->acc[lane] = acc;
-  st->msucc[lane] = msucc;
-  st->misses[lane] = misses;
-  st->milestones[lane] = miles;
-  st->status[lane] = status;
+  if (have) chain_save(pe[c >> 6].st, c & (kLanes - 1), R);
+  P.slot_chain[slot] = have ? c : (drained ? -2 : -1);
 }
 
 // Lane-pair form for symmetric tables: a chain is held by two adjacent lanes,
@@ -281,59 +367,92 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
   extern __shared__ float lw[];  // 64 rows of kPairRow floats
   const DevEns it = ens[blockIdx.x];
   const DevProblem &P = probs[it.problem];
+  const int w = static_cast<int>((it.chain0 - P.chain0) >> 6);  // worker index
+  if (w >= P.workers) return;  // not a worker: its chains are run by the queue
   const int n = P.n;
   // the weight table first: no wave leaves before the barrier.  W rows are
   // 64 floats (P.ldw == 64: hreg_halves admits only such problems), zero
   // past column n; rows n..63 of the LDS table are zero.
   for (int i = threadIdx.x; i < 64 * kPairRow; i += 128) {
-    const int row = i / kPairRow, c = i % kPairRow;
-    const int col = c < 32 ? 2 * c : (c >= kPairOdd ? 2 * (c - kPairOdd) + 1 : -1);
+    const int row = i / kPairRow, cc = i % kPairRow;
+    const int col = cc < 32 ? 2 * cc : (cc >= kPairOdd ? 2 * (cc - kPairOdd) + 1 : -1);
     lw[i] = (row < n && col >= 0) ? P.W[row * 64 + col] : 0.f;
   }
   __syncthreads();
 
+  const DevEns *pe = ens + (static_cast<int>(blockIdx.x) - w);  // the problem's ensembles
   const int half = threadIdx.x & 1;
-  const int cl = threadIdx.x >> 1;  // chain within the ensemble, 0..63
-  EnsState *__restrict__ st = it.st;
-  int status = st->status[cl];
-  if (__ballot(status == 0) == 0) return;
-
-  float PMMC_AS_GLOBAL *g = (float PMMC_AS_GLOBAL *)(it.pos) + cl;
-  float X[32], Y[32], Z[32];
-#pragma unroll
-  for (int i = 0; i < 32; ++i) {
-    const int j = 2 * i + half;
-    X[i] = j < n ? g[(3 * j + 0) * kLanes] : 0.f;
-    Y[i] = j < n ? g[(3 * j + 1) * kLanes] : 0.f;
-    Z[i] = j < n ? g[(3 * j + 2) * kLanes] : 0.f;
-  }
-
+  const int cl = threadIdx.x >> 1;  // slot within the worker, 0..63
+  const int wl = threadIdx.x & (kLanes - 1);  // lane within the wavefront
+  const int slot = w * kLanes + cl;
   const pmmc_schedule S = P.sched;
   const uint64_t seed = P.seed;
   const uint32_t segment = P.segment;
   const int nfree = P.nfree;
   const float r = S.sphere_radius;
+  const int nch = P.n_chains;
+  const bool all_free = nfree == n;
+  const bool aligned4 = (S.block & 3) == 0;
   const float *lhalf = lw + (half ? kPairOdd : 0);
 
-  double E = st->E[cl], T = st->T[cl], ms = st->ms[cl], best = st->best[cl];
-  float step = st->step[cl];
-  uint32_t iters = st->iters[cl], acc = st->acc[cl];
-  int msucc = st->msucc[cl], misses = st->misses[cl], miles = st->milestones[cl];
-  bool active = (status == 0);
-  const uint32_t chain = it.chain0 + static_cast<uint32_t>(cl);
-  const uint32_t group = it.chain0 >> kPickShift;
+  // The slot's chain (both lanes of a pair run the same one): memory copy of
+  // its positions (global address space) and its state.  Lanes without a
+  // chain keep a valid address (their own ensemble slot) and never store.
+  int c = P.slot_chain[slot];
+  bool drained = c == -2, have = false, active = false;
+  float PMMC_AS_GLOBAL *g = (float PMMC_AS_GLOBAL *)(it.pos) + cl;
+  ChainRegs R{};
+  float X[32], Y[32], Z[32];
 
   for (int b = 0; b < max_blocks; ++b) {
-    if (active && static_cast<int64_t>(iters) >= S.max_moves) {
-      status = PMMC_STOP_CAP;
-      active = false;
+    // Every slot ends this loop running a chain, or with the queue drained;
+    // the two lanes of a pair take the same path.
+    for (;;) {
+      if (have && active && static_cast<int64_t>(R.iters) >= S.max_moves) {
+        R.status = PMMC_STOP_CAP;
+        active = false;
+      }
+      if (have && !active) {
+        if (half == 0) chain_save(pe[c >> 6].st, c & (kLanes - 1), R);
+        have = false;
+        c = -1;
+      }
+      if (have || drained) break;
+      if (c < 0) {
+        int t = 0;
+        if (half == 0) t = atomicAdd(P.queue, 1);
+        t = __shfl(t, wl & ~1, kLanes);
+        if (t >= nch) {
+          drained = true;
+          break;
+        }
+        c = t;
+      }
+      const DevEns ce = pe[c >> 6];
+      g = (float PMMC_AS_GLOBAL *)(ce.pos) + (c & (kLanes - 1));
+      chain_load(ce.st, c & (kLanes - 1), R);
+      have = true;
+      active = R.status == 0;
     }
-    const uint64_t am = __ballot(active);
-    if (am == 0) break;
-    const uint32_t m0 = readlane_u32(iters, __builtin_ctzll(am));
+    if (__ballot(active) == 0) break;
+    // registers from the chain's memory copy (always current), every block
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+      const int j = 2 * i + half;
+      X[i] = j < n ? g[(3 * j + 0) * kLanes] : 0.f;
+      Y[i] = j < n ? g[(3 * j + 1) * kLanes] : 0.f;
+      Z[i] = j < n ? g[(3 * j + 2) * kLanes] : 0.f;
+    }
+    const uint32_t m0 = R.iters;
+    // bead picks per chain (pick group = chain: the host sets pick_shift 0
+    // for every problem these kernels run)
+    const uint32_t chain = P.chain0 + static_cast<uint32_t>(c < 0 ? 0 : c);
+    const uint32_t group = chain;
     U4 rbn = draw(seed, m0 >> 2, group, segment, PMMC_LEVEL_HEATMAP, kSubBead);
-    int pn = cst(P.free_list)[pick(pick_word(rbn, m0), static_cast<uint32_t>(nfree))];
+    int pn = free_bead(P, all_free, pick(pick_word(rbn, m0), static_cast<uint32_t>(nfree)));
     float qx = g[(3 * pn + 0) * kLanes], qy = g[(3 * pn + 1) * kLanes], qz = g[(3 * pn + 2) * kLanes];
+    double E = R.E, T = R.T;
+    const float step = R.step;
     for (int k = 0; k < S.block; ++k) {
       const uint32_t m = m0 + static_cast<uint32_t>(k);
       const int p = pn;
@@ -341,8 +460,11 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
       const bool more = k + 1 < S.block;
       if (more) {
         const uint32_t mn = m + 1;
-        if ((mn & 3u) == 0) rbn = draw(seed, mn >> 2, group, segment, PMMC_LEVEL_HEATMAP, kSubBead);
-        pn = cst(P.free_list)[pick(pick_word(rbn, mn), static_cast<uint32_t>(nfree))];
+        // a chain's move count is a multiple of the block, so with blocks of a
+        // multiple of 4 moves the redraw is the same in every lane
+        if (aligned4 ? ((k + 1) & 3) == 0 : (mn & 3u) == 0)
+          rbn = draw(seed, mn >> 2, group, segment, PMMC_LEVEL_HEATMAP, kSubBead);
+        pn = free_bead(P, all_free, pick(pick_word(rbn, mn), static_cast<uint32_t>(nfree)));
         qx = g[(3 * pn + 0) * kLanes];
         qy = g[(3 * pn + 1) * kLanes];
         qz = g[(3 * pn + 2) * kLanes];
@@ -375,44 +497,37 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
       qz = same ? nz : qz;
       if (ok) {
         E = En;
-        ++msucc;
-        ++acc;
-        best = E < best ? E : best;
+        ++R.msucc;
+        ++R.acc;
+        R.best = E < R.best ? E : R.best;
       }
       if (active && !S.cool_per_block) T *= S.dt;
     }
+    R.E = E;
+    R.T = T;
     bool due = false;
     if (active) {
-      iters += static_cast<uint32_t>(S.block);
-      if (S.cool_per_block) T *= S.dt;
-      step *= S.step_decay;
-      due = (iters % static_cast<uint32_t>(S.milestone_period)) < static_cast<uint32_t>(S.block);
+      R.iters += static_cast<uint32_t>(S.block);
+      if (S.cool_per_block) R.T *= S.dt;
+      R.step *= S.step_decay;
+      due = (R.iters % static_cast<uint32_t>(S.milestone_period)) < static_cast<uint32_t>(S.block);
     }
     if (__ballot(due) != 0) {
-      double Er = E;
-      const bool resum_now = S.resum > 0 && ((miles + 1) % S.resum) == 0;
+      double Er = R.E;
+      const bool resum_now = S.resum > 0 && ((R.miles + 1) % S.resum) == 0;
       if (__ballot(due && resum_now) != 0) {
-        Er = hreg_resum(P, it.pos, cl, r);
+        Er = hreg_resum(P, g, r);
       }
       if (due) {
-        ++miles;
-        E = Er;
-        milestone(S, E, ms, msucc, misses, status, active);
+        ++R.miles;
+        R.E = Er;
+        milestone(S, R.E, R.ms, R.msucc, R.misses, R.status, active);
       }
     }
   }
-  if (status == -1 || half != 0) return;  // padding lane, or the pair's second lane
-  st->E[cl] = E;
-  st->T[cl] = T;
-  st->ms[cl] = ms;
-  st->best[cl] = best;
-  st->step[cl] = step;
-  st->iters[cl] = iters;
-  st->acc[cl] = acc;
-  st->msucc[cl] = msucc;
-  st->misses[cl] = misses;
-  st->milestones[cl] = miles;
-  st->status[cl] = status;
+  if (half != 0) return;  // the pair's second lane
+  if (have) chain_save(pe[c >> 6].st, c & (kLanes - 1), R);
+  P.slot_chain[slot] = have ? c : (drained ? -2 : -1);
 }
 
 // heatmaps of 33..64 beads (weight rows padded to 64).  Up to 32 beads the

@@ -97,3 +97,14 @@ def test_presets_follow_reference_constants():
     assert sg.stop_rule == 2
     with pytest.raises(Exception):
         schedule_preset(7, 0, s, 1.0)
+
+
+def test_pick_shift_rule_matches_oracle():
+    from oracle import binding as O
+    from pmmc_amd import api
+    for level in (1, 2, 3):
+        for n in (1, 2, 31, 32, 33, 48, 64, 65, 400):
+            P = O.Problem.__new__(O.Problem)
+            P.level, P.n = level, n
+            assert api.pick_shift(level, n) == P.pick_shift(), (level, n)
+    assert api.pick_shift(1, 64) == 0 and api.pick_shift(1, 65) == 6 and api.pick_shift(2, 48) == 6
