@@ -327,6 +327,9 @@ def singleton_leg(ctx, with_cpu=True, cpu_seconds=8.0):
 
 LEGS = {"heatmap_64": heatmap_leg, "heatmap_64_sweep": heatmap_sweep_leg,
         "smooth_19951": smooth_leg, "singleton_20M": singleton_leg}
+# legs runnable alone with --only (not part of `levels`)
+ONLY = dict(LEGS, batched=lambda ctx, with_cpu=True: batched(ctx),
+            served=lambda ctx, with_cpu=True: served(ctx))
 
 
 def levels(ctx, with_cpu=True, cpu_seconds=8.0):
@@ -342,13 +345,24 @@ def batched(ctx, chains_per_problem=256, plain_subset=16):
     probs, meta = synth.batch_arcs_problems(n_segments=32, n_seeds=16, seed=0)
     sched = schedule_preset(2, 1, s, 0.03)
     n = len(probs)
-    # batched
+    seeds = [1 + i for i in range(n)]
+    # batched: launch shape chosen by the library's tuner (chains per
+    # wavefront, blocks per step, steps per graph), timed with its probes
+    shape = {}
     t0 = time.perf_counter()
-    res = ctx.anneal_batch(probs, sched, seeds=[1 + i for i in range(n)], n_chains=chains_per_problem,
-                           use_graph=True)
+    res = ctx.anneal_batch(probs, sched, seeds=seeds, n_chains=chains_per_problem, use_graph=True,
+                           opts_out=shape)
     tb = time.perf_counter() - t0
     mv_b = sum(r[1]["moves"] for r in res)
     ac_b = sum(r[1]["accepted"] for r in res)
+    # the same batch with one lane per chain at the chosen step shape
+    t0 = time.perf_counter()
+    res64 = ctx.anneal_batch(probs, sched, seeds=seeds, n_chains=chains_per_problem, use_graph=True,
+                             blocks_per_step=shape["blocks_per_step"],
+                             steps_per_graph=shape["steps_per_graph"], chains_per_wave=64)
+    t64 = time.perf_counter() - t0
+    same_shape = all(np.array_equal(a[0], b[0]) and a[1]["energy"] == b[1]["energy"]
+                     for a, b in zip(res, res64))
     # plain, one problem at a time, on an evenly spread subset
     idx = list(range(0, n, max(1, n // plain_subset)))[:plain_subset]
     t0 = time.perf_counter()
@@ -378,6 +392,10 @@ def batched(ctx, chains_per_problem=256, plain_subset=16):
     sub_moves_b = sum(res[i][1]["moves"] for i in idx)
     return {
         "problems": n, "chains_per_problem": chains_per_problem,
+        "launch_shape": {k: shape[k] for k in ("chains_per_wave", "blocks_per_step",
+                                                 "steps_per_graph", "tuned")},
+        "one_lane_per_chain_moves_per_s": sum(r[1]["moves"] for r in res64) / t64,
+        "results_identical_across_launch_shapes": same_shape,
         "anchors_min_max": [int(min(m[2] for m in meta)), int(max(m[2] for m in meta))],
         "batched_solves_per_s": n / tb, "batched_moves_per_s": mv_b / tb,
         "batched_accepted_moves_per_s": ac_b / tb,
@@ -526,7 +544,7 @@ def main(argv=None):
     ap.add_argument("--no-levels", action="store_true")
     ap.add_argument("--probe", action="store_true",
                     help="only measure the move kernels' access-pattern bandwidth ceilings")
-    ap.add_argument("--only", choices=sorted(LEGS),
+    ap.add_argument("--only", choices=sorted(ONLY),
                     help="run one level workload alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB three times (device to device), the known-size "
@@ -546,7 +564,7 @@ def main(argv=None):
 
     if a.only:
         with Context(0) as ctx:
-            print(json.dumps({a.only: LEGS[a.only](ctx, with_cpu=not a.no_cpu)}))
+            print(json.dumps({a.only: ONLY[a.only](ctx, with_cpu=not a.no_cpu)}))
         return 0
 
     prob, s, step, arcs = workload()

@@ -416,12 +416,23 @@ typedef struct {
   int32_t blocks_per_step;   /* blocks of moves per step launch (0 = autotune)  */
   int32_t steps_per_graph;   /* step launches captured per hipGraph (0 = auto)  */
   int32_t use_graph;         /* 1: replay captured step loops                  */
+  int32_t chains_per_wave;   /* arc level: 64 (a lane per chain) or 32 (a lane
+                                pair per chain: twice the wavefronts for the
+                                same chains); 0 = autotune.  Results do not
+                                depend on it.                                  */
+  int32_t tuned;             /* out: 1 when a field above was chosen by timing  */
 } pmmc_batch_opts;
 
 /* Run all jobs concurrently (one device-resident chain pool, step launches
- * replayed from a captured hipGraph until every chain has stopped). */
+ * replayed from a captured hipGraph until every chain has stopped).  Fields
+ * of *opts left 0 are chosen by an autotuner that times the first steps of
+ * the run itself (each candidate launch shape runs one short step of about
+ * 1,024 moves per chain; the chains advance meanwhile, so no work is
+ * repeated): chains_per_wave by the higher moves/s, blocks_per_step for
+ * about 20 ms of device time per step, steps_per_graph for about 100 ms per
+ * graph replay; the values used are written back to *opts. */
 pmmc_status pmmc_anneal_batch(pmmc_ctx *ctx, const pmmc_job *jobs, int32_t n_jobs,
-                              const pmmc_batch_opts *opts);
+                              pmmc_batch_opts *opts);
 
 /* ---------------- served form (continuous batching) ---------------- */
 typedef struct pmmc_server pmmc_server;

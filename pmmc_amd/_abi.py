@@ -119,7 +119,10 @@ class Job(C.Structure):
 
 class BatchOpts(C.Structure):
     _fields_ = [("blocks_per_step", C.c_int32), ("steps_per_graph", C.c_int32),
-                ("use_graph", C.c_int32)]
+                ("use_graph", C.c_int32), ("chains_per_wave", C.c_int32), ("tuned", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class SweepSchedule(C.Structure):

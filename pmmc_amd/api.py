@@ -346,10 +346,12 @@ class Context:
 
     # ---- batched ----
     def anneal_batch(self, problems, schedules, seeds, segments=None, n_chains=0,
-                     blocks_per_step=0, steps_per_graph=0, use_graph=True, chain0s=None):
+                     blocks_per_step=0, steps_per_graph=0, use_graph=True, chain0s=None,
+                     chains_per_wave=0, opts_out=None):
         """Run many problems concurrently; returns list of (pos, stats).
         Job i uses seed seeds[i], segment segments[i] (default i) and chain ids
-        chain0s[i] (default 0) onwards."""
+        chain0s[i] (default 0) onwards.  Launch-shape fields left 0 are
+        autotuned; opts_out (a dict, optional) receives the values used."""
         k = len(problems)
         jobs = (A.Job * k)()
         keep, outs, stats = [], [], []
@@ -367,8 +369,11 @@ class Context:
                             A.Run(int(seeds[i]), int(seg), 0 if chain0s is None else int(chain0s[i]),
                                   int(nc)),
                             o.ctypes.data_as(A.f32p), C.pointer(s))
-        opts = A.BatchOpts(blocks_per_step, steps_per_graph, 1 if use_graph else 0)
+        opts = A.BatchOpts(blocks_per_step, steps_per_graph, 1 if use_graph else 0,
+                           chains_per_wave, 0)
         self._check(self.lib.pmmc_anneal_batch(self.h, jobs, k, C.byref(opts)))
+        if opts_out is not None:
+            opts_out.update(opts.as_dict())
         return [(outs[i], stats[i].as_dict()) for i in range(k)]
 
 

@@ -31,7 +31,8 @@ extern "C" pmmc_status pmmc_ctx_create(int device, pmmc_ctx **out) {
   c->cu_count = prop.multiProcessorCount;
   c->hbm_bytes = prop.totalGlobalMem;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipEventCreate(&c->ev2) != hipSuccess) {
     delete c;
     return PMMC_ERR_HIP;
   }
@@ -49,6 +50,7 @@ extern "C" void pmmc_ctx_destroy(pmmc_ctx *c) {
   if (c->arena) hipFree(c->arena);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->ev2) hipEventDestroy(c->ev2);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -371,15 +373,6 @@ pmmc_status prepare_problem(pmmc_ctx *ctx, int level, const void *problem, const
     h.workers = std::min(n_ens, std::max(1, resident_ensembles_per_cu(h) * cus));
   }
   return PMMC_OK;
-}
-
-// Blocks of moves per step launch: about 8k moves per chain per step, so a
-// step is long enough to amortise the launch yet short enough for admission.
-int autotune_blocks_per_step(const std::vector<HostProblem> &hp) {
-  int blk = 1;
-  for (const auto &h : hp) blk = std::max(blk, h.sched.block);
-  const int b = 8192 / blk;
-  return b < 1 ? 1 : b;
 }
 
 }  // namespace pmmc

@@ -24,7 +24,7 @@ struct pmmc_ctx {
   char *arena = nullptr;
   size_t arena_cap = 0;
   size_t arena_top = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   // the device's static scratch array (pmmc_anneal_noalloc): its device
   // address, looked up when the context is created, or null with the reason
   // in scratch_error

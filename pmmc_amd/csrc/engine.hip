@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ctx.hpp"
@@ -475,6 +476,20 @@ __global__ __launch_bounds__(64) void k_zero_int(int *p) {
   if (threadIdx.x == 0) *p = 0;
 }
 
+// Moves done so far, summed over every chain (for the launch-shape tuner).
+__global__ __launch_bounds__(64) void k_sum_iters(const DevEns *__restrict__ ens, int n_ens,
+                                                  unsigned long long *sum) {
+  unsigned long long c = 0;
+  for (int e = blockIdx.x; e < n_ens; e += gridDim.x)
+    if (threadIdx.x < ens[e].nlanes) c += ens[e].st->iters[threadIdx.x];
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  if (threadIdx.x == 0 && c) atomicAdd(sum, c);
+}
+
+__global__ __launch_bounds__(64) void k_zero_u64(unsigned long long *p) {
+  if (threadIdx.x == 0) *p = 0;
+}
+
 __global__ __launch_bounds__(64) void k_zero_n(int *p, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
@@ -698,7 +713,10 @@ hipError_t launch_groups(hipStream_t s, const DevProblem *probs, const DevEns *e
           err = launch_level<PMMC_LEVEL_HEATMAP>(s, probs, e, g.n_ens, max_blocks, g.stage, g.n);
         break;
       case PMMC_LEVEL_ARCS:
-        err = launch_level<PMMC_LEVEL_ARCS>(s, probs, e, g.n_ens, max_blocks, g.stage, g.n);
+        if (g.cpw == 32 && !g.stage)
+          err = launch_arcs_pair(s, probs, e, g.n_ens, max_blocks);
+        else
+          err = launch_level<PMMC_LEVEL_ARCS>(s, probs, e, g.n_ens, max_blocks, g.stage, g.n);
         break;
       default:
         err = launch_level<PMMC_LEVEL_SMOOTH>(s, probs, e, g.n_ens, max_blocks, g.stage, g.n);
@@ -744,7 +762,11 @@ hipError_t launch_finalize(hipStream_t s, DevProblem *probs, int n_probs, const 
   return hipGetLastError();
 }
 
-static void size_problem(Sizer &z, const HostProblem &h) {
+// A problem's device data in two parts: the static tables the host writes
+// (carved contiguously, so they go up in one copy) and the device-written
+// state (initial energy, ensembles, chain queue).  The result positions
+// out_pos are carved by the caller.
+static void size_static(Sizer &z, const HostProblem &h) {
   z.add<int32_t>(std::max<size_t>(1, h.free_list.size()));
   z.add<float>(h.pos0.size());
   z.add<float>(h.W.size());
@@ -759,8 +781,10 @@ static void size_problem(Sizer &z, const HostProblem &h) {
   z.add<PairEntry>(std::max<size_t>(1, h.pairs.size()));
   z.add<int32_t>(std::max<size_t>(1, h.zpairs.size()));
   z.add<float>(h.dtn.size());
+}
+
+static void size_dynamic(Sizer &z, const HostProblem &h) {
   z.add<double>(2);
-  z.add<float>(h.pos0.size());  // out_pos
   const int ne = (h.n_chains + kLanes - 1) / kLanes;
   z.add<EnsState>(ne);
   z.add<float>(static_cast<size_t>(ne) * h.pos0.size() * kLanes);
@@ -770,49 +794,62 @@ static void size_problem(Sizer &z, const HostProblem &h) {
   }
 }
 
-template <class T>
-static T *put(Carver &cv, hipStream_t s, const std::vector<T> &v, hipError_t &err, size_t min_count = 0) {
-  T *p = cv.take<T>(std::max(v.size(), min_count));
-  if (!v.empty() && err == hipSuccess)
-    err = hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s);
-  return p;
-}
-
 size_t problem_bytes(const HostProblem &h) {
   Sizer z;
-  size_problem(z, h);
+  size_static(z, h);
+  size_dynamic(z, h);
+  z.add<float>(h.pos0.size());  // out_pos
   return z.top + 256;
 }
 
-// Upload one problem into carved device memory; append its ensembles (with
-// problem index `index`) to `ens`.
-void fill_problem(Carver &cv, hipStream_t s, const HostProblem &h, int index, DevResult *result,
-                  DevProblem &P, std::vector<DevEns> &ens, hipError_t &err) {
-  std::memset(&P, 0, sizeof(P));
+// Carves device addresses and writes the data at the same offsets of a host
+// staging buffer (both regions 256-byte aligned, offsets as Sizer's).
+struct StageCarver {
+  char *dev;
+  char *host;
+  size_t top = 0;
+  template <class T>
+  T *put(const std::vector<T> &v, size_t min_count = 0) {
+    top = (top + 255) & ~size_t(255);
+    T *d = reinterpret_cast<T *>(dev + top);
+    if (!v.empty()) std::memcpy(host + top, v.data(), sizeof(T) * v.size());
+    top += sizeof(T) * std::max(v.size(), min_count);
+    return d;
+  }
+};
+
+static void fill_static(StageCarver &sc, const HostProblem &h, DevProblem &P) {
+  P.free_list = sc.put(h.free_list, 1);
+  P.pos0 = sc.put(h.pos0);
+  P.W = sc.put(h.W);
+  P.W2 = h.W2.empty() ? nullptr : sc.put(h.W2);
+  if (h.W2.empty()) sc.put(h.W2);
+  P.Hc = sc.put(h.Hc);
+  P.off = sc.put(h.off);
+  P.ent = sc.put(h.ent, 1);
+  P.ej = sc.put(h.ej, 1);
+  P.ec = sc.put(h.ec, 1);
+  P.eka = sc.put(h.eka, 1);
+  P.ekb = sc.put(h.ekb, 1);
+  P.pairs = sc.put(h.pairs, 1);
+  P.zpairs = sc.put(h.zpairs, 1);
+  P.dtn = sc.put(h.dtn);
+}
+
+// Scalars, device-written state and ensembles of one problem (problem index
+// `index` in the ensembles appended to `ens`).
+static void fill_dynamic(Carver &cv, const HostProblem &h, int index, DevResult *result,
+                         float *out_pos, DevProblem &P, std::vector<DevEns> &ens) {
   P.level = h.level;
   P.n = h.n;
   P.nfree = static_cast<int32_t>(h.free_list.size());
   P.symmetric = h.symmetric ? 1 : 0;
   P.ldw = h.ldw > 0 ? h.ldw : h.n;
-  P.free_list = put(cv, s, h.free_list, err, 1);
-  P.pos0 = put(cv, s, h.pos0, err);
-  P.W = put(cv, s, h.W, err);
-  P.W2 = h.W2.empty() ? nullptr : put(cv, s, h.W2, err);
-  P.Hc = put(cv, s, h.Hc, err);
-  P.off = put(cv, s, h.off, err);
-  P.ent = put(cv, s, h.ent, err, 1);
-  P.ej = put(cv, s, h.ej, err, 1);
-  P.ec = put(cv, s, h.ec, err, 1);
-  P.eka = put(cv, s, h.eka, err, 1);
-  P.ekb = put(cv, s, h.ekb, err, 1);
-  P.pairs = put(cv, s, h.pairs, err, 1);
   P.npairs = static_cast<int32_t>(h.pairs.size());
-  P.zpairs = put(cv, s, h.zpairs, err, 1);
   P.nnz = static_cast<int32_t>(h.ent.size());
   P.nz = static_cast<int32_t>(h.zpairs.size() / 2);
   P.ks = h.ks;
   P.kq = h.kq;
-  P.dtn = put(cv, s, h.dtn, err);
   P.ka = h.ka;
   P.wd = h.wd;
   P.wa = h.wa;
@@ -824,7 +861,7 @@ void fill_problem(Carver &cv, hipStream_t s, const HostProblem &h, int index, De
   P.segment = h.segment;
   P.chain0 = h.chain0;
   P.n_chains = h.n_chains;
-  P.out_pos = cv.take<float>(h.pos0.size());
+  P.out_pos = out_pos;
   P.result = result;
   const int ne = (h.n_chains + kLanes - 1) / kLanes;
   EnsState *sts = cv.take<EnsState>(ne);
@@ -849,26 +886,68 @@ void fill_problem(Carver &cv, hipStream_t s, const HostProblem &h, int index, De
   }
 }
 
-// Device bytes batch_setup carves for these problems.
+// Upload one problem into carved device memory (its static tables staged in
+// `stage`, which must stay alive until the stream has synchronised); append
+// its ensembles (with problem index `index`) to `ens`.
+void fill_problem(Carver &cv, hipStream_t s, const HostProblem &h, int index, DevResult *result,
+                  DevProblem &P, std::vector<DevEns> &ens, hipError_t &err, std::vector<char> &stage) {
+  std::memset(&P, 0, sizeof(P));
+  Sizer zs;
+  size_static(zs, h);
+  stage.assign(zs.top, 0);
+  char *sdev = cv.take<char>(zs.top);
+  StageCarver sc{sdev, stage.data()};
+  fill_static(sc, h, P);
+  if (err == hipSuccess && zs.top > 0)
+    err = hipMemcpyAsync(sdev, stage.data(), zs.top, hipMemcpyHostToDevice, s);
+  float *out_pos = cv.take<float>(h.pos0.size());
+  fill_dynamic(cv, h, index, result, out_pos, P, ens);
+}
+
+// Bytes of a problem's static-table block and of its result positions,
+// each block starting 256-byte aligned (so a region's size does not depend
+// on the order of its problems).
+static size_t static_block(const HostProblem &h) {
+  Sizer z;
+  size_static(z, h);
+  return (z.top + 255) & ~size_t(255);
+}
+
+static size_t out_block(const HostProblem &h) {
+  return (sizeof(float) * h.pos0.size() + 255) & ~size_t(255);
+}
+
+// Device bytes batch_setup carves for these problems, in its order: problem
+// and result arrays, counters, ensembles, every problem's static tables,
+// every problem's result positions, every problem's device-written state.
 size_t batch_bytes(const std::vector<HostProblem> &hp) {
   Sizer z;
   z.add<DevProblem>(hp.size());
   z.add<DevResult>(hp.size());
   z.add<int>(64);
-  size_t n_ens = 0;
+  size_t n_ens = 0, sb = 0, ob = 0;
   for (auto &h : hp) {
-    size_problem(z, h);
     n_ens += (h.n_chains + kLanes - 1) / kLanes;
+    sb += static_block(h);
+    ob += out_block(h);
   }
   z.add<DevEns>(n_ens);
+  z.add<char>(sb);
+  z.add<char>(ob);
+  for (auto &h : hp) size_dynamic(z, h);
   return z.top + 4096;
 }
 
 // Carves the batch from `span` when it has a base (the caller's device
-// memory, nothing is allocated), else from the context's arena.
+// memory, nothing is allocated), else from the context's arena.  The static
+// tables of every problem go up in one copy.
 pmmc_status batch_setup(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &bd, DevSpan span) {
-  size_t n_ens = 0;
-  for (auto &h : hp) n_ens += (h.n_chains + kLanes - 1) / kLanes;
+  size_t n_ens = 0, sb = 0, ob = 0;
+  for (auto &h : hp) {
+    n_ens += (h.n_chains + kLanes - 1) / kLanes;
+    sb += static_block(h);
+    ob += out_block(h);
+  }
   const size_t need = batch_bytes(hp);
   if (span.base) {
     if (need > span.cap)
@@ -888,6 +967,13 @@ pmmc_status batch_setup(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &b
   bd.results = cv.take<DevResult>(hp.size());
   bd.counter = cv.take<int>(64);
   bd.ens = cv.take<DevEns>(n_ens);
+  char *sdev = cv.take<char>(sb);
+  bd.outs = cv.take<char>(ob);
+  bd.outs_bytes = ob;
+  bd.stage.assign(sb, 0);
+  StageCarver sc{sdev, bd.stage.data()};
+  size_t otop = 0;
+  bd.out_off.assign(hp.size(), 0);
   bd.hprobs.resize(hp.size());
   bd.hens.clear();
   bd.groups.clear();
@@ -904,10 +990,19 @@ pmmc_status batch_setup(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &b
       bd.groups.back().ens0 = static_cast<int>(bd.hens.size());
     }
     const size_t before = bd.hens.size();
-    fill_problem(cv, s, h, oi, bd.results + oi, bd.hprobs[oi], bd.hens, err);
+    DevProblem &P = bd.hprobs[oi];
+    std::memset(&P, 0, sizeof(P));
+    sc.top = (sc.top + 255) & ~size_t(255);
+    fill_static(sc, h, P);
+    bd.out_off[oi] = otop;
+    float *out_pos = reinterpret_cast<float *>(bd.outs + otop);
+    otop += out_block(h);
+    fill_dynamic(cv, h, oi, bd.results + oi, out_pos, P, bd.hens);
     bd.groups.back().n_ens += static_cast<int>(bd.hens.size() - before);
   }
   bd.n_ens_total = static_cast<int>(bd.hens.size());
+  // one upload of every problem's static tables, then the descriptors
+  if (sb > 0) err = hipMemcpyAsync(sdev, bd.stage.data(), sb, hipMemcpyHostToDevice, s);
   if (err == hipSuccess)
     err = hipMemcpyAsync(bd.probs, bd.hprobs.data(), sizeof(DevProblem) * hp.size(),
                          hipMemcpyHostToDevice, s);
@@ -943,23 +1038,22 @@ pmmc_status batch_finish(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &
   std::vector<DevResult> res(hp.size());
   if (err == hipSuccess)
     err = hipMemcpyAsync(res.data(), bd.results, sizeof(DevResult) * hp.size(), hipMemcpyDeviceToHost, s);
-  std::vector<std::vector<float>> tmp(hp.size());
-  for (size_t i = 0; i < hp.size() && err == hipSuccess; ++i) {
-    tmp[i].resize(hp[i].pos0.size());
-    err = hipMemcpyAsync(tmp[i].data(), bd.hprobs[i].out_pos, sizeof(float) * tmp[i].size(),
-                         hipMemcpyDeviceToHost, s);
-  }
+  // every problem's result positions come back in one copy
+  std::vector<char> tmp(bd.outs_bytes);
+  if (err == hipSuccess && bd.outs_bytes > 0)
+    err = hipMemcpyAsync(tmp.data(), bd.outs, bd.outs_bytes, hipMemcpyDeviceToHost, s);
   if (err == hipSuccess) err = hipStreamSynchronize(s);
   if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("finalize: ") + hipGetErrorString(err));
   for (size_t i = 0; i < hp.size(); ++i) {
+    const float *res_pos = reinterpret_cast<const float *>(tmp.data() + bd.out_off[i]);
     if (outs[i]) {
       if (hp[i].level == PMMC_LEVEL_SMOOTH) {
         // only non-fixed beads are written back (ParallelMonteCarloSmooth.cu:485-491)
         for (int b = 0; b < hp[i].n; ++b)
           if (!hp[i].fixed[b])
-            for (int k = 0; k < 3; ++k) outs[i][3 * b + k] = tmp[i][3 * b + k];
+            for (int k = 0; k < 3; ++k) outs[i][3 * b + k] = res_pos[3 * b + k];
       } else {
-        std::memcpy(outs[i], tmp[i].data(), sizeof(float) * tmp[i].size());
+        std::memcpy(outs[i], res_pos, sizeof(float) * hp[i].pos0.size());
       }
     }
     if (stats[i]) {
@@ -982,20 +1076,96 @@ pmmc_status batch_finish(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &
   return PMMC_OK;
 }
 
-// Run a set of prepared problems to completion.  blocks_per_launch <= 0:
-// a single launch per level group that runs every chain to its end.
+// Moves summed over every chain of the batch (synchronises the stream).
+static hipError_t batch_moves(pmmc_ctx *ctx, const BatchDev &bd, unsigned long long *host) {
+  unsigned long long *d = reinterpret_cast<unsigned long long *>(bd.counter + 2);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, d);
+  const int grid = std::max(1, std::min(bd.n_ens_total, 1024));
+  hipLaunchKernelGGL(k_sum_iters, dim3(grid), dim3(64), 0, ctx->stream, bd.ens, bd.n_ens_total, d);
+  hipError_t err = hipMemcpyAsync(host, d, sizeof(*host), hipMemcpyDeviceToHost, ctx->stream);
+  if (err != hipSuccess) return err;
+  return hipStreamSynchronize(ctx->stream);
+}
+
+static void set_cpw(BatchDev &bd, int cpw) {
+  for (auto &g : bd.groups)
+    if (g.level == PMMC_LEVEL_ARCS) g.cpw = cpw;
+}
+
+// Launch-shape tuner: fields of `sh` that are 0 are chosen by timing short
+// steps of the run itself (the chains advance; nothing is repeated).
+static pmmc_status tune_shape(pmmc_ctx *ctx, BatchDev &bd, BatchShape &sh, int max_block) {
+  hipStream_t s = ctx->stream;
+  const bool has_arcs = std::any_of(bd.groups.begin(), bd.groups.end(),
+                                    [](const LaunchGroup &g) { return g.level == PMMC_LEVEL_ARCS && !g.stage; });
+  std::vector<int> cands;
+  if (sh.cpw > 0 || !has_arcs) cands.push_back(sh.cpw > 0 ? sh.cpw : 64);
+  else cands = {64, 32};
+  // a probe step of about 1,024 moves per chain
+  const int probe = std::max(1, 1024 / std::max(1, max_block));
+  double best_rate = -1.0, best_ms = 0.0;
+  int best_cpw = cands[0];
+  unsigned long long before = 0, after = 0;
+  hipError_t err = batch_moves(ctx, bd, &before);
+  for (int cpw : cands) {
+    if (err != hipSuccess) break;
+    set_cpw(bd, cpw);
+    PMMC_HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
+    err = batch_step(ctx, bd, probe);
+    if (err != hipSuccess) break;
+    PMMC_HIP_TRY(ctx, hipEventRecord(ctx->ev2, s));
+    err = batch_moves(ctx, bd, &after);
+    if (err != hipSuccess) break;
+    float ms = 0.f;
+    PMMC_HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev1, ctx->ev2));
+    const double rate = ms > 0.f ? static_cast<double>(after - before) / ms : 0.0;
+    if (rate > best_rate) {
+      best_rate = rate;
+      best_cpw = cpw;
+      best_ms = ms;
+    }
+    before = after;
+  }
+  if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("tune: ") + hipGetErrorString(err));
+  if (sh.cpw <= 0) sh.cpw = best_cpw;
+  set_cpw(bd, sh.cpw);
+  const double per_block_ms = best_ms > 0.0 ? best_ms / probe : 0.0;
+  if (sh.bps <= 0)
+    sh.bps = per_block_ms > 0.0 ? static_cast<int>(std::min(1e6, std::max(1.0, 20.0 / per_block_ms)))
+                                : std::max(1, 8192 / std::max(1, max_block));
+  if (sh.spg <= 0)
+    sh.spg = per_block_ms > 0.0
+                 ? static_cast<int>(std::min(64.0, std::max(1.0, 100.0 / (per_block_ms * sh.bps))))
+                 : 8;
+  sh.tuned = 1;
+  return PMMC_OK;
+}
+
+// Run a set of prepared problems to completion.  sh.bps <= 0 with sh.tune
+// off: a single launch per level group that runs every chain to its end.
+// Otherwise step launches of sh.bps blocks, sh.spg of them captured per
+// hipGraph replay when sh.graph, with launch-shape fields left 0 autotuned
+// when sh.tune.
 pmmc_status run_problems(pmmc_ctx *ctx, std::vector<HostProblem> &hp, std::vector<float *> &outs,
-                         std::vector<pmmc_anneal_stats *> &stats, int blocks_per_launch,
-                         bool use_graph, int steps_per_graph, DevSpan span) {
+                         std::vector<pmmc_anneal_stats *> &stats, BatchShape &sh, DevSpan span) {
   BatchDev bd;
   pmmc_status ps = batch_setup(ctx, hp, bd, span);
   if (ps != PMMC_OK) return ps;
   hipStream_t s = ctx->stream;
-  PMMC_HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
-  if (blocks_per_launch <= 0) {
+  if (sh.cpw > 0) set_cpw(bd, sh.cpw);
+  if (!sh.tune && sh.bps <= 0) {
+    PMMC_HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
     hipError_t err = batch_step(ctx, bd, INT_MAX);
     if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("anneal: ") + hipGetErrorString(err));
   } else {
+    int max_block = 1;
+    for (const auto &h : hp) max_block = std::max(max_block, h.sched.block);
+    // the tuner's probe steps are part of the timed run
+    PMMC_HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+    if (sh.tune) {
+      ps = tune_shape(ctx, bd, sh, max_block);
+      if (ps != PMMC_OK) return ps;
+    }
     // the captured step loop; destroyed on every return path
     struct GraphGuard {
       hipGraph_t graph = nullptr;
@@ -1005,25 +1175,25 @@ pmmc_status run_problems(pmmc_ctx *ctx, std::vector<HostProblem> &hp, std::vecto
         if (graph) (void)hipGraphDestroy(graph);
       }
     } gg;
-    if (use_graph) {
-      if (steps_per_graph <= 0) steps_per_graph = 8;
+    const int bps = sh.bps > 0 ? sh.bps : 1;
+    if (sh.graph) {
+      if (sh.spg <= 0) sh.spg = 8;
       PMMC_HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       hipError_t cerr = hipSuccess;
-      for (int i = 0; i < steps_per_graph && cerr == hipSuccess; ++i) cerr = batch_step(ctx, bd, blocks_per_launch);
+      for (int i = 0; i < sh.spg && cerr == hipSuccess; ++i) cerr = batch_step(ctx, bd, bps);
       hipError_t eerr = hipStreamEndCapture(s, &gg.graph);
       if (cerr != hipSuccess || eerr != hipSuccess)
         return fail(ctx, PMMC_ERR_HIP, "graph capture failed");
       PMMC_HIP_TRY(ctx, hipGraphInstantiate(&gg.exec, gg.graph, nullptr, nullptr, 0));
     }
     for (;;) {
-      hipError_t err;
-      if (use_graph) err = hipGraphLaunch(gg.exec, s);
-      else err = batch_step(ctx, bd, blocks_per_launch);
-      if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("step: ") + hipGetErrorString(err));
       int active = 0;
-      err = batch_count_active(ctx, bd, &active);
+      hipError_t err = batch_count_active(ctx, bd, &active);
       if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("count: ") + hipGetErrorString(err));
       if (active == 0) break;
+      if (sh.graph) err = hipGraphLaunch(gg.exec, s);
+      else err = batch_step(ctx, bd, bps);
+      if (err != hipSuccess) return fail(ctx, PMMC_ERR_HIP, std::string("step: ") + hipGetErrorString(err));
     }
   }
   PMMC_HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
@@ -1048,7 +1218,8 @@ static pmmc_status plain(pmmc_ctx *ctx, int level, const void *prob, const pmmc_
   if (ps != PMMC_OK) return ps;
   std::vector<float *> outs{pos_out};
   std::vector<pmmc_anneal_stats *> st{stats};
-  return run_problems(ctx, hp, outs, st, 0, false, 0);
+  BatchShape sh;
+  return run_problems(ctx, hp, outs, st, sh);
 }
 
 extern "C" pmmc_status pmmc_heatmap_anneal(pmmc_ctx *ctx, const pmmc_heatmap_problem *p,
@@ -1070,7 +1241,7 @@ extern "C" pmmc_status pmmc_smooth_anneal(pmmc_ctx *ctx, const pmmc_smooth_probl
 }
 
 extern "C" pmmc_status pmmc_anneal_batch(pmmc_ctx *ctx, const pmmc_job *jobs, int32_t n_jobs,
-                                         const pmmc_batch_opts *opts) {
+                                         pmmc_batch_opts *opts) {
   if (!ctx || (!jobs && n_jobs > 0) || n_jobs < 0) return fail(ctx, PMMC_ERR_INVALID, "bad batch");
   if (n_jobs == 0) return PMMC_OK;
   std::vector<HostProblem> hp(n_jobs);
@@ -1078,16 +1249,52 @@ extern "C" pmmc_status pmmc_anneal_batch(pmmc_ctx *ctx, const pmmc_job *jobs, in
   std::vector<pmmc_anneal_stats *> st(n_jobs);
   for (int i = 0; i < n_jobs; ++i) {
     if (!jobs[i].pos_out) return fail(ctx, PMMC_ERR_INVALID, "job without pos_out");
-    pmmc_status ps = prepare_problem(ctx, jobs[i].level, jobs[i].problem, jobs[i].schedule, jobs[i].run, hp[i]);
-    if (ps != PMMC_OK) return ps;
     outs[i] = jobs[i].pos_out;
     st[i] = jobs[i].stats;
   }
-  int bps = opts ? opts->blocks_per_step : 0;
-  int spg = opts ? opts->steps_per_graph : 0;
-  bool graph = opts ? opts->use_graph != 0 : false;
-  if (bps <= 0) bps = autotune_blocks_per_step(hp);
-  return run_problems(ctx, hp, outs, st, bps, graph, spg);
+  // Prepare the jobs on host threads: building the tables (an n x n scan per
+  // arc problem) is most of a large batch's host time.  Each thread reports
+  // errors through its own copy of the context; the first failing job's
+  // message is kept.
+  std::vector<pmmc_status> pst(n_jobs, PMMC_OK);
+  std::vector<std::string> pmsg(n_jobs);
+  const int nt = std::max(1, std::min({n_jobs / 4, 16, static_cast<int>(std::thread::hardware_concurrency())}));
+  auto prep = [&](int t) {
+    pmmc_ctx local;
+    local.device = ctx->device;
+    local.cu_count = ctx->cu_count;
+    local.hbm_bytes = ctx->hbm_bytes;
+    local.default_seed = ctx->default_seed;
+    for (int i = t; i < n_jobs; i += nt) {
+      pst[i] = prepare_problem(&local, jobs[i].level, jobs[i].problem, jobs[i].schedule, jobs[i].run, hp[i]);
+      if (pst[i] != PMMC_OK) pmsg[i] = local.last_error;
+    }
+  };
+  if (nt == 1) {
+    prep(0);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) th.emplace_back(prep, t);
+    for (auto &x : th) x.join();
+  }
+  for (int i = 0; i < n_jobs; ++i)
+    if (pst[i] != PMMC_OK) return fail(ctx, pst[i], "job " + std::to_string(i) + ": " + pmsg[i]);
+  BatchShape sh;
+  sh.bps = opts ? opts->blocks_per_step : 0;
+  sh.spg = opts ? opts->steps_per_graph : 0;
+  sh.graph = opts ? opts->use_graph != 0 : false;
+  sh.cpw = opts ? opts->chains_per_wave : 0;
+  if (sh.cpw != 0 && sh.cpw != 64 && sh.cpw != 32)
+    return fail(ctx, PMMC_ERR_INVALID, "chains_per_wave must be 0, 32 or 64");
+  sh.tune = sh.bps <= 0 || sh.cpw <= 0 || (sh.graph && sh.spg <= 0);
+  const pmmc_status ps = run_problems(ctx, hp, outs, st, sh);
+  if (ps == PMMC_OK && opts) {
+    opts->blocks_per_step = sh.bps;
+    opts->steps_per_graph = sh.graph ? sh.spg : 0;
+    opts->chains_per_wave = sh.cpw > 0 ? sh.cpw : 64;
+    opts->tuned = sh.tuned;
+  }
+  return ps;
 }
 
 // ------------------------------------------------------------------ //

@@ -51,7 +51,6 @@ int pick_shift_of(int level, int n);
 pmmc_status prepare_problem(pmmc_ctx *ctx, int level, const void *problem, const pmmc_schedule &s,
                             const pmmc_run &run, HostProblem &out);
 pmmc_status check_schedule(pmmc_ctx *ctx, const pmmc_schedule &s);
-int autotune_blocks_per_step(const std::vector<HostProblem> &hp);
 int default_chains(const pmmc_ctx *ctx, const HostProblem &h);
 // ensembles (one wavefront each) of h's annealing kernel one CU holds at once
 // (the runtime's occupancy figure for that kernel; 16 if the query fails)
@@ -66,6 +65,9 @@ struct LaunchGroup {
   int hreg;   // heatmap kept in registers: 2 (33..64 beads, two 32-bead halves); 0 = no
   bool asym;  // register heatmap group holds a non-symmetric heatmap
   int ens0, n_ens;
+  // arc level: chains per wavefront of the launch, 64 (one lane per chain) or
+  // 32 (a lane pair per chain, arcs_pair.hip); the same results either way
+  int cpw = 64;
 };
 LaunchGroup group_of(const HostProblem &h);
 std::tuple<int, int, int, int, int> group_key(const LaunchGroup &g);
@@ -80,6 +82,10 @@ struct BatchDev {
   std::vector<DevEns> hens;
   std::vector<LaunchGroup> groups;
   int n_ens_total = 0;
+  std::vector<char> stage;      // host copy of the static tables (one upload)
+  char *outs = nullptr;         // every problem's result positions
+  size_t outs_bytes = 0;
+  std::vector<size_t> out_off;  // problem i's positions at outs + out_off[i]
 };
 
 // Device memory a batch is carved from: a caller-given span (base non-null;
@@ -90,17 +96,26 @@ struct DevSpan {
 };
 // device bytes a batch of these problems needs
 size_t batch_bytes(const std::vector<HostProblem> &hp);
-// Run prepared problems to completion.  blocks_per_launch <= 0: one launch
-// per launch group runs every chain to its end; else step launches
-// (replayed from a captured hipGraph when use_graph) until every chain stops.
+// Launch shape of a run: blocks per step launch (bps <= 0 without tune: one
+// launch runs every chain to its end), steps per captured graph, hipGraph
+// replay on/off, arc chains per wavefront (0 = not set); tune: fields left 0
+// are chosen by timing the run's first steps (tuned = 1 afterwards).
+struct BatchShape {
+  int bps = 0, spg = 0, cpw = 0;
+  bool graph = false, tune = false;
+  int tuned = 0;
+};
+// Run prepared problems to completion with launch shape `sh` (in/out).
 pmmc_status run_problems(pmmc_ctx *ctx, std::vector<HostProblem> &hp, std::vector<float *> &outs,
-                         std::vector<pmmc_anneal_stats *> &stats, int blocks_per_launch,
-                         bool use_graph, int steps_per_graph, DevSpan span = DevSpan{});
+                         std::vector<pmmc_anneal_stats *> &stats, BatchShape &sh,
+                         DevSpan span = DevSpan{});
 
 bool stage_ok(int n);
 size_t problem_bytes(const HostProblem &h);
+// Upload one problem into carved device memory; its static tables are staged
+// in `stage`, which must stay alive until the stream has synchronised.
 void fill_problem(Carver &cv, hipStream_t s, const HostProblem &h, int index, DevResult *result,
-                  DevProblem &P, std::vector<DevEns> &ens, hipError_t &err);
+                  DevProblem &P, std::vector<DevEns> &ens, hipError_t &err, std::vector<char> &stage);
 hipError_t launch_groups(hipStream_t s, const DevProblem *probs, const DevEns *ens,
                          const std::vector<LaunchGroup> &groups, int max_blocks);
 // max_n: largest bead count among the problems (sizes the replication grid)
@@ -116,6 +131,10 @@ int hreg_resident_per_cu(bool asym);
 hipError_t launch_heatmap_reg(hipStream_t s, const DevProblem *probs, const DevEns *ens, int n_ens,
                               int max_blocks, int halves, bool asym);
 hipError_t launch_finalize(hipStream_t s, DevProblem *probs, int n_probs, const DevEns *ens);
+// lane-pair arc kernel (arcs_pair.hip): one 128-lane workgroup per ensemble
+int arcs_pair_resident_per_cu();
+hipError_t launch_arcs_pair(hipStream_t s, const DevProblem *probs, const DevEns *ens, int n_ens,
+                            int max_blocks);
 hipError_t launch_count_per_problem(hipStream_t s, const DevEns *ens, int n_ens, int *counts,
                                     int n_counts);
 

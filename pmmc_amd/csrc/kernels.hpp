@@ -244,22 +244,34 @@ __device__ __forceinline__ float heatmap_delta(const DevProblem &P, const View &
   return d + acc2;
 }
 
-// Arc + loop terms of one chunk of CSR entries of bead p, in entry order:
-// acc += (d1 - c)^2 k1 - (d0 - c)^2 k0 for the first `cnt` entries, with d0 /
-// d1 the distances from the old / new position of p to the partner.
+// The arc-level ΔE in FP32 is defined as two accumulators: the terms of the
+// even entries of bead p's CSR row (relative to its start) in increasing
+// order, and those of the odd entries likewise, then acc[0] + acc[1].  A
+// kernel that splits a chain's partners over two lanes by entry parity
+// forms the same two sums, so every arc kernel gives the same bits.
+// Term of one entry: (d1 - c)^2 k1 - (d0 - c)^2 k0, with d0 / d1 the
+// distances from the old / new position of p to the partner.
+__device__ __forceinline__ float arcs_term(float c, float ka, float kb, float px, float py,
+                                           float pz, float nx, float ny, float nz, float xj,
+                                           float yj, float zj) {
+  const float t0 = dist3(px, py, pz, xj, yj, zj) - c;
+  const float t1 = dist3(nx, ny, nz, xj, yj, zj) - c;
+  const float k0 = t0 >= 0.f ? ka : kb;
+  const float k1 = t1 >= 0.f ? ka : kb;
+  return t1 * t1 * k1 - t0 * t0 * k0;
+}
+
+// One chunk (kArcsChunk entries, an even offset into the row) of bead p's
+// entries: entry u adds to acc[u & 1], for the first `cnt` entries.
 __device__ __forceinline__ void arcs_chunk(const ArcEntry (&en)[kArcsChunk], const float (&xj)[kArcsChunk],
                                            const float (&yj)[kArcsChunk], const float (&zj)[kArcsChunk],
                                            int cnt, float px, float py, float pz, float nx, float ny,
-                                           float nz, float &acc) {
+                                           float nz, float (&acc)[2]) {
+  static_assert(kArcsChunk % 2 == 0, "chunks start at even entries");
 #pragma unroll
   for (int u = 0; u < kArcsChunk; ++u) {
-    if (u < cnt) {
-      const float t0 = dist3(px, py, pz, xj[u], yj[u], zj[u]) - en[u].c;
-      const float t1 = dist3(nx, ny, nz, xj[u], yj[u], zj[u]) - en[u].c;
-      const float k0 = t0 >= 0.f ? en[u].ka : en[u].kb;
-      const float k1 = t1 >= 0.f ? en[u].ka : en[u].kb;
-      acc += t1 * t1 * k1 - t0 * t0 * k0;
-    }
+    if (u < cnt)
+      acc[u & 1] += arcs_term(en[u].c, en[u].ka, en[u].kb, px, py, pz, nx, ny, nz, xj[u], yj[u], zj[u]);
   }
 }
 
@@ -287,14 +299,14 @@ template <class View>
 __device__ __forceinline__ float arcs_delta(const DevProblem &P, const View &X, int p, float px,
                                             float py, float pz, float nx, float ny, float nz) {
   const int e0 = cst(P.off)[p], e1 = cst(P.off)[p + 1];
-  float acc = 0.f;
+  float acc[2] = {0.f, 0.f};
   for (int e = e0; e < e1; e += kArcsChunk) {
     ArcEntry en[kArcsChunk];
     float xj[kArcsChunk], yj[kArcsChunk], zj[kArcsChunk];
     arcs_load_chunk(P, X, e, e1, en, xj, yj, zj);
     arcs_chunk(en, xj, yj, zj, e1 - e, px, py, pz, nx, ny, nz, acc);
   }
-  return acc;
+  return acc[0] + acc[1];
 }
 
 // Everything one arc-level move reads from the chains' state before its
@@ -356,18 +368,14 @@ template <class View>
 __device__ __forceinline__ float arcs_delta_fetched(const DevProblem &P, const View &X,
                                                     const ArcsFetch &f, float nx, float ny,
                                                     float nz) {
-  float acc = 0.f;
+  float acc[2] = {0.f, 0.f};
   const int cnt = min(kArcsChunk, f.e1 - f.e0);
 #pragma unroll
   for (int u = 0; u < kArcsChunk; ++u) {
     if (u < cnt) {
       const int e = f.e0 + u;
-      const float c = cst(P.ec)[e], ka = cst(P.eka)[e], kb = cst(P.ekb)[e];
-      const float t0 = dist3(f.px, f.py, f.pz, f.xj[u], f.yj[u], f.zj[u]) - c;
-      const float t1 = dist3(nx, ny, nz, f.xj[u], f.yj[u], f.zj[u]) - c;
-      const float k0 = t0 >= 0.f ? ka : kb;
-      const float k1 = t1 >= 0.f ? ka : kb;
-      acc += t1 * t1 * k1 - t0 * t0 * k0;
+      acc[u & 1] += arcs_term(cst(P.ec)[e], cst(P.eka)[e], cst(P.ekb)[e], f.px, f.py, f.pz, nx, ny,
+                              nz, f.xj[u], f.yj[u], f.zj[u]);
     }
   }
   for (int e = f.e0 + kArcsChunk; e < f.e1; e += kArcsChunk) {
@@ -376,7 +384,7 @@ __device__ __forceinline__ float arcs_delta_fetched(const DevProblem &P, const V
     arcs_load_chunk(P, X, e, f.e1, en, xj, yj, zj);
     arcs_chunk(en, xj, yj, zj, f.e1 - e, f.px, f.py, f.pz, nx, ny, nz, acc);
   }
-  return acc;
+  return acc[0] + acc[1];
 }
 
 // smooth: bond term ((d - d0)/d0)^2 * k_{stretch|squeeze}, d0 = max(dtn, 1e-6)

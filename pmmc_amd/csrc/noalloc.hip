@@ -84,5 +84,6 @@ extern "C" pmmc_status pmmc_anneal_noalloc(pmmc_ctx *ctx, int32_t level, const v
                     std::to_string(cap) + " are available (fewer chains or beads)");
   std::vector<float *> outs{pos_out};
   std::vector<pmmc_anneal_stats *> st{stats};
-  return run_problems(ctx, hp, outs, st, 0, false, 0, DevSpan{ctx->scratch, cap});
+  BatchShape sh;
+  return run_problems(ctx, hp, outs, st, sh, DevSpan{ctx->scratch, cap});
 }

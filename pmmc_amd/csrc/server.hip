@@ -55,6 +55,7 @@ struct SrvJob {
   DevResult *dres = nullptr;
   DevProblem hprob{};
   std::vector<DevEns> hens;
+  std::vector<char> stage;      // the job's static tables as uploaded at admission
 };
 
 // launch group of the server: kernel group key + size class
@@ -193,7 +194,7 @@ static pmmc_status admit(pmmc_server *s, SrvJob *j, bool *placed) {
   j->dres = cv.take<DevResult>(1);
   hipError_t err = hipSuccess;
   j->hens.clear();
-  fill_problem(cv, ctx->stream, j->hp, 0, j->dres, j->hprob, j->hens, err);
+  fill_problem(cv, ctx->stream, j->hp, 0, j->dres, j->hprob, j->hens, err, j->stage);
   if (err == hipSuccess)
     err = hipMemcpyAsync(j->dprob, &j->hprob, sizeof(DevProblem), hipMemcpyHostToDevice, ctx->stream);
   if (err == hipSuccess)

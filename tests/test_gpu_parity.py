@@ -343,3 +343,75 @@ def test_server_budget_admission(ctx):
         srv2 = ctx.server(hbm_budget=1024, max_jobs=4)
         srv2.submit(probs[0], sched, n_chains=64)   # larger than the budget
     srv.close()
+
+
+# ---------------- lane-pair arc kernel and the launch-shape tuner ----------------
+def _pair_cases():
+    s = settings(max_temp=5.0, dt_temp=0.9999, improvement=0.975, steps=2048, min_successes=100)
+    probs, scheds = [], []
+    for n, seed in ((30, 1), (150, 2), (400, 3)):
+        p = synth.arcs_problem(n=n, n_arcs=3 * n, seed=90 + seed)
+        probs.append(p)
+        scheds.append(schedule_preset(2, 1, s, 0.3))
+    # a hub bead with 40 arcs (rows longer than one fetched chunk), loops and
+    # the sphere penalty, 2D moves
+    hub = synth.arcs_problem(n=120, n_arcs=360, seed=97)
+    ed = hub.exp_dist.copy()
+    for j in range(1, 41):
+        ed[0, j] = ed[j, 0] = 1.0 + 0.05 * j
+    hub.exp_dist = ed
+    hub.loop_pairs = np.array([[0, 60], [5, 100]], np.int32)
+    hub.loop_params = np.array([[1.0, 1.5], [0.5, 3.0]], np.float32)
+    sh = schedule_preset(2, 1, settings(max_temp=5.0, dt_temp=0.9999, improvement=0.975,
+                                        steps=2048, min_successes=100, sphere_radius=6.0,
+                                        use2d=1), 0.3)
+    probs.append(hub)
+    scheds.append(sh)
+    return probs, scheds
+
+
+@pytest.mark.parametrize("cpw", [32, 64])
+def test_batch_arcs_lane_pair_equals_plain(ctx, cpw):
+    probs, scheds = _pair_cases()
+    chains = [100, 256, 64, 192]          # ragged: a partial last ensemble
+    print(f"arcs batch chains_per_wave={cpw}", flush=True)
+    alone = [ctx.anneal(p, sc, seed=21, segment=i, n_chains=chains[i])
+             for i, (p, sc) in enumerate(zip(probs, scheds))]
+    got = {}
+    mixed = ctx.anneal_batch(probs, scheds, seeds=[21] * 4, n_chains=chains, blocks_per_step=3,
+                             steps_per_graph=4, use_graph=True, chains_per_wave=cpw, opts_out=got)
+    assert got["chains_per_wave"] == cpw and got["tuned"] == 0
+    for (pa, sa), (pb, sb) in zip(alone, mixed):
+        assert np.array_equal(pa, pb)
+        for k in ("energy", "energy_resummed", "best_energy", "moves", "accepted", "best_chain"):
+            assert sa[k] == sb[k], k
+
+
+def test_batch_arcs_lane_pair_matches_oracle(ctx):
+    # the lane-pair kernel's best chain against the oracle's chains
+    probs, scheds = _pair_cases()
+    p, sc = probs[3], scheds[3]
+    sc.max_moves = 2048
+    (pos, st), = ctx.anneal_batch([p], [sc], seeds=[7], segments=[3], n_chains=64,
+                                  blocks_per_step=2, use_graph=False, chains_per_wave=32)
+    P = O.Problem(p)
+    res = [P.anneal(O.schedule_from(sc), 7, 3, c) for c in range(64)]
+    best = int(np.argmin([r[1]["energy"] for r in res]))
+    assert st["best_chain"] == best
+    assert np.array_equal(pos, res[best][0])
+    assert st["moves"] == sum(r[1]["moves"] for r in res)
+
+
+def test_batch_tuner_chooses_and_keeps_results(ctx):
+    probs, scheds = _pair_cases()
+    chains = [64, 64, 128, 64]
+    alone = [ctx.anneal(p, sc, seed=4, segment=i, n_chains=chains[i])
+             for i, (p, sc) in enumerate(zip(probs, scheds))]
+    got = {}
+    mixed = ctx.anneal_batch(probs, scheds, seeds=[4] * 4, n_chains=chains, use_graph=True,
+                             opts_out=got)
+    print(f"tuner chose {got}", flush=True)
+    assert got["tuned"] == 1 and got["chains_per_wave"] in (32, 64)
+    assert got["blocks_per_step"] >= 1 and 1 <= got["steps_per_graph"] <= 64
+    for (pa, sa), (pb, sb) in zip(alone, mixed):
+        assert np.array_equal(pa, pb) and sa["energy"] == sb["energy"]
