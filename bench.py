@@ -411,35 +411,59 @@ def batched(ctx, chains_per_problem=256, plain_subset=16):
     }
 
 
-def served(ctx, seconds=3.0, rate=200.0, seed=0, plain_sample=24):
-    """Config 4: Poisson arrivals of mixed jobs through the server."""
+def _alg_bytes_per_move(level, n, deg, acc_rate):
+    """Algorithmic HBM bytes of one chain-move per kernel (the roofline
+    accounting): heatmap 33-64 beads (register kernels): 12 B read + 12 B x
+    acceptance written; other heatmaps: the N partners' and the moved bead's
+    positions (12 B each) and the weight row (4 N B, shared by 64 chains);
+    arcs: 12 B x (1 + deg) + 16 B x deg / 64 + 12 B x acceptance; smooth: the
+    5-bead window (60 B) + the rest lengths (8 B / 64) + 12 B x acceptance."""
+    if level == 1:
+        if 32 < n <= 64:
+            return 12.0 + 12.0 * acc_rate
+        return 12.0 * (n + 1) + 4.0 * n / 64.0 + 12.0 * acc_rate
+    if level == 2:
+        return 12.0 * (1 + deg) + 16.0 * deg / 64.0 + 12.0 * acc_rate
+    return 60.0 + 8.0 / 64.0 + 12.0 * acc_rate
+
+
+def served(ctx, seconds=20.0, rate=200.0, seed=0, plain_sample=24, window=5.0):
+    """Config 4: Poisson arrivals of mixed jobs through the server, `seconds`
+    of arrivals at `rate` per second; latency percentiles per `window` of
+    arrival time show whether the server keeps up."""
     import numpy as np
     from pmmc_amd import schedule_preset, settings, synth
     rng = np.random.default_rng(seed)
+    cap = 8192
+    # GPU-kernel presets of default.ini; every solve capped at 8,192 moves per
+    # chain (arcs and smooth: the preset's stop_condition_steps cap; heatmap:
+    # max_moves) so a request's latency stays in the sub-second range
     hs = schedule_preset(1, 1, settings(steps=50000), 1.0)
-    hs.max_moves = 8192
-    asched = schedule_preset(2, 1, settings(steps=8192, improvement=0.975, min_successes=100), 0.03)
-    ss = schedule_preset(3, 1, settings(steps=8192, dt_temp=0.9995, improvement=0.99,
+    hs.max_moves = cap
+    asched = schedule_preset(2, 1, settings(steps=cap, improvement=0.975, min_successes=100), 0.03)
+    ss = schedule_preset(3, 1, settings(steps=cap, dt_temp=0.9995, improvement=0.99,
                                         min_successes=50), 0.3)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 1.5)))
     arrivals = arrivals[arrivals < seconds]
+    # a pool of distinct inputs per kind, drawn from by the requests (input
+    # construction is not part of serving)
+    pool = {"heatmap": [], "arcs": [], "smooth": []}
+    for k in range(48):
+        n = int(np.exp(rng.uniform(np.log(64), np.log(512))))
+        pool["heatmap"].append((synth.heatmap_problem(n=n, seed=k)[0], hs, 1, n, 0.0))
+        n = int(np.exp(rng.uniform(np.log(20), np.log(2000))))
+        p = synth.arcs_problem(n=n, n_arcs=3 * n, seed=k)
+        deg = float(np.count_nonzero(p.exp_dist > 1e-6)) / n
+        pool["arcs"].append((p, asched, 2, n, deg))
+        a = int(np.exp(rng.uniform(np.log(21), np.log(401))))
+        sp = synth.smooth_problem(n_anchors=a, add=49, seed=k)
+        pool["smooth"].append((sp, ss, 3, len(np.asarray(sp.pos).reshape(-1, 3)), 0.0))
     reqs = []
     for k, t in enumerate(arrivals):
         u = rng.random()
         ens = int(rng.integers(1, 17))
-        if u < 0.4:
-            n = int(np.exp(rng.uniform(np.log(64), np.log(512))))
-            p = synth.heatmap_problem(n=n, seed=k)[0]
-            sc = hs
-        elif u < 0.8:
-            n = int(np.exp(rng.uniform(np.log(20), np.log(2000))))
-            p = synth.arcs_problem(n=n, n_arcs=3 * n, seed=k)
-            sc = asched
-        else:
-            a = int(np.exp(rng.uniform(np.log(21), np.log(401))))
-            p = synth.smooth_problem(n_anchors=a, add=49, seed=k)
-            sc = ss
-        reqs.append((float(t), p, sc, ens))
+        kind = "heatmap" if u < 0.4 else ("arcs" if u < 0.8 else "smooth")
+        reqs.append((float(t), pool[kind][int(rng.integers(0, 48))], ens))
     srv = ctx.server(hbm_budget=64 << 30, max_jobs=4096, blocks_per_step=0)
     t0 = time.perf_counter()
     jobs = {}
@@ -450,7 +474,7 @@ def served(ctx, seconds=3.0, rate=200.0, seed=0, plain_sample=24):
     while len(done_at) < len(reqs):
         now = time.perf_counter() - t0
         while nxt < len(reqs) and reqs[nxt][0] <= now:
-            t, p, sc, ens = reqs[nxt]
+            t, (p, sc, _, _, _), ens = reqs[nxt]
             jobs[nxt] = [srv.submit(p, sc, seed=1 + e, segment=nxt, n_chains=64) for e in range(ens)]
             pending.add(nxt)
             nxt += 1
@@ -467,8 +491,25 @@ def served(ctx, seconds=3.0, rate=200.0, seed=0, plain_sample=24):
                 pending.discard(k)
     wall = time.perf_counter() - t0
     lat = np.array([done_at[k] - reqs[k][0] for k in range(len(reqs))]) * 1e3
+    arr = np.array([r[0] for r in reqs])
+    windows = []
+    for w0 in np.arange(0.0, seconds, window):
+        sel = (arr >= w0) & (arr < w0 + window)
+        if sel.any():
+            windows.append({"arrivals_s": [float(w0), float(w0 + window)], "requests": int(sel.sum()),
+                            "p50_ms": float(np.percentile(lat[sel], 50)),
+                            "p99_ms": float(np.percentile(lat[sel], 99))})
     solves = sum(len(v) for v in jobs.values())
-    acc_served = sum(srv.result(j)[1]["accepted"] for v in jobs.values() for j in v)
+    moves = acc_served = 0
+    alg_bytes = 0.0
+    for k, v in jobs.items():
+        _, (_, _, level, n, deg), _ = reqs[k]
+        for j in v:
+            st = srv.result(j)[1]
+            moves += st["moves"]
+            acc_served += st["accepted"]
+            ar = st["accepted"] / max(1, st["moves"])
+            alg_bytes += st["moves"] * _alg_bytes_per_move(level, n, deg, ar)
     # the plain form on an evenly spread sample of the same requests, one
     # solve per call, results compared with the served ones
     sample = list(range(0, len(reqs), max(1, len(reqs) // plain_sample)))[:plain_sample]
@@ -476,7 +517,7 @@ def served(ctx, seconds=3.0, rate=200.0, seed=0, plain_sample=24):
     n_plain = acc_plain = 0
     t1 = time.perf_counter()
     for k in sample:
-        _, p, sc, ens = reqs[k]
+        _, (p, sc, _, _, _), ens = reqs[k]
         for e in range(ens):
             pos, st = ctx.anneal(p, sc, seed=1 + e, segment=k, n_chains=64)
             spos, sst = srv.result(jobs[k][e])
@@ -485,18 +526,29 @@ def served(ctx, seconds=3.0, rate=200.0, seed=0, plain_sample=24):
             acc_plain += st["accepted"]
     tp = time.perf_counter() - t1
     srv.close()
-    return {"requests": len(reqs), "arrival_rate_per_s": rate, "solves": solves, "steps": steps,
+    achieved = alg_bytes / wall / 1e9
+    return {"requests": len(reqs), "arrival_rate_per_s": rate, "arrival_seconds": seconds,
+            "solves": solves, "steps": steps, "wall_s": wall,
             "solves_per_s": solves / wall, "requests_per_s": len(reqs) / wall,
-            "accepted_moves_per_s": acc_served / wall,
+            "moves_per_s": moves / wall, "accepted_moves_per_s": acc_served / wall,
             "p50_latency_ms": float(np.percentile(lat, 50)),
             "p99_latency_ms": float(np.percentile(lat, 99)),
+            "latency_per_window": windows,
+            "hbm": {"achieved_GBps": achieved, "frac": achieved / HBM_PEAK_GBS,
+                    "accounting": "algorithmic bytes per chain-move of each job's kernel x its "
+                                  "moves, over the whole served run's wall time"},
+            "move_cap_per_chain": cap,
+            "schedules": "GPU-kernel presets of default.ini; every solve capped at 8,192 moves "
+                         "per chain (arcs, smooth: the preset's stop_condition_steps cap; "
+                         "heatmap: max_moves)",
             "plain_sample_requests": len(sample), "plain_solves_per_s": n_plain / tp,
             "plain_accepted_moves_per_s": acc_plain / tp,
             "solves_speedup_vs_plain": (solves / wall) / (n_plain / tp),
             "accepted_moves_speedup_vs_plain": (acc_served / wall) / (acc_plain / tp),
             "results_identical_alone_and_served": same,
             "mix": "40% heatmap 64-512 beads, 40% arcs 20-2000 anchors, 20% smooth 1k-20k beads; "
-                   "ensemble 1-16 solves of 64 chains per request"}
+                   "ensemble 1-16 solves of 64 chains per request, inputs drawn from 48 of each "
+                   "kind"}
 
 
 def probe() -> int:
