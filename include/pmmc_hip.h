@@ -480,6 +480,76 @@ pmmc_status pmmc_spline_positions(pmmc_ctx *ctx, int32_t n_beads, const int32_t 
                                   int64_t pos_start, int32_t resolution_bp, int64_t n_out,
                                   float *out_xyz);
 
+/* ---------------- stress refinement of anchor positions ---------------- */
+/* One problem of LooperSolver::stressRefineAnchors (LooperSolver.hpp:5638-5915)
+ * after its pair list is built: gradient descent on
+ *   E = sum_i k ((|x_i - x_i+1| - b_i) / b_i)^2      springs with b_i > 1e-6,
+ *                                                     k = k_stretch when
+ *                                                     stretched, else k_squeeze
+ *     + sum_p ((|x_i - x_j| - t_p) / t_p)^2          pairs (i, j, t_p)
+ *     + sum_{i=1..n-2} k_angular theta_i^3            bending angle at bead i
+ * with the bending gradient by forward differences (step 1e-3).  Each
+ * iteration halves the step if the energy rose (stopping, before moving, once
+ * it falls below 1e-6), moves x -= step * gradient, and stops after iteration
+ * 21 once the energy fell by less than 1e-7 max(1, E).  The reference builds
+ * the pairs from the chromosome's arcs and long arcs (pmmc_tree_stress_refine
+ * does that from a tree); here they are given. */
+typedef struct {
+  int32_t n;                   /* beads, in chain (genomic) order                */
+  const float *pos;            /* [3n] start positions                           */
+  const float *backbone;       /* [n-1] spring target to the next bead; <= 1e-6:
+                                  no spring                                      */
+  int32_t n_pairs;
+  const int32_t *pair_i;       /* [n_pairs] bead indices, i != j                 */
+  const int32_t *pair_j;
+  const float *pair_target;    /* [n_pairs] target distance                      */
+  float k_stretch, k_squeeze, k_angular;
+} pmmc_stress_problem;
+
+typedef struct {
+  int32_t max_iter;       /* stressRefineIterations (default 0: nothing to do) */
+  float lr;               /* stressRefineLR (default 0.05)                      */
+  int32_t in_order_sums;  /* 1: sum every iteration's energy terms one at a time,
+                             in the reference's order (slower).  0: a tree sum,
+                             falling back to the in-order sums only when a
+                             decision lies within their rounding-error bound;
+                             the results are the same either way.             */
+} pmmc_stress_opts;
+
+void pmmc_stress_opts_default(pmmc_stress_opts *o);
+
+typedef enum {
+  PMMC_STRESS_MAX_ITER = 0,   /* ran max_iter iterations                        */
+  PMMC_STRESS_CONVERGED = 1,  /* energy change below 1e-7 max(1, E)             */
+  PMMC_STRESS_LR_FLOOR = 2,   /* step halved below 1e-6                         */
+  PMMC_STRESS_NOTHING = 3     /* max_iter <= 0 or fewer than 2 beads            */
+} pmmc_stress_stop;
+
+typedef struct {
+  int32_t iterations;     /* position updates made                             */
+  int32_t stop;           /* pmmc_stress_stop                                  */
+  float lr;               /* step at the end                                   */
+  int32_t exact_sums;     /* iterations whose decision used the in-order sums  */
+  double total;           /* energy of the last evaluated positions (in-order  */
+  double backbone;        /*   sums, as the reference prints them): total =    */
+  double pairs;           /*   (backbone + pairs) + angular                    */
+  double angular;
+} pmmc_stress_stats;
+
+/* Refines `count` problems in one launch (one workgroup each); pos_out[c]
+ * [3n] receives problem c's positions (it may alias probs[c].pos); stats
+ * nullable.  Results are the reference's bit for bit. */
+pmmc_status pmmc_stress_refine(pmmc_ctx *ctx, int32_t count, const pmmc_stress_problem *probs,
+                               const pmmc_stress_opts *opts, float *const *pos_out,
+                               pmmc_stress_stats *stats);
+
+/* Diagnostics (no reference counterpart): the device's arc cosine (the
+ * kernels' replacement for the reference's libm acosf) over the floats whose
+ * bit patterns are first_bits .. first_bits + count - 1, folded into
+ * *sum = sum over inputs of splitmix64((bits << 32) | result bits) mod 2^64,
+ * so a host computing the same sum with libm's acosf can check every input. */
+pmmc_status pmmc_probe_acosf(pmmc_ctx *ctx, uint32_t first_bits, uint32_t count, uint64_t *sum);
+
 #ifdef __cplusplus
 }
 #endif

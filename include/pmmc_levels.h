@@ -32,6 +32,11 @@
  *                           :3146-3172
  *   pmmc_normalize_heatmap_inter
  *                           LooperSolver::normalizeHeatmapInter :3106-3144
+ *   pmmc_tree_stress_refine LooperSolver::stressRefineAnchors :5638-5915
+ *   pmmc_arcs_level (global_refine)
+ *                           the global anchor refinement at the end of
+ *                           reconstructClustersArcsDistances' chromosome
+ *                           loop :4141-4243
  *
  * Errors: the message of the last failure on the calling thread is returned
  * by pmmc_host_last_error() (the reference prints and exits instead).
@@ -242,6 +247,10 @@ typedef struct {
   double score_smooth;       /*   anchor / sub-anchor stage (heatmap: score_anchor) */
   double ms_anchor;          /* host wall time of each batched stage          */
   double ms_smooth;
+  double score_global;       /* global refinement: sum over chromosomes of the */
+  double ms_global;          /*   best restart's score; wall time incl. stress */
+  int32_t stress_iterations; /* stress updates summed over chromosomes         */
+  int32_t reinterpolated;    /* sub-anchor beads re-placed between anchors     */
 } pmmc_level_report;
 
 typedef struct {
@@ -257,6 +266,8 @@ typedef struct {
   int32_t chains_per_restart; /* chains of each job; 0 = share the library
                                  default chain pool among the jobs             */
   int32_t densify;            /* 0: stop after the anchor stage                */
+  int32_t global_refine;      /* useGlobalAnchorRefinement (default 0)         */
+  pmmc_stress_opts stress;    /* stressRefineIterations / LR of the global pass */
 } pmmc_arcs_level_opts;
 
 void pmmc_arcs_level_opts_default(pmmc_arcs_level_opts *o);
@@ -270,10 +281,40 @@ void pmmc_arcs_level_opts_default(pmmc_arcs_level_opts *o);
  * noise_anchor_small on every bead; then densification (fixing the anchors)
  * and restarts of the smooth annealer with step = restart noise = mean
  * dist_to_next * noise_subanchor on the non-fixed beads.  The interaction
- * blocks' own positions are the caller's (positionInteractionBlocks). */
+ * blocks' own positions are the caller's (positionInteractionBlocks).
+ *
+ * global_refine (:4141-4243), per chromosome after its blocks: the active
+ * region becomes all its anchors (tree order); expected distances over them
+ * (arcs and long arcs); restarts of the arc annealer from the current
+ * positions (step = mean dist_to_next * noise_anchor, restart noise
+ * noise_anchor_small on every bead; anchors the sub-anchor stage fixed stay
+ * fixed, so their chains stop at once, PMMC_STOP_NOFREE); then, when
+ * stress.max_iter > 0, pmmc_tree_stress_refine's descent over the anchors
+ * sorted by genomic position; then every sub-anchor bead (level 5) that
+ * existed when the reference's pass ran (the beads of this and earlier
+ * chromosomes) whose genomic position lies within the anchors' range is
+ * re-placed on the line between the two anchors around it -- whichever
+ * chromosome the bead belongs to, as the reference's range test does.
+ * Random streams: the global restarts take the segments that follow the
+ * chromosome's block calls. */
 pmmc_status pmmc_arcs_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t chr,
                             const pmmc_level_params *p, const pmmc_arcs_level_opts *o,
                             uint64_t seed, uint32_t segment, pmmc_level_report *report);
+
+/* LooperSolver::stressRefineAnchors (LooperSolver.hpp:5638-5915) over the
+ * anchors of chromosome `chr` (-1: every chromosome, one launch), the active
+ * region the global refinement sets: anchors sorted by genomic position;
+ * pairs from the chromosome's arcs (targets freq_to_distance of the count),
+ * then its long arcs resolved to the first anchor containing each end, the
+ * first pair of two anchors kept, none of anchors adjacent in the sorted
+ * order while a spring constant exceeds 1e-6, none with a target <= 1e-6;
+ * spring targets genomic_length_to_distance of the gap to the next anchor
+ * (none for a zero gap); then pmmc_stress_refine, positions written back.
+ * stats (nullable) receives one record per chromosome processed. */
+pmmc_status pmmc_tree_stress_refine(pmmc_ctx *ctx, pmmc_tree *t, int32_t chr,
+                                    const pmmc_level_params *p, float k_stretch, float k_squeeze,
+                                    float k_angular, const pmmc_stress_opts *opts,
+                                    pmmc_stress_stats *stats);
 
 typedef struct {
   pmmc_level_settings heatmap;  /* [simulation_heatmap] + [cuda] settings       */

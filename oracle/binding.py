@@ -68,6 +68,12 @@ class OrcSweep(C.Structure):
                 ("max_iters", C.c_int32), ("use2d", C.c_int32)]
 
 
+class OrcStressStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("stop", C.c_int), ("lr", C.c_float),
+                ("total", C.c_double), ("backbone", C.c_double), ("pairs", C.c_double),
+                ("angular", C.c_double)]
+
+
 _lib = None
 
 
@@ -111,6 +117,17 @@ def load():
         L.orc_sweep_argmin.restype = C.c_int
         L.orc_sweep_argmin.argtypes = [C.POINTER(C.c_double), C.c_int]
         L.orc_mark_arcs.argtypes = [i32p, i32p, C.c_int32, i32p, i32p, C.c_int32, i32p, i32p]
+        L.orc_stress_gradient.argtypes = [C.c_int, f32p, f32p, C.c_int, i32p, i32p, f32p, C.c_float,
+                                          C.c_float, C.c_float, f32p, C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_stress_refine.argtypes = [C.c_int, f32p, f32p, C.c_int, i32p, i32p, f32p, C.c_float,
+                                        C.c_float, C.c_float, C.c_int, C.c_float,
+                                        C.POINTER(OrcStressStats)]
+        L.orc_stress_energy64.restype = C.c_double
+        L.orc_stress_energy64.argtypes = [C.c_int, f32p, f32p, C.c_int, i32p, i32p, f32p,
+                                          C.c_double, C.c_double, C.c_double]
+        L.orc_acosf_sum.restype = C.c_uint64
+        L.orc_acosf_sum.argtypes = [C.c_uint32, C.c_uint32, C.c_int]
         L.orc_group_arcs.restype = C.c_int
         L.orc_group_arcs.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, i32p, i32p, i32p, i32p,
                                      i32p]
@@ -325,3 +342,48 @@ def ref_mark_arcs(chromosomes):
         k += 7 * m
         out.append(arr)
     return out
+
+
+def _stress_args(pos, bb, pairs):
+    x = np.ascontiguousarray(pos, np.float32).reshape(-1, 3).copy()
+    n = len(x)
+    b = np.ascontiguousarray(bb, np.float32) if n > 1 else np.zeros(1, np.float32)
+    pr = np.asarray(pairs, np.float64).reshape(-1, 3)
+    pi = np.ascontiguousarray(pr[:, 0], np.int32)
+    pj = np.ascontiguousarray(pr[:, 1], np.int32)
+    pt = np.ascontiguousarray(pr[:, 2], np.float32)
+    return x, n, b, pi, pj, pt
+
+
+def stress_refine(pos, bb, pairs, ks, kq, ka, max_iter, lr, targets=None):
+    """orc_stress_refine: stressRefineAnchors' descent; pairs [(i, j, t)]
+    (targets, when given, replace t as exact float32 values).  Returns
+    (positions [n,3], stats dict)."""
+    x, n, b, pi, pj, pt = _stress_args(pos, bb, pairs)
+    if targets is not None:
+        pt = np.ascontiguousarray(targets, np.float32)
+    st = OrcStressStats()
+    load().orc_stress_refine(n, _p(x, f32p), _p(b, f32p), len(pi), _p(pi, i32p), _p(pj, i32p),
+                             _p(pt, f32p), ks, kq, ka, int(max_iter), lr, C.byref(st))
+    return x, {k: getattr(st, k) for k, _ in st._fields_}
+
+
+def stress_gradient(pos, bb, pairs, ks, kq, ka):
+    """orc_stress_gradient: (gradient [n,3], e_back, e_hic, e_ang)."""
+    x, n, b, pi, pj, pt = _stress_args(pos, bb, pairs)
+    G = np.zeros((n, 3), np.float32)
+    e = [C.c_double(), C.c_double(), C.c_double()]
+    load().orc_stress_gradient(n, _p(x, f32p), _p(b, f32p), len(pi), _p(pi, i32p), _p(pj, i32p),
+                               _p(pt, f32p), ks, kq, ka, _p(G, f32p), *[C.byref(v) for v in e])
+    return G, e[0].value, e[1].value, e[2].value
+
+
+def stress_energy64(pos, bb, pairs, ks, kq, ka):
+    x, n, b, pi, pj, pt = _stress_args(pos, bb, pairs)
+    return load().orc_stress_energy64(n, _p(x, f32p), _p(b, f32p), len(pi), _p(pi, i32p),
+                                      _p(pj, i32p), _p(pt, f32p), ks, kq, ka)
+
+
+def acosf_sum(first_bits, count, threads=8):
+    """orc_acosf_sum: libm acosf checksum over a range of float bit patterns."""
+    return int(load().orc_acosf_sum(int(first_bits), int(count), int(threads)))

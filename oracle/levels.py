@@ -19,6 +19,11 @@ powf/pow/exp are libm's, through ctypes):
                     freqToDistanceHeatmap(Inter) (:3934-3942) and
                     Heatmap::getDiagonalSize / getAvg / getRange
                     (Heatmap.hpp:356-367, :427-451)
+  stress_input      the set-up of LooperSolver::stressRefineAnchors
+                    (:5642-5760): sorted active region, pair list, spring
+                    targets (the descent itself is oracle/stress.c)
+  reinterpolate     the sub-anchor re-placement of the global anchor
+                    refinement (:4179-4239)
 
 PINNED BY tests/golden/levels_ref.json, written by the reference's own code
 (oracle/_ref/levels_ref), and by the invariants in tests/test_levels.py
@@ -320,15 +325,17 @@ def heat_avg(h):
         return F(s / F(cnt))
 
 
-def expected_distances(t, chr_name, ib, long_arcs, p, anchor_heatmap=None, influence=0.5):
+def expected_distances(t, chr_name, ib, long_arcs, p, anchor_heatmap=None, influence=0.5,
+                       active=None):
     """calcAnchorExpectedDistancesHeatmap (LooperSolver.hpp:5494-5627) with
-    active_region = children of ib and current_chr = chr_name.  An arc whose
-    other end lies outside the active region lands in slot 0, as the
-    reference's std::map::operator[] default does (:5538-5539)."""
+    active_region = children of ib (or `active`) and current_chr = chr_name.
+    An arc whose other end lies outside the active region lands in slot 0, as
+    the reference's std::map::operator[] default does (:5538-5539)."""
     if long_arcs is None:        # the chromosome's long arcs stored in the tree
         long_arcs = t["long_arcs"].get(chr_name, [])
     cl = t["clusters"]
-    active = cl[ib]["children"]
+    if active is None:
+        active = cl[ib]["children"]
     n = len(active)
     m = np.full((n, n), -1.0, np.float32)
     for i in range(n):
@@ -450,17 +457,25 @@ def _offer(best, b, score, payload, gpu_member=True):
         best[b] = (score, payload)
 
 
-def call_segments(sizes, restarts_anchor, restarts_subanchor, loop_density, segment):
+def call_segments(sizes, restarts_anchor, restarts_subanchor, loop_density, segment,
+                  chroms=None, global_anchors=None):
     """Philox segments of the anchor and sub-anchor restarts of each block, in
     the order the reference driver issues its GPU calls (LooperSolver.hpp:
     4027-4138; blocks of at most one child make none; sub-anchor problems
     below 4 beads run the CPU annealer, :4431-4437, and take segments from
-    2^30 on).  sizes: children per block before densification."""
+    2^30 on).  sizes: children per block before densification.  With
+    global_anchors ({chromosome: anchors}, in chromosome order) and chroms
+    (each block's chromosome), the global refinement's restarts follow each
+    chromosome's blocks (:4141-4170) and ({chromosome: first segment}) is
+    returned as well."""
     call, cpu, out = 0, 0, []
-    for n in sizes:
+    gseg = {}
+
+    def block(n):
+        nonlocal call, cpu
         if n <= 1:
             out.append((0, 0))
-            continue
+            return
         a = segment + call
         call += restarts_anchor
         if n + (n - 1) * loop_density >= 4:
@@ -470,7 +485,75 @@ def call_segments(sizes, restarts_anchor, restarts_subanchor, loop_density, segm
             sm = segment + (1 << 30) + cpu
             cpu += restarts_subanchor
         out.append((a, sm))
-    return out
+
+    if global_anchors is None:
+        for n in sizes:
+            block(n)
+        return out
+    k = 0
+    for name, n_anc in global_anchors.items():
+        while k < len(sizes) and chroms[k] == name:
+            block(sizes[k])
+            k += 1
+        if n_anc > 1:
+            gseg[name] = segment + call
+            call += restarts_anchor
+    return out, gseg
+
+
+def global_stage(t, chr_list, p, o, seed, gseg, limit, run_job, rep):
+    """The global anchor refinement of reconstructClustersArcsDistances'
+    chromosome loop (LooperSolver.hpp:4141-4243) for each listed chromosome:
+    restarts of the arc annealer over all its anchors from their current
+    positions (restart noise noise_anchor_small on every bead, step = mean
+    dist_to_next x noise_anchor), the best restart kept; then, when
+    o["stress_iters"] > 0, stressRefineAnchors (stress_input + the oracle's
+    descent); then the sub-anchor re-placement among the clusters that existed
+    when the reference ran it (limit[chromosome])."""
+    from oracle import binding as O
+    cl = t["clusters"]
+    todo = [name for name in chr_list if len(t["anchors"][name]) > 1]
+    best = {}
+    for name in todo:
+        cs = t["chr_first"][name]
+        active = list(range(cs, cs + len(t["anchors"][name])))
+        ed = expected_distances(t, name, None, None, p, active=active)
+        dtn, _, noise = dist_to_next([cl[c]["genomic_pos"] for c in active], p,
+                                     p["noise_subanchor"], p["noise_anchor"])
+        for i, c in enumerate(active[:-1]):
+            cl[c]["dist_to_next"] = float(dtn[i])
+        sched = o["arcs_sched"](noise)
+        for k in range(p["restarts_anchor"]):
+            seg = gseg[name] + k
+            pos = np.array([[F(cl[c]["pos"][a] + restart_noise(seed, seg, i, p["noise_anchor_small"],
+                                                               p["use2d"])[a]) for a in range(3)]
+                            for i, c in enumerate(active)], np.float32)
+            prob = dict(pos=pos, fixed=np.array([cl[c]["is_fixed"] for c in active], np.uint8),
+                        exp_dist=ed, k_stretch=o["k_stretch_arcs"], k_squeeze=o["k_squeeze_arcs"])
+            r = run_job(2, prob, sched, seed, seg, o["chains"])
+            rep["n_jobs"] += 1
+            _offer(best, name, r["energy"], (active, r))
+    rep.setdefault("score_global", 0.0)
+    for name, (score, (active, r)) in best.items():
+        for i, c in enumerate(active):
+            cl[c]["pos"] = [F(x) for x in r["pos"][i]]
+        rep["score_global"] += score
+    rep.setdefault("stress_iterations", 0)
+    rep.setdefault("reinterpolated", 0)
+    orders = {}
+    for name in todo:
+        cs = t["chr_first"][name]
+        act = list(range(cs, cs + len(t["anchors"][name])))
+        if o.get("stress_iters", 0) > 0:
+            act, p0, bb, pairs = stress_input(t, name, act, p, o["k_stretch"], o["k_squeeze"])
+            out, st = O.stress_refine(p0, bb, pairs, o["k_stretch"], o["k_squeeze"], o["k_angular"],
+                                      o["stress_iters"], o["stress_lr"])
+            for i, c in enumerate(act):
+                cl[c]["pos"] = [F(x) for x in out[i]]
+            rep["stress_iterations"] += st["iterations"]
+        orders[name] = act
+    for name in todo:
+        rep["reinterpolated"] += reinterpolate(t, orders[name], limit[name])
 
 
 def arcs_level(t, chr_name, p, o, seed, segment, run_job):
@@ -493,8 +576,17 @@ def arcs_level(t, chr_name, p, o, seed, segment, run_job):
             if cl[ib]["level"] == 3:
                 blocks.append((name, ib))
     rep = dict(n_blocks=len(blocks), n_jobs=0, score_anchor=0.0, score_smooth=0.0)
-    segs = call_segments([len(cl[ib]["children"]) for _, ib in blocks], p["restarts_anchor"],
-                         p["restarts_subanchor"], p["loop_density"], segment)
+    chr_list = [name for name in t["chrs"] if chr_name is None or name == chr_name]
+    glob = bool(o.get("global_refine", False))
+    sizes = [len(cl[ib]["children"]) for _, ib in blocks]
+    if glob:
+        segs, gseg = call_segments(sizes, p["restarts_anchor"], p["restarts_subanchor"],
+                                   p["loop_density"], segment, chroms=[n for n, _ in blocks],
+                                   global_anchors={n: len(t["anchors"][n]) for n in chr_list})
+    else:
+        segs = call_segments(sizes, p["restarts_anchor"], p["restarts_subanchor"],
+                             p["loop_density"], segment)
+    limit = {name: len(cl) for name in chr_list}
     best = {}
     for bi, (name, ib) in enumerate(blocks):
         active = list(cl[ib]["children"])
@@ -524,9 +616,14 @@ def arcs_level(t, chr_name, p, o, seed, segment, run_job):
             cl[c]["pos"] = [F(x) for x in r["pos"][i]]
         rep["score_anchor"] += score
     if not o.get("densify", True):
+        if glob:
+            global_stage(t, chr_list, p, o, seed, gseg, limit, run_job, rep)
         return rep
-    for name, ib in blocks:
-        densify(t, ib, p["loop_density"], True)
+    for name in chr_list:
+        for bname, ib in blocks:
+            if bname == name:
+                densify(t, ib, p["loop_density"], True)
+        limit[name] = len(cl)
     best = {}
     for bi, (name, ib) in enumerate(blocks):
         active = list(cl[ib]["children"])
@@ -558,6 +655,8 @@ def arcs_level(t, chr_name, p, o, seed, segment, run_job):
                 cl[c]["pos"] = [F(x) for x in r["pos"][i]]
         rep["score_smooth"] += score
     t["_candidates"] = cand
+    if glob:
+        global_stage(t, chr_list, p, o, seed, gseg, limit, run_job, rep)
     return rep
 
 
@@ -798,3 +897,103 @@ def normalize_heatmap_inter(heat, sizes, scale):
         h[o:o + k, o:o + k] = (h[o:o + k, o:o + k] / s).astype(np.float32)
         o += k
     return h
+
+
+# ------------------------------------------------------------------ stress refinement
+def stress_input(t, chr_name, active, p, ks, kq):
+    """stressRefineAnchors' set-up (LooperSolver.hpp:5642-5760): the active
+    region sorted by genomic position (a stable sort: the reference's
+    std::sort agrees where positions differ), pairs from the chromosome's arcs
+    then its long arcs (first pair of two anchors kept; none between
+    neighbours in the sorted order while a spring constant exceeds 1e-6; none
+    with a target <= 1e-6), spring targets from the genomic gaps (-1 for a
+    zero gap).  Returns (sorted active, positions [N,3], spring targets,
+    pairs [(i, j, t)])."""
+    cl = t["clusters"]
+    active = sorted(active, key=lambda c: cl[c]["genomic_pos"])
+    N = len(active)
+    anc = t["anchors"][chr_name]
+    n_anc, cs = len(anc), t["chr_first"][chr_name]
+    a2a = [-1] * n_anc
+    for i, c in enumerate(active):
+        if cs <= c < cs + n_anc:
+            a2a[c - cs] = i
+    backbone = F(ks) > F(1e-6) or F(kq) > F(1e-6)
+    pairs, seen = [], set()
+
+    def add(a, b, tgt):
+        if a == b:
+            return
+        a, b = min(a, b), max(a, b)
+        if backbone and b - a <= 1:
+            return
+        if (a, b) in seen:
+            return
+        seen.add((a, b))
+        pairs.append((a, b, tgt))
+
+    for (a, b), sc in zip(t["arcs"][chr_name], t["scores"][chr_name]):
+        if not (cs <= a < cs + n_anc and cs <= b < cs + n_anc):
+            continue
+        ai, aj = a2a[a - cs], a2a[b - cs]
+        if ai < 0 or aj < 0:
+            continue
+        tt = freq_to_distance(sc, p)
+        if tt > F(1e-6):
+            add(ai, aj, tt)
+
+    def pos_to_anc(x):
+        for k, (s0, e0) in enumerate(anc):
+            if s0 <= x <= e0:
+                return k
+        return -1
+
+    for s0, e0, sc in t["long_arcs"].get(chr_name, []):
+        ia, ib = pos_to_anc(s0), pos_to_anc(e0)
+        if ia < 0 or ib < 0:
+            continue
+        a, b = a2a[ia], a2a[ib]
+        if a < 0 or b < 0:
+            continue
+        tt = freq_to_distance(sc, p)
+        if tt > F(1e-6):
+            add(a, b, tt)
+    bb = np.full(max(N - 1, 1), -1.0, np.float32)
+    for i in range(N - 1):
+        gap = abs(cl[active[i + 1]]["genomic_pos"] - cl[active[i]]["genomic_pos"])
+        if gap > 0:
+            bb[i] = F(genomic_length_to_distance(gap, p))
+    pos = np.array([cl[c]["pos"] for c in active], np.float32).reshape(-1, 3)
+    return active, pos, bb, pairs
+
+
+def reinterpolate(t, active, limit):
+    """The global refinement's sub-anchor re-placement (LooperSolver.hpp:
+    4179-4239): every bead of level (anchors' level + 1) among the first
+    `limit` clusters whose genomic position lies within [first, last] of
+    `active` (its order) moves onto the segment between the anchors around
+    it.  Returns the number of beads moved."""
+    import bisect
+    cl = t["clusters"]
+    if not active:
+        return 0
+    sub = cl[active[0]]["level"] + 1
+    gp = [cl[c]["genomic_pos"] for c in active]
+    moved = 0
+    for ci in range(min(limit, len(cl))):
+        b = cl[ci]
+        if b["level"] != sub or b["genomic_pos"] < gp[0] or b["genomic_pos"] > gp[-1]:
+            continue
+        g = b["genomic_pos"]
+        j = bisect.bisect_left(gp, g)
+        if j == 0:
+            b["pos"] = [F(x) for x in cl[active[0]]["pos"]]
+        elif j >= len(gp):
+            b["pos"] = [F(x) for x in cl[active[-1]]["pos"]]
+        else:
+            g0, g1 = gp[j - 1], gp[j]
+            u = F(F(g - g0) / F(g1 - g0)) if g1 > g0 else F(0.5)
+            p0, p1 = cl[active[j - 1]]["pos"], cl[active[j]]["pos"]
+            b["pos"] = [F(F(F(F(1.0) - u) * F(p0[a])) + F(u * F(p1[a]))) for a in range(3)]
+        moved += 1
+    return moved

@@ -219,6 +219,25 @@ void orc_mark_arcs(const int32_t *anc_s, const int32_t *anc_e, int32_t A,
                    const int32_t *arc_s, const int32_t *arc_e, int32_t R,
                    int32_t *out_s, int32_t *out_e);
 
+/* ---- stress refinement (stress.c): LooperSolver::stressRefineAnchors'
+ * descent, LooperSolver.hpp:5762-5910, float arithmetic as the reference ---- */
+typedef struct {
+  int iterations; /* position updates made */
+  int stop;       /* 0 max_iter, 1 converged, 2 step below 1e-6, 3 nothing to do */
+  float lr;
+  double total, backbone, pairs, angular; /* last evaluated positions */
+} orc_stress_stats;
+
+void orc_stress_gradient(int n, const float *x, const float *bb, int np, const int *pi,
+                         const int *pj, const float *pt, float ks, float kq, float ka, float *G,
+                         double *e_back, double *e_hic, double *e_ang);
+void orc_stress_refine(int n, float *x, const float *bb, int np, const int *pi, const int *pj,
+                       const float *pt, float ks, float kq, float ka, int max_iter, float lr,
+                       orc_stress_stats *st);
+double orc_stress_energy64(int n, const float *x, const float *bb, int np, const int *pi,
+                           const int *pj, const float *pt, double ks, double kq, double ka);
+uint64_t orc_acosf_sum(uint32_t first, uint32_t count, int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,16 @@
 //             densifyActiveRegion, reconstructClusterArcsDistances(ib, i, true);
 //             the annealing calls go through the product's reference-side
 //             glue to libpmmc_hip.so (a GPU is needed)
+//             With a trailing "global ..." record: long arcs, and after each
+//             chromosome's blocks the global anchor refinement of
+//             :4141-4243 -- calcAnchorExpectedDistancesHeatmap and
+//             reconstructClusterArcsDistances(root, 0, false, false) over all
+//             anchors, stressRefineAnchors (:5638-5915), then the re-placement
+//             of sub-anchor beads between the anchors, which the reference
+//             writes inline (:4179-4239) and this driver restates.  The
+//             chromosome name "*" drives every chromosome in order.
+//   stress    tree, then LooperSolver::stressRefineAnchors (:5638-5915) over
+//             one chromosome's anchors with given positions and long arcs
 //   drive_heatmap
 //             tree, then reconstructClustersHeatmapSingleLevel(level)
 //             (:1258-1422) on a given distance heatmap, with the chromosome
@@ -55,6 +65,8 @@
 #include <Static.hpp>
 
 #include "../pmmc_amd/integration/reference_glue.h"
+
+static bool maybe_tok(std::string &s) { return static_cast<bool>(std::cin >> s); }
 
 static std::string tok() {
   std::string s;
@@ -613,28 +625,127 @@ static void mode_drive_arcs(LooperSolver &s) {
   Settings::useAnchorHeatmap = false;
   Settings::useCTCFMotifOrientation = false;
   read_run();
+  std::vector<std::string> chrs;
+  if (chr == "*")
+    chrs = s.chrs;
+  else
+    chrs.push_back(chr);
   s.setLevel(LVL_INTERACTION_BLOCK);
-  std::vector<int> ibs = s.current_level[chr];
-  for (int ib : ibs) {
-    float x = getf(), y = getf(), z = getf();
-    s.clusters[ib].pos.set(x, y, z);
+  std::vector<int> all_ibs;
+  for (const std::string &c : chrs)
+    for (int ib : s.current_level[c]) {
+      float x = getf(), y = getf(), z = getf();
+      s.clusters[ib].pos.set(x, y, z);
+      all_ibs.push_back(ib);
+    }
+  // optional: global <on> <stress iterations> <stress lr> <n long arcs>
+  // followed by "chr start end count" per long arc
+  std::string g;
+  if (maybe_tok(g) && g == "global") {
+    Settings::useGlobalAnchorRefinement = geti() != 0;
+    Settings::stressRefineIterations = geti();
+    Settings::stressRefineLR = getf();
+    const int n_long = geti();
+    for (int i = 0; i < n_long; ++i) {
+      const std::string c = tok();
+      const int st = geti(), en = geti(), sc = geti();
+      s.arcs.long_arcs[c].push_back(InteractionArc(st, en, sc));
+    }
   }
-  s.current_chr = chr;
   s.gpu_cache.init();
   fflush(stdout);
-  for (size_t i = 0; i < ibs.size(); ++i) {
-    const int ib = ibs[i];
-    s.active_region.assign(s.clusters[ib].children.begin(), s.clusters[ib].children.end());
-    for (int c : s.active_region) s.clusters[c].pos = s.clusters[ib].pos;
+  for (const std::string &c : chrs) {
+    s.current_chr = c;
+    const std::vector<int> ibs = s.current_level[c];
+    for (size_t i = 0; i < ibs.size(); ++i) {
+      const int ib = ibs[i];
+      s.active_region.assign(s.clusters[ib].children.begin(), s.clusters[ib].children.end());
+      for (int k : s.active_region) s.clusters[k].pos = s.clusters[ib].pos;
+      s.calcAnchorExpectedDistancesHeatmap();
+      s.reconstructClusterArcsDistances(ib, static_cast<int>(i), false);
+      s.densifyActiveRegion(ib, true);
+      s.reconstructClusterArcsDistances(ib, static_cast<int>(i), true, false);
+    }
+    if (!Settings::useGlobalAnchorRefinement) continue;
+    // the global pass, as the reference's chromosome loop runs it after the
+    // blocks (LooperSolver.hpp:4148-4178)
+    s.active_region.clear();
+    const int cstart = s.chr_first_cluster[c];
+    const int n_anc = s.arcs.anchors_cnt[c];
+    for (int i = 0; i < n_anc; ++i) s.active_region.push_back(cstart + i);
+    if (s.active_region.size() <= 1) continue;
     s.calcAnchorExpectedDistancesHeatmap();
-    s.reconstructClusterArcsDistances(ib, static_cast<int>(i), false);
-    s.densifyActiveRegion(ib, true);
-    s.reconstructClusterArcsDistances(ib, static_cast<int>(i), true, false);
+    s.reconstructClusterArcsDistances(s.chr_root[c], 0, false, false);
+    if (Settings::stressRefineIterations > 0)
+      s.stressRefineAnchors(c, Settings::stressRefineIterations, Settings::stressRefineLR);
+    // sub-anchor beads (the anchors' level + 1) whose genomic position lies
+    // in [first, last] anchor position (active_region's order) move onto
+    // the segment between the anchors around them (:4179-4239)
+    const int sub_level = s.clusters[s.active_region[0]].level + 1;
+    std::vector<int> gp;
+    for (int a : s.active_region) gp.push_back(s.clusters[a].genomic_pos);
+    for (size_t k = 0; k < s.clusters.size(); ++k) {
+      Cluster &b = s.clusters[k];
+      if (b.level != sub_level || b.genomic_pos < gp.front() || b.genomic_pos > gp.back()) continue;
+      const size_t j = std::lower_bound(gp.begin(), gp.end(), b.genomic_pos) - gp.begin();
+      if (j == 0) {
+        b.pos = s.clusters[s.active_region.front()].pos;
+      } else if (j >= gp.size()) {
+        b.pos = s.clusters[s.active_region.back()].pos;
+      } else {
+        const float u = gp[j] > gp[j - 1]
+                            ? static_cast<float>(b.genomic_pos - gp[j - 1]) / static_cast<float>(gp[j] - gp[j - 1])
+                            : 0.5f;
+        const vector3 &a0 = s.clusters[s.active_region[j - 1]].pos;
+        const vector3 &a1 = s.clusters[s.active_region[j]].pos;
+        b.pos.x = (1.0f - u) * a0.x + u * a1.x;
+        b.pos.y = (1.0f - u) * a0.y + u * a1.y;
+        b.pos.z = (1.0f - u) * a0.z + u * a1.z;
+      }
+    }
   }
-  printf("\n@@drive %d %zu\n", s.total_mc_steps, ibs.size());
-  for (int ib : ibs) printf("%d ", ib);
+  printf("\n@@drive %d %zu\n", s.total_mc_steps, all_ibs.size());
+  for (int ib : all_ibs) printf("%d ", ib);
   printf("\n");
   print_clusters(s);
+}
+
+// stress: tree, chromosome name, springs "ks kq ka", "iterations lr", long
+// arcs ("n" then "start end count" each), then the positions of the
+// chromosome's anchors in cluster order.  Prints the refined active region
+// (sorted by the function) and every anchor's position.
+static void mode_stress(LooperSolver &s) {
+  const std::string chr = tok();
+  Settings::springConstantStretch = getf();
+  Settings::springConstantSqueeze = getf();
+  Settings::springAngularConstant = getf();
+  const int iters = geti();
+  const float lr = getf();
+  const int n_long = geti();
+  for (int i = 0; i < n_long; ++i) {
+    const int st = geti(), en = geti(), sc = geti();
+    s.arcs.long_arcs[chr].push_back(InteractionArc(st, en, sc));
+  }
+  const int cstart = s.chr_first_cluster[chr];
+  const int n_anc = s.arcs.anchors_cnt[chr];
+  s.active_region.clear();
+  for (int i = 0; i < n_anc; ++i) {
+    float x = getf(), y = getf(), z = getf();
+    s.clusters[cstart + i].pos.set(x, y, z);
+    s.active_region.push_back(cstart + i);
+  }
+  s.current_chr = chr;
+  fflush(stdout);
+  const auto t0 = std::chrono::steady_clock::now();
+  s.stressRefineAnchors(chr, iters, lr);
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  printf("\n@@stress %zu %.6f\n", s.active_region.size(), ms);
+  for (int c : s.active_region) printf("%d ", c);
+  printf("\n");
+  for (int i = 0; i < n_anc; ++i) {
+    const vector3 &p = s.clusters[cstart + i].pos;
+    printf("%a %a %a\n", p.x, p.y, p.z);
+  }
 }
 
 // drive_heatmap: tree, then level (0 or 1), noise_chr noise_segment
@@ -744,6 +855,9 @@ int main() {
   } else if (mode == "drive_arcs") {
     read_tree(s);
     mode_drive_arcs(s);
+  } else if (mode == "stress") {
+    read_tree(s);
+    mode_stress(s);
   } else if (mode == "drive_heatmap") {
     read_tree(s);
     mode_drive_heatmap(s);

@@ -178,7 +178,28 @@ class Cluster(C.Structure):
 class LevelReport(C.Structure):
     _fields_ = [("n_blocks", C.c_int32), ("n_jobs", C.c_int32), ("moves", C.c_int64),
                 ("accepted", C.c_int64), ("score_anchor", C.c_double), ("score_smooth", C.c_double),
-                ("ms_anchor", C.c_double), ("ms_smooth", C.c_double)]
+                ("ms_anchor", C.c_double), ("ms_smooth", C.c_double),
+                ("score_global", C.c_double), ("ms_global", C.c_double),
+                ("stress_iterations", C.c_int32), ("reinterpolated", C.c_int32)]
+
+
+class StressProblem(C.Structure):
+    _fields_ = [("n", C.c_int32), ("pos", f32p), ("backbone", f32p), ("n_pairs", C.c_int32),
+                ("pair_i", i32p), ("pair_j", i32p), ("pair_target", f32p),
+                ("k_stretch", C.c_float), ("k_squeeze", C.c_float), ("k_angular", C.c_float)]
+
+
+class StressOpts(C.Structure):
+    _fields_ = [("max_iter", C.c_int32), ("lr", C.c_float), ("in_order_sums", C.c_int32)]
+
+
+class StressStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("stop", C.c_int32), ("lr", C.c_float),
+                ("exact_sums", C.c_int32), ("total", C.c_double), ("backbone", C.c_double),
+                ("pairs", C.c_double), ("angular", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class ArcsLevelOpts(C.Structure):
@@ -186,7 +207,8 @@ class ArcsLevelOpts(C.Structure):
                 ("smooth_preset", C.c_int32), ("k_stretch_arcs", C.c_float),
                 ("k_squeeze_arcs", C.c_float), ("k_stretch", C.c_float), ("k_squeeze", C.c_float),
                 ("k_angular", C.c_float), ("w_dist", C.c_float), ("w_angle", C.c_float),
-                ("chains_per_restart", C.c_int32), ("densify", C.c_int32)]
+                ("chains_per_restart", C.c_int32), ("densify", C.c_int32),
+                ("global_refine", C.c_int32), ("stress", StressOpts)]
 
 
 class HeatmapLevelOpts(C.Structure):
@@ -231,6 +253,12 @@ SIGNATURES = {
                                          C.POINTER(SingletonCounts)]),
     "pmmc_probe_move_pattern": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int32, C.c_int32, C.c_int32,
                                           C.c_int32, C.c_int32, C.POINTER(ProbeResult)]),
+    "pmmc_stress_opts_default": (None, [C.POINTER(StressOpts)]),
+    "pmmc_stress_refine": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(StressProblem),
+                                     C.POINTER(StressOpts), C.POINTER(f32p),
+                                     C.POINTER(StressStats)]),
+    "pmmc_probe_acosf": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32,
+                                   C.POINTER(C.c_uint64)]),
     "pmmc_spline_count": (C.c_int64, [C.c_int64, C.c_int64, C.c_int32]),
     "pmmc_spline_positions": (C.c_int, [C.c_void_p, C.c_int32, i32p, f32p, C.c_int32, C.c_int32,
                                         C.c_int64, C.c_int32, C.c_int64, f32p]),
@@ -262,6 +290,9 @@ SIGNATURES = {
                                              f32p, f32p]),
     "pmmc_tree_expected_distances": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32,
                                                C.POINTER(LevelParams), f32p, f32p]),
+    "pmmc_tree_stress_refine": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32,
+                                          C.POINTER(LevelParams), C.c_float, C.c_float, C.c_float,
+                                          C.POINTER(StressOpts), C.POINTER(StressStats)]),
     "pmmc_arcs_level_opts_default": (None, [C.POINTER(ArcsLevelOpts)]),
     "pmmc_heatmap_level_opts_default": (None, [C.POINTER(HeatmapLevelOpts)]),
     "pmmc_arcs_level": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(LevelParams),

@@ -175,6 +175,47 @@ class Context:
             out.ctypes.data_as(A.f32p)))
         return out
 
+    def stress_refine(self, problems, max_iter: int, lr: float = 0.05, in_order_sums=False):
+        """pmmc_stress_refine over a list of dicts {pos [n,3], backbone [n-1],
+        pairs [(i, j, target), ...] (or arrays pair_i, pair_j, targets),
+        k_stretch, k_squeeze, k_angular}, one launch; returns
+        [(positions, stats dict), ...]."""
+        keep, cps, outs = [], [], []
+        for q in problems:
+            pos = np.ascontiguousarray(q["pos"], np.float32).reshape(-1, 3)
+            n = len(pos)
+            bb = np.ascontiguousarray(q.get("backbone", np.full(max(n - 1, 1), -1.0)), np.float32)
+            if "pair_i" in q:      # arrays: pair_i, pair_j, targets
+                pi = np.ascontiguousarray(q["pair_i"], np.int32)
+                pj = np.ascontiguousarray(q["pair_j"], np.int32)
+                pt = np.ascontiguousarray(q["targets"], np.float32)
+            else:
+                pr = np.asarray(q.get("pairs", []), np.float64).reshape(-1, 3)
+                pi = np.ascontiguousarray(pr[:, 0], np.int32)
+                pj = np.ascontiguousarray(pr[:, 1], np.int32)
+                pt = np.ascontiguousarray(q.get("targets", pr[:, 2]), np.float32)
+            out = pos.copy()
+            keep += [pos, bb, pi, pj, pt]
+            outs.append(out)
+            cps.append(A.StressProblem(n, pos.ctypes.data_as(A.f32p), bb.ctypes.data_as(A.f32p),
+                                       len(pi), pi.ctypes.data_as(A.i32p), pj.ctypes.data_as(A.i32p),
+                                       pt.ctypes.data_as(A.f32p), float(q.get("k_stretch", 0.1)),
+                                       float(q.get("k_squeeze", 0.1)),
+                                       float(q.get("k_angular", 0.1))))
+        arr = (A.StressProblem * max(1, len(cps)))(*cps)
+        optr = (A.f32p * max(1, len(outs)))(*[o.ctypes.data_as(A.f32p) for o in outs])
+        st = (A.StressStats * max(1, len(cps)))()
+        o = A.StressOpts(int(max_iter), float(lr), 1 if in_order_sums else 0)
+        self._check(self.lib.pmmc_stress_refine(self.h, len(cps), arr, C.byref(o), optr, st))
+        return [(outs[i], st[i].as_dict()) for i in range(len(cps))]
+
+    def probe_acosf(self, first_bits: int, count: int) -> int:
+        """pmmc_probe_acosf: the device arc cosine's checksum over a range of
+        float bit patterns."""
+        out = C.c_uint64()
+        self._check(self.lib.pmmc_probe_acosf(self.h, int(first_bits), int(count), C.byref(out)))
+        return int(out.value)
+
     def singleton_heatmap(self, chrs, clusters, records, region=(0, 0), centromeres=(),
                           long_arcs=(), long_scale=10.0, long_power=2.0, diag=1):
         """pmmc_singleton_heatmap: contact heatmap of a level's clusters from

@@ -25,7 +25,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpmmc_hip.so")
 SOURCES = ["engine.hip", "heatmap_reg.hip", "api.hip", "markarcs.hip", "sweep.hip", "server.hip",
            "spline.hip", "probe.hip", "singleton.hip", "noalloc.hip",
-           "arcs_pair.hip"]
+           "arcs_pair.hip", "stress.hip"]
 HOST_SOURCES = ["levels.cpp", "drivers.cpp", "metrics.cpp"]
 CXX = os.environ.get("CXX", "g++")
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wextra",

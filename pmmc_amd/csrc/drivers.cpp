@@ -10,10 +10,13 @@
 //
 // Plain host C++ (g++, -ffp-contract=off): the restart noise is computed in
 // float with separate roundings, as the reference's vector3 arithmetic.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <unordered_set>
 #include <vector>
 
 #include "levels_host.hpp"
@@ -129,6 +132,252 @@ struct Best {
   }
 };
 
+
+// stressRefineAnchors' problem over the active region `active` of chromosome
+// `chr` (LooperSolver.hpp:5642-5760): the region sorted by genomic position,
+// the pair list (arcs, then long arcs; first pair of two anchors kept) and
+// the spring targets
+struct StressInput {
+  int32_t chr = -1;
+  std::vector<int32_t> active;
+  std::vector<float> pos, bb, pt;
+  std::vector<int32_t> pi, pj;
+};
+
+void stress_input(const pmmc_tree &t, int32_t chr, std::vector<int32_t> active,
+                  const pmmc_level_params *p, float ks, float kq, StressInput &in) {
+  // std::sort with the reference's comparator, so equal genomic positions
+  // (if any) end in the same order as there
+  std::sort(active.begin(), active.end(),
+            [&](int a, int b) { return t.cl[a].genomic_pos < t.cl[b].genomic_pos; });
+  const TChr &c = t.chrs[chr];
+  const int N = static_cast<int>(active.size());
+  const int n_anc = static_cast<int>(c.anchors.size());
+  const int cstart = c.first;
+  std::vector<int> anc_to_active(n_anc, -1);
+  for (int i = 0; i < N; ++i)
+    if (active[i] >= cstart && active[i] < cstart + n_anc) anc_to_active[active[i] - cstart] = i;
+  // beads adjacent in the sorted order are left to the springs while either
+  // spring constant is on
+  const bool backbone_active = ks > 1e-6f || kq > 1e-6f;
+  std::unordered_set<int64_t> seen;
+  auto add_pair = [&](int ai, int aj, float tgt) {
+    if (ai == aj) return;
+    if (ai > aj) std::swap(ai, aj);
+    if (backbone_active && aj - ai <= 1) return;
+    if (!seen.insert(static_cast<int64_t>(ai) * N + aj).second) return;
+    in.pi.push_back(ai);
+    in.pj.push_back(aj);
+    in.pt.push_back(tgt);
+  };
+  for (size_t k = 0; k < c.arcs.size(); ++k) {
+    const int a = c.arcs[k][0], b = c.arcs[k][1];
+    if (a < cstart || a >= cstart + n_anc || b < cstart || b >= cstart + n_anc) continue;
+    const int ai = anc_to_active[a - cstart], aj = anc_to_active[b - cstart];
+    if (ai < 0 || aj < 0) continue;
+    const float tt = pmmc_freq_to_distance(c.scores[k], p);
+    if (tt > 1e-6f) add_pair(ai, aj, tt);
+  }
+  const AnchorLookup look(c);
+  for (const auto &la : c.long_arcs) {
+    const int ia = look(la[0]), ib = look(la[1]);
+    if (ia < 0 || ib < 0) continue;
+    const int as = anc_to_active[ia], bs = anc_to_active[ib];
+    if (as < 0 || bs < 0) continue;
+    const float tt = pmmc_freq_to_distance(la[2], p);
+    if (tt > 1e-6f) add_pair(as, bs, tt);
+  }
+  in.bb.assign(N > 0 ? N : 1, -1.0f);
+  for (int i = 0; i + 1 < N; ++i) {
+    const int gap = std::abs(t.cl[active[i + 1]].genomic_pos - t.cl[active[i]].genomic_pos);
+    if (gap > 0) in.bb[i] = static_cast<float>(pmmc_genomic_length_to_distance(gap, p));
+  }
+  in.pos.resize(3 * static_cast<size_t>(N));
+  for (int i = 0; i < N; ++i)
+    for (int k = 0; k < 3; ++k) in.pos[3 * i + k] = t.cl[active[i]].pos[k];
+  in.chr = chr;
+  in.active = std::move(active);
+}
+
+// one launch over every input; refined positions written back to the tree
+pmmc_status run_stress(pmmc_ctx *ctx, pmmc_tree &t, std::vector<StressInput> &ins, float ks,
+                       float kq, float ka, const pmmc_stress_opts *opts,
+                       std::vector<pmmc_stress_stats> &stats) {
+  std::vector<pmmc_stress_problem> probs(ins.size());
+  std::vector<float *> outs(ins.size());
+  for (size_t i = 0; i < ins.size(); ++i) {
+    StressInput &in = ins[i];
+    probs[i] = pmmc_stress_problem{static_cast<int32_t>(in.active.size()), in.pos.data(), in.bb.data(),
+                                   static_cast<int32_t>(in.pi.size()), in.pi.data(), in.pj.data(),
+                                   in.pt.data(), ks, kq, ka};
+    outs[i] = in.pos.data();
+  }
+  stats.assign(ins.size(), pmmc_stress_stats{});
+  const pmmc_status st = pmmc_stress_refine(ctx, static_cast<int32_t>(ins.size()), probs.data(), opts,
+                                            outs.data(), stats.data());
+  if (st != PMMC_OK) return fail(std::string("stress refinement: ") + pmmc_last_error(ctx), st);
+  for (StressInput &in : ins)
+    for (size_t i = 0; i < in.active.size(); ++i)
+      for (int k = 0; k < 3; ++k) t.cl[in.active[i]].pos[k] = in.pos[3 * i + k];
+  return PMMC_OK;
+}
+
+// the chromosome's anchors, the active region of the global refinement
+// (LooperSolver.hpp:4158-4162)
+std::vector<int32_t> chromosome_anchors(const pmmc_tree &t, int32_t c) {
+  std::vector<int32_t> a(t.chrs[c].anchors.size());
+  for (size_t i = 0; i < a.size(); ++i) a[i] = t.chrs[c].first + static_cast<int32_t>(i);
+  return a;
+}
+
+// Sub-anchor beads re-placed between the refined anchors
+// (LooperSolver.hpp:4179-4239): every bead of the anchors' level + 1 among
+// the first `limit` clusters whose genomic position lies within
+// [first anchor, last anchor] of `active` (in its order) takes the position
+// on the line between the anchors around it.  Returns the beads moved.
+int32_t reinterpolate(pmmc_tree &t, const std::vector<int32_t> &active, size_t limit) {
+  if (active.empty()) return 0;
+  const int32_t sub_level = t.cl[active[0]].level + 1;
+  std::vector<int32_t> gp(active.size());
+  for (size_t i = 0; i < active.size(); ++i) gp[i] = t.cl[active[i]].genomic_pos;
+  int32_t moved = 0;
+  for (size_t ci = 0; ci < limit && ci < t.cl.size(); ++ci) {
+    TCluster &b = t.cl[ci];
+    if (b.level != sub_level) continue;
+    const int32_t g = b.genomic_pos;
+    if (g < gp.front() || g > gp.back()) continue;
+    const size_t j = static_cast<size_t>(std::lower_bound(gp.begin(), gp.end(), g) - gp.begin());
+    if (j == 0) {
+      for (int k = 0; k < 3; ++k) b.pos[k] = t.cl[active[0]].pos[k];
+    } else if (j >= gp.size()) {
+      for (int k = 0; k < 3; ++k) b.pos[k] = t.cl[active.back()].pos[k];
+    } else {
+      const int32_t g0 = gp[j - 1], g1 = gp[j];
+      const float u = g1 > g0 ? static_cast<float>(g - g0) / static_cast<float>(g1 - g0) : 0.5f;
+      const float *p0 = t.cl[active[j - 1]].pos, *p1 = t.cl[active[j]].pos;
+      for (int k = 0; k < 3; ++k) b.pos[k] = (1.0f - u) * p0[k] + u * p1[k];
+    }
+    ++moved;
+  }
+  return moved;
+}
+
+}  // namespace
+
+extern "C" pmmc_status pmmc_tree_stress_refine(pmmc_ctx *ctx, pmmc_tree *t, int32_t chr,
+                                               const pmmc_level_params *p, float k_stretch,
+                                               float k_squeeze, float k_angular,
+                                               const pmmc_stress_opts *opts,
+                                               pmmc_stress_stats *stats) {
+  if (!ctx || !t || !p || !opts) return fail("pmmc_tree_stress_refine: null argument", PMMC_ERR_INVALID);
+  if (chr < -1 || chr >= (int32_t)t->chrs.size())
+    return fail("pmmc_tree_stress_refine: bad chromosome", PMMC_ERR_INVALID);
+  std::vector<StressInput> ins;
+  for (int32_t c = 0; c < (int32_t)t->chrs.size(); ++c) {
+    if (chr >= 0 && c != chr) continue;
+    ins.emplace_back();
+    stress_input(*t, c, chromosome_anchors(*t, c), p, k_stretch, k_squeeze, ins.back());
+  }
+  std::vector<pmmc_stress_stats> st;
+  const pmmc_status ps = run_stress(ctx, *t, ins, k_stretch, k_squeeze, k_angular, opts, st);
+  if (ps != PMMC_OK) return ps;
+  if (stats)
+    for (size_t i = 0; i < st.size(); ++i) stats[i] = st[i];
+  return PMMC_OK;
+}
+
+namespace {
+
+// Global anchor refinement of the listed chromosomes (LooperSolver.hpp:
+// 4141-4243): one batched stage of every chromosome's anchor restarts, one
+// stress launch, then the re-placement of sub-anchor beads chromosome by
+// chromosome, in the reference's order.
+pmmc_status global_stage(pmmc_ctx *ctx, pmmc_tree &t, const std::vector<int32_t> &chrs,
+                         const pmmc_level_params *p, const pmmc_arcs_level_opts *o, uint64_t seed,
+                         const std::vector<uint32_t> &gseg, const std::vector<size_t> &limit,
+                         pmmc_level_report &rep) {
+  const double t0 = now_ms();
+  std::vector<int32_t> todo;
+  for (int32_t c : chrs)
+    if (t.chrs[c].anchors.size() > 1) todo.push_back(c);
+  std::vector<std::unique_ptr<StageJob>> jobs;
+  int32_t max_n = 1;
+  for (int32_t c : todo) max_n = std::max<int32_t>(max_n, (int32_t)t.chrs[c].anchors.size());
+  const size_t n_jobs = todo.size() * (size_t)std::max(0, p->restarts_anchor);
+  const int32_t chains = chains_for(ctx, o->chains_per_restart, n_jobs, PMMC_LEVEL_ARCS, max_n);
+  for (size_t gi = 0; gi < todo.size(); ++gi) {
+    const int32_t c = todo[gi];
+    const std::vector<int32_t> active = chromosome_anchors(t, c);
+    const int n = (int)active.size();
+    std::vector<float> ed((size_t)n * n);
+    if (!expected_distances(t, c, active, p, nullptr, ed.data())) return fail(g_err, PMMC_ERR_INVALID);
+    const float noise_size = set_dist_to_next(t, active, p) * p->noise_anchor;
+    pmmc_schedule sched{};
+    pmmc_status st = pmmc_schedule_preset(PMMC_LEVEL_ARCS, o->arcs_preset, &o->arcs, noise_size, &sched);
+    if (st != PMMC_OK) return fail("pmmc_arcs_level: arc schedule preset", st);
+    for (int32_t k = 0; k < p->restarts_anchor; ++k) {
+      auto j = std::make_unique<StageJob>();
+      const uint32_t jseg = gseg[c] + (uint32_t)k;
+      j->level = PMMC_LEVEL_ARCS;
+      j->block = (int32_t)gi;
+      j->beads = active;
+      j->pos.resize((size_t)n * 3);
+      j->fixed.resize(n);
+      for (int i = 0; i < n; ++i) {
+        // restart noise on every bead at the anchor level (:4415-4420)
+        const TCluster &b = t.cl[active[i]];
+        float d[3];
+        restart_noise(seed, jseg, (uint32_t)i, p->noise_anchor_small, p->use2d != 0, d);
+        for (int a = 0; a < 3; ++a) j->pos[3 * i + a] = b.pos[a] + d[a];
+        j->fixed[i] = b.is_fixed ? 1 : 0;
+      }
+      j->pos_out = j->pos;
+      j->mat = ed;
+      j->ap = pmmc_arcs_problem{n, j->pos.data(), j->fixed.data(), j->mat.data(), 0, nullptr,
+                                nullptr, o->k_stretch_arcs, o->k_squeeze_arcs};
+      j->job.level = PMMC_LEVEL_ARCS;
+      j->job.problem = &j->ap;
+      j->job.schedule = sched;
+      j->job.run = pmmc_run{seed, jseg, 0u, chains};
+      j->job.pos_out = j->pos_out.data();
+      j->job.stats = &j->st;
+      jobs.push_back(std::move(j));
+    }
+  }
+  pmmc_status st = run_stage(ctx, jobs, &rep);
+  if (st != PMMC_OK) return st;
+  {
+    std::vector<Best> best(todo.size());
+    for (size_t ji = 0; ji < jobs.size(); ++ji) best[jobs[ji]->block].offer(jobs[ji]->st.energy, (int)ji);
+    for (size_t gi = 0; gi < todo.size(); ++gi) {
+      if (best[gi].job < 0) continue;
+      const StageJob &j = *jobs[best[gi].job];
+      for (size_t i = 0; i < j.beads.size(); ++i)
+        for (int a = 0; a < 3; ++a) t.cl[j.beads[i]].pos[a] = j.pos_out[3 * i + a];
+      rep.score_global += best[gi].score;
+    }
+  }
+  jobs.clear();
+  // stress descent over the anchors sorted by genomic position (:4175-4178)
+  std::vector<std::vector<int32_t>> order(todo.size());
+  for (size_t gi = 0; gi < todo.size(); ++gi) order[gi] = chromosome_anchors(t, todo[gi]);
+  if (o->stress.max_iter > 0) {
+    std::vector<StressInput> ins(todo.size());
+    for (size_t gi = 0; gi < todo.size(); ++gi)
+      stress_input(t, todo[gi], order[gi], p, o->k_stretch, o->k_squeeze, ins[gi]);
+    std::vector<pmmc_stress_stats> sst;
+    st = run_stress(ctx, t, ins, o->k_stretch, o->k_squeeze, o->k_angular, &o->stress, sst);
+    if (st != PMMC_OK) return st;
+    for (size_t gi = 0; gi < todo.size(); ++gi) {
+      order[gi] = ins[gi].active;
+      rep.stress_iterations += sst[gi].iterations;
+    }
+  }
+  for (size_t gi = 0; gi < todo.size(); ++gi)
+    rep.reinterpolated += reinterpolate(t, order[gi], limit[todo[gi]]);
+  rep.ms_global = now_ms() - t0;
+  return PMMC_OK;
+}
 }  // namespace
 
 extern "C" void pmmc_arcs_level_opts_default(pmmc_arcs_level_opts *o) {
@@ -156,6 +405,8 @@ extern "C" void pmmc_arcs_level_opts_default(pmmc_arcs_level_opts *o) {
   o->w_angle = 1.0f;
   o->chains_per_restart = 0;
   o->densify = 1;
+  o->global_refine = 0;  // useGlobalAnchorRefinement (Settings.hpp:406)
+  pmmc_stress_opts_default(&o->stress);
 }
 
 extern "C" void pmmc_heatmap_level_opts_default(pmmc_heatmap_level_opts *o) {
@@ -203,24 +454,39 @@ extern "C" pmmc_status pmmc_arcs_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t chr,
   // :4431-4437), so a run matches the reference's driver over the
   // reference-side glue, whose per-call segment counter starts at `segment`.
   // Jobs that are not GPU calls there take segments from 2^30 on.
-  std::vector<uint32_t> aseg(blocks.size(), 0), sseg(blocks.size(), 0);
+  // The global refinement's restarts follow each chromosome's blocks.
+  const int32_t n_chr = (int32_t)t->chrs.size();
+  std::vector<int32_t> chr_list;
+  for (int32_t c = 0; c < n_chr; ++c)
+    if (chr < 0 || c == chr) chr_list.push_back(c);
+  std::vector<uint32_t> aseg(blocks.size(), 0), sseg(blocks.size(), 0), gseg(n_chr, 0);
   {
     uint32_t call = 0, cpu = 0;
-    for (size_t bi = 0; bi < blocks.size(); ++bi) {
-      const size_t na = t->cl[blocks[bi].cluster].children.size();
-      if (na <= 1) continue;
-      aseg[bi] = segment + call;
-      call += (uint32_t)p->restarts_anchor;
-      const size_t ns = na + (na - 1) * (size_t)p->loop_density;  // after densification
-      if (ns >= 4) {
-        sseg[bi] = segment + call;
-        call += (uint32_t)p->restarts_subanchor;
-      } else {
-        sseg[bi] = segment + (1u << 30) + cpu;
-        cpu += (uint32_t)p->restarts_subanchor;
+    size_t bi = 0;
+    for (int32_t c : chr_list) {
+      for (; bi < blocks.size() && blocks[bi].chr == c; ++bi) {
+        const size_t na = t->cl[blocks[bi].cluster].children.size();
+        if (na <= 1) continue;
+        aseg[bi] = segment + call;
+        call += (uint32_t)p->restarts_anchor;
+        const size_t ns = na + (na - 1) * (size_t)p->loop_density;  // after densification
+        if (ns >= 4) {
+          sseg[bi] = segment + call;
+          call += (uint32_t)p->restarts_subanchor;
+        } else {
+          sseg[bi] = segment + (1u << 30) + cpu;
+          cpu += (uint32_t)p->restarts_subanchor;
+        }
+      }
+      if (o->global_refine && t->chrs[c].anchors.size() > 1) {
+        gseg[c] = segment + call;
+        call += (uint32_t)p->restarts_anchor;
       }
     }
   }
+  // clusters that exist when each chromosome's global pass runs there: the
+  // tree plus the sub-anchor beads of this and earlier chromosomes
+  std::vector<size_t> limit(n_chr, t->cl.size());
 
   // ---- anchor stage: every restart of every block with >= 2 children
   std::vector<std::unique_ptr<StageJob>> jobs;
@@ -289,12 +555,23 @@ extern "C" pmmc_status pmmc_arcs_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t chr,
   }
   jobs.clear();
   if (!o->densify) {
+    if (o->global_refine) {
+      st = global_stage(ctx, *t, chr_list, p, o, seed, gseg, limit, rep);
+      if (st != PMMC_OK) return st;
+    }
     if (report) *report = rep;
     return PMMC_OK;
   }
 
   // ---- densify every block, then the sub-anchor stage
-  for (const Block &b : blocks) densify(*t, b.cluster, p->loop_density, true);
+  {
+    size_t bi = 0;
+    for (int32_t c : chr_list) {
+      for (; bi < blocks.size() && blocks[bi].chr == c; ++bi)
+        densify(*t, blocks[bi].cluster, p->loop_density, true);
+      limit[c] = t->cl.size();
+    }
+  }
   max_n = 1;
   size_t n_jobs_s = 0;
   for (const Block &b : blocks) {
@@ -358,6 +635,11 @@ extern "C" pmmc_status pmmc_arcs_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t chr,
         for (int a = 0; a < 3; ++a) t->cl[j.beads[i]].pos[a] = j.pos_out[3 * i + a];
       rep.score_smooth += best[bi].score;
     }
+  }
+  jobs.clear();
+  if (o->global_refine) {
+    st = global_stage(ctx, *t, chr_list, p, o, seed, gseg, limit, rep);
+    if (st != PMMC_OK) return st;
   }
   if (report) *report = rep;
   return PMMC_OK;

@@ -673,6 +673,33 @@ extern "C" pmmc_status pmmc_tree_set_dist_to_next(pmmc_tree *t, int32_t cluster,
 
 namespace pmmc_host {
 
+// first anchor containing x (Anchor::contains, Anchor.hpp:87: start <= x <=
+// end); binary search when the anchors are sorted and disjoint, where it
+// finds the same anchor
+AnchorLookup::AnchorLookup(const TChr &chr) : c(chr) {
+  const int n_anc = (int)c.anchors.size();
+  for (int k = 0; k + 1 < n_anc && disjoint; ++k)
+    disjoint = c.anchors[k][0] <= c.anchors[k][1] && c.anchors[k][1] < c.anchors[k + 1][0];
+}
+
+int AnchorLookup::operator()(int32_t x) const {
+  const int n_anc = (int)c.anchors.size();
+  if (disjoint) {
+    int lo = 0, hi = n_anc;
+    while (lo < hi) {
+      const int mid = (lo + hi) / 2;
+      if (c.anchors[mid][1] < x)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    return (lo < n_anc && c.anchors[lo][0] <= x && x <= c.anchors[lo][1]) ? lo : -1;
+  }
+  for (int k = 0; k < n_anc; ++k)
+    if (x >= c.anchors[k][0] && x <= c.anchors[k][1]) return k;
+  return -1;
+}
+
 // LooperSolver::calcAnchorExpectedDistancesHeatmap (LooperSolver.hpp:5494-5627)
 bool expected_distances(const pmmc_tree &t, int32_t chr, const std::vector<int32_t> &active,
                         const pmmc_level_params *p, const float *anchor_heatmap, float *m) {
@@ -705,27 +732,7 @@ bool expected_distances(const pmmc_tree &t, int32_t chr, const std::vector<int32
     std::vector<int32_t> a2a(n_anc, -1);
     for (const auto &kv : c2a)
       if (kv.first >= c.first && kv.first < c.first + n_anc) a2a[kv.first - c.first] = kv.second;
-    // first anchor containing x (Anchor::contains, Anchor.hpp:87); binary
-    // search when the anchors are sorted and disjoint, where it is the same
-    bool disjoint = true;
-    for (int k = 0; k + 1 < n_anc && disjoint; ++k)
-      disjoint = c.anchors[k][0] <= c.anchors[k][1] && c.anchors[k][1] < c.anchors[k + 1][0];
-    auto pos_to_anc = [&](int32_t x) -> int {
-      if (disjoint) {
-        int lo = 0, hi = n_anc;
-        while (lo < hi) {
-          const int mid = (lo + hi) / 2;
-          if (c.anchors[mid][1] < x)
-            lo = mid + 1;
-          else
-            hi = mid;
-        }
-        return (lo < n_anc && c.anchors[lo][0] <= x && x <= c.anchors[lo][1]) ? lo : -1;
-      }
-      for (int k = 0; k < n_anc; ++k)
-        if (x >= c.anchors[k][0] && x <= c.anchors[k][1]) return k;
-      return -1;
-    };
+    const AnchorLookup pos_to_anc(c);
     for (const auto &la : c.long_arcs) {
       const int ia = pos_to_anc(la[0]), ib = pos_to_anc(la[1]);
       if (ia < 0 || ib < 0 || ia == ib) continue;

@@ -43,6 +43,14 @@ struct pmmc_tree {
 };
 
 namespace pmmc_host {
+// first anchor of a chromosome containing a genomic position, -1 for none
+struct AnchorLookup {
+  explicit AnchorLookup(const TChr &chr);
+  int operator()(int32_t x) const;
+  const TChr &c;
+  bool disjoint = true;
+};
+
 extern thread_local std::string g_err;
 pmmc_status fail(const std::string &msg, pmmc_status st);
 bool valid_cluster(const pmmc_tree *t, int32_t c);

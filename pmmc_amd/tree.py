@@ -245,6 +245,20 @@ def heatmap_level_opts(**overrides) -> A.HeatmapLevelOpts:
     return o
 
 
+def stress_refine(ctx, tree: Tree, chr_index: int, p: A.LevelParams, k_stretch=0.1,
+                  k_squeeze=0.1, k_angular=0.1, max_iter=0, lr=0.05, in_order_sums=False):
+    """pmmc_tree_stress_refine on a pmmc_amd.api.Context: the anchors of
+    chromosome chr_index (-1: all, one launch) refined in place; returns the
+    stats dict of each chromosome processed."""
+    n = tree.n_chromosomes if chr_index < 0 else 1
+    st = (A.StressStats * max(1, n))()
+    o = A.StressOpts(int(max_iter), float(lr), 1 if in_order_sums else 0)
+    ctx._check(_lib().pmmc_tree_stress_refine(ctx.h, tree.handle, int(chr_index), C.byref(p),
+                                              float(k_stretch), float(k_squeeze), float(k_angular),
+                                              C.byref(o), st))
+    return [st[i].as_dict() for i in range(n)]
+
+
 def _report(r: A.LevelReport) -> dict:
     return {k: getattr(r, k) for k, _ in A.LevelReport._fields_}
 

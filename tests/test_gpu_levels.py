@@ -181,6 +181,64 @@ def test_arcs_level_single_chromosome_anchor_stage_only(ctx):
     assert_tree_positions_match(pt, ot)
 
 
+def run_job_product(ctx):
+    """The product's single-problem call as the oracle driver's job runner:
+    the oracle's driver logic (segments, restarts, best-of, the global stage)
+    over the same kernels the batched driver launches."""
+    def run(level, prob, sched, seed, seg, n_chains):
+        ap = {1: api.Heatmap, 2: api.Arcs, 3: api.Smooth}[level](**prob)
+        pos, st = ctx.anneal(ap, sched, seed=seed, segment=seg, n_chains=n_chains)
+        return dict(pos=pos, energy=st["energy"], resummed=st["energy_resummed"])
+    return run
+
+
+def global_genome(seed):
+    """Two chromosomes with overlapping genomic ranges (as real chromosomes
+    have), long arcs, and arc counts 2-6 (targets of a few units)."""
+    case, rng = genome(seed, n_chr=2, n_anchors=(40, 32), span=5, splits_per_chr=2)
+    for c in case["chrs"]:
+        c["arcs"] = [(a, b, int(rng.integers(2, 7))) for a, b, _ in c["arcs"]]
+        anc, longs = c["anchors"], []
+        for _ in range(8):
+            a, b = sorted(int(x) for x in rng.integers(0, len(anc), size=2))
+            longs.append((int(rng.integers(anc[a][0], anc[a][1] + 1)),
+                          int(rng.integers(anc[b][0], anc[b][1] + 1)), int(rng.integers(2, 7))))
+        c["long_arcs"] = longs
+    return case, rng
+
+
+@pytest.mark.parametrize("stress_iters,densify", [(400, 1), (0, 1), (300, 0)])
+def test_arcs_level_global_refinement_matches_oracle_driver(ctx, stress_iters, densify):
+    case, rng = global_genome(13)
+    pt = T.Tree(case["chrs"], case["splits"])
+    ot = L.build_tree(case["chrs"], case["splits"])
+    ib_positions(pt, ot, rng)
+    po, oo = arcs_opts(steps=1024, chains=64)
+    po.densify = densify
+    po.global_refine = 1
+    po.stress.max_iter = stress_iters
+    po.stress.lr = 0.05
+    oo.update(densify=bool(densify), global_refine=True, stress_iters=stress_iters, stress_lr=0.05)
+    print(f"global refinement stress_iters={stress_iters} densify={densify}", flush=True)
+    rep = T.arcs_level(ctx, pt, -1, product_params(PARAMS), po, seed=31, segment=200)
+    orep = L.arcs_level(ot, None, PARAMS, oo, 31, 200, run_job_product(ctx))
+    assert len(pt) == len(ot["clusters"])
+    n = len(pt)
+    got = pt.positions(list(range(n)))
+    want = np.array([[float(x) for x in ot["clusters"][i]["pos"]] for i in range(n)], np.float32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert rep["n_jobs"] == orep["n_jobs"]
+    assert rep["score_global"] == orep["score_global"]
+    assert rep["stress_iterations"] == orep["stress_iterations"]
+    assert rep["reinterpolated"] == orep["reinterpolated"]
+    if densify:
+        # beads of both chromosomes were re-placed: chr2's anchor range
+        # covers chr1's sub-anchor beads too, as the reference's range test has it
+        assert rep["reinterpolated"] > 0
+    if stress_iters:
+        assert rep["stress_iterations"] > 0
+
+
 @pytest.mark.parametrize("level", [0, 1])
 def test_heatmap_level_matches_oracle(ctx, level):
     case, rng = genome(21, n_chr=2, n_anchors=(40, 30), span=3, splits_per_chr=4)

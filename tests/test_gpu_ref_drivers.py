@@ -180,10 +180,12 @@ def arcs_case(seed):
     return case, rng
 
 
-def ref_drive_arcs(case, params, seed, segment, ib_pos):
+def ref_drive_arcs(case, params, seed, segment, ib_pos, chrom=None, glob=None):
+    """glob: (stress iterations, stress lr) to run the global refinement, with
+    the case's long arcs; chrom "*" drives every chromosome."""
     p = params
     text = header("drive_arcs", p) + tree_text(case)
-    text += f"{case['chrs'][0]['name']}\n"
+    text += f"{chrom or case['chrs'][0]['name']}\n"
     text += (f"{p['loop_density']} {f32(p['noise_anchor'])!r} {f32(p['noise_subanchor'])!r} "
              f"{f32(p['noise_anchor_small'])!r} {p['restarts_anchor']} {p['restarts_subanchor']} "
              f"{p['use2d']} {FAILS}\n")
@@ -191,21 +193,31 @@ def ref_drive_arcs(case, params, seed, segment, ib_pos):
     text += " ".join(repr(f32(SPRINGS[k])) for k in ("ks_arcs", "kq_arcs", "ks", "kq", "ka", "wd", "wa"))
     text += f"\n{seed} {segment} {CHAINS}\n"
     text += " ".join(float(x).hex() for x in np.asarray(ib_pos, np.float32).reshape(-1)) + "\n"
+    if glob is not None:
+        longs = [(c["name"], s0, e0, sc) for c in case["chrs"] for s0, e0, sc in c.get("long_arcs", [])]
+        text += f"global 1 {glob[0]} {f32(glob[1])!r} {len(longs)}\n"
+        text += "".join(f"{n} {s0} {e0} {sc}\n" for n, s0, e0, sc in longs)
     lines, calls = run_ref(text)
     steps, ibs, cl = parse_drive(lines)
     return steps, ibs, cl, calls
 
 
-def product_drive_arcs(ctx, case, params, seed, segment, ib_pos):
+def product_drive_arcs(ctx, case, params, seed, segment, ib_pos, all_chrs=False, glob=None):
     pt = T.Tree(case["chrs"], case["splits"])
-    ibs = pt.level(0, 2)
+    chrs = range(len(case["chrs"])) if all_chrs else [0]
+    ibs = [ib for c in chrs for ib in pt.level(c, 2)]
     pt.set_positions(ibs, np.asarray(ib_pos, np.float32))
     po = T.arcs_level_opts(arcs=lvl_settings(ARCS), smooth=lvl_settings(SMOOTH),
                            chains_per_restart=CHAINS, k_stretch_arcs=SPRINGS["ks_arcs"],
                            k_squeeze_arcs=SPRINGS["kq_arcs"], k_stretch=SPRINGS["ks"],
                            k_squeeze=SPRINGS["kq"], k_angular=SPRINGS["ka"], w_dist=SPRINGS["wd"],
                            w_angle=SPRINGS["wa"])
-    rep = T.arcs_level(ctx, pt, 0, product_params(params), po, seed=seed, segment=segment)
+    if glob is not None:
+        po.global_refine = 1
+        po.stress.max_iter = glob[0]
+        po.stress.lr = glob[1]
+    rep = T.arcs_level(ctx, pt, -1 if all_chrs else 0, product_params(params), po, seed=seed,
+                       segment=segment)
     return pt, ibs, rep
 
 
@@ -260,6 +272,56 @@ def test_ref_arcs_driver_noise_free_bit_exact(ctx, seed):
              for k, ib in enumerate(ibs) if len(rcl[ib]["children"]) > 1]
     assert moved and all(moved)
     assert rep["score_anchor"] == pytest.approx(block_scores(calls, 2), rel=1e-6)
+    check_glue_calls(ctx, calls)
+
+
+def global_case(seed):
+    """Two chromosomes over the same genomic range, long arcs, counts 2-6."""
+    case, rng = genome(seed, n_chr=2, n_anchors=(24, 20), span=4, splits_per_chr=2)
+    for c in case["chrs"]:
+        c["arcs"] = [(a, b, int(rng.integers(2, 7))) for a, b, _ in c["arcs"]]
+        anc, longs = c["anchors"], []
+        for _ in range(6):
+            a, b = sorted(int(x) for x in rng.integers(0, len(anc), size=2))
+            longs.append((int(rng.integers(anc[a][0], anc[a][1] + 1)),
+                          int(rng.integers(anc[b][0], anc[b][1] + 1)), int(rng.integers(2, 7))))
+        c["long_arcs"] = longs
+    return case, rng
+
+
+@pytest.mark.parametrize("stress_iters", [0, 600])
+def test_ref_arcs_driver_global_refinement_bit_exact(ctx, stress_iters):
+    """The reference's driver with useGlobalAnchorRefinement over two
+    chromosomes, noise-free: per chromosome its blocks, then the global pass
+    (calcAnchorExpectedDistancesHeatmap and reconstructClusterArcsDistances
+    over all anchors through the glue, stressRefineAnchors, the sub-anchor
+    re-placement), against pmmc_arcs_level with global_refine: every cluster's
+    position, fixed flag and dist_to_next, bit for bit."""
+    case, rng = global_case(17)
+    params = dict(PARAMS, noise_anchor_small=0.0, noise_subanchor=0.0, restarts_anchor=2,
+                  restarts_subanchor=2, loop_density=2)
+    pt0 = T.Tree(case["chrs"], case["splits"])
+    n_ib = sum(len(pt0.level(c, 2)) for c in range(2))
+    ib_pos = np.cumsum(rng.normal(0, 1.0, size=(n_ib, 3)), 0).astype(np.float32)
+    print(f"reference global refinement stress_iters={stress_iters}", flush=True)
+    _, ibs, rcl, calls = ref_drive_arcs(case, params, 41, 300, ib_pos, chrom="*",
+                                        glob=(stress_iters, 0.05))
+    pt, pibs, rep = product_drive_arcs(ctx, case, params, 41, 300, ib_pos, all_chrs=True,
+                                       glob=(stress_iters, 0.05))
+    assert ibs == pibs and len(pt) == len(rcl)
+    pcl = pt.clusters()
+    for i, (a, b) in enumerate(zip(pcl, rcl)):
+        assert bool(a.is_fixed) == bool(b["fixed"]), i
+        assert np.float32(a.dist_to_next) == np.float32(b["dtn"]), i
+    got = pt.positions(list(range(len(pt))))
+    want = np.array([c["pos"] for c in rcl], np.float32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert rep["reinterpolated"] > 0
+    if stress_iters:
+        assert rep["stress_iterations"] > 0
+    # the global restarts are GPU calls of the anchor level over all anchors
+    n_anc = [len(c["anchors"]) for c in case["chrs"]]
+    assert sum(1 for c in calls if c["level"] == 2 and c["n"] in n_anc) >= 2 * params["restarts_anchor"]
     check_glue_calls(ctx, calls)
 
 
