@@ -45,6 +45,11 @@
 //             of sub-anchor beads between the anchors, which the reference
 //             writes inline (:4179-4239) and this driver restates.  The
 //             chromosome name "*" drives every chromosome in order.
+//   records   Cluster::toFile / toFilePreviousFormat (Cluster.hpp:175-189) and
+//             InteractionArc::toFile (InteractionArc.hpp:122-125) of given
+//             clusters and arcs, as the HCM writers emit them
+//   heat      Heatmap::toFile (Heatmap.hpp:679-700) of a given matrix to a
+//             given path, then Heatmap::fromFile (:702-732) of a given path
 //   stress    tree, then LooperSolver::stressRefineAnchors (:5638-5915) over
 //             one chromosome's anchors with given positions and long arcs
 //   drive_heatmap
@@ -710,6 +715,75 @@ static void mode_drive_arcs(LooperSolver &s) {
   print_clusters(s);
 }
 
+// records: n, then per cluster "start end orientation x y z k children...",
+// then m, then per arc "start end genomic_start genomic_end score eff_score
+// factor".  Prints each record set after a marker line.
+static void mode_records() {
+  const int n = geti();
+  std::vector<Cluster> cl(n);
+  for (int i = 0; i < n; ++i) {
+    Cluster &c = cl[i];
+    c.start = geti();
+    c.end = geti();
+    c.orientation = tok()[0];
+    const float x = getf(), y = getf(), z = getf();
+    c.pos.set(x, y, z);
+    const int k = geti();
+    for (int j = 0; j < k; ++j) c.children.push_back(geti());
+  }
+  const int m = geti();
+  std::vector<InteractionArc> arcs(m);
+  for (int i = 0; i < m; ++i) {
+    InteractionArc &a = arcs[i];
+    a.start = geti();
+    a.end = geti();
+    a.genomic_start = geti();
+    a.genomic_end = geti();
+    a.score = geti();
+    a.eff_score = geti();
+    a.factor = geti();
+  }
+  fflush(stdout);
+  printf("@@cluster\n");
+  fflush(stdout);
+  for (Cluster &c : cl) c.toFile(stdout);
+  fflush(stdout);
+  printf("@@cluster_previous\n");
+  fflush(stdout);
+  for (Cluster &c : cl) c.toFilePreviousFormat(stdout);
+  fflush(stdout);
+  printf("@@arc\n");
+  fflush(stdout);
+  for (InteractionArc &a : arcs) a.toFile(stdout);
+  fflush(stdout);
+  printf("@@end\n");
+}
+
+// heat: path_out n total_count zero_diagonal as_integers, n*n values, then
+// path_in.  Writes the matrix to path_out with Heatmap::toFile, then reads
+// path_in with Heatmap::fromFile and prints size, start, resolution,
+// diagonal size and the values.
+static void mode_heat() {
+  const std::string out = tok();
+  const int n = geti();
+  const bool total = geti() != 0, zero = geti() != 0, ints = geti() != 0;
+  Heatmap h;
+  h.init(n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) h.v[i][j] = getf();
+  h.toFile(out, total, zero, ints);
+  const std::string in = tok();
+  Heatmap r;
+  r.start = -1;
+  r.resolution = -1;
+  r.fromFile(in);
+  printf("\n@@heat %zu %d %d %d\n", r.size, r.start, r.resolution, r.diagonal_size);
+  for (size_t i = 0; i < r.size; ++i) {
+    for (size_t j = 0; j < r.size; ++j) printf("%a ", r.v[i][j]);
+    printf("\n");
+  }
+}
+
 // stress: tree, chromosome name, springs "ks kq ka", "iterations lr", long
 // arcs ("n" then "start end count" each), then the positions of the
 // chromosome's anchors in cluster order.  Prints the refined active region
@@ -855,6 +929,10 @@ int main() {
   } else if (mode == "drive_arcs") {
     read_tree(s);
     mode_drive_arcs(s);
+  } else if (mode == "records") {
+    mode_records();
+  } else if (mode == "heat") {
+    mode_heat();
   } else if (mode == "stress") {
     read_tree(s);
     mode_stress(s);

@@ -301,6 +301,18 @@ SIGNATURES = {
     "pmmc_heatmap_level": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, f32p, C.c_int32,
                                      C.POINTER(LevelParams), C.POINTER(HeatmapLevelOpts), C.c_uint64,
                                      C.c_uint32, C.POINTER(LevelReport)]),
+    # include/pmmc_formats.h
+    "pmmc_tree_set_factors": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_char_p)]),
+    "pmmc_tree_set_arc_info": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(Arc)]),
+    "pmmc_tree_set_annotation": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p]),
+    "pmmc_tree_write_hcm": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
+    "pmmc_tree_read_hcm": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "pmmc_tree_write_pdb": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p]),
+    "pmmc_tree_write_cif": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_char_p]),
+    "pmmc_write_pdb_flat": (C.c_int, [f32p, C.c_int32, C.c_char_p, C.c_char_p]),
+    "pmmc_write_cif_flat": (C.c_int, [f32p, i32p, C.c_int32, C.c_int32, C.c_char_p, C.c_char_p]),
+    "pmmc_heat_write": (C.c_int, [f32p, C.c_int32, C.c_char_p, C.c_int32, C.c_int32, C.c_int32]),
+    "pmmc_heat_read": (C.c_int, [C.c_char_p, f32p, C.c_int64, i32p, i32p, i32p, i32p]),
     "pmmc_pearson": (C.c_float, [f32p, f32p, C.c_int64]),
     "pmmc_distance_correlation": (C.c_float, [f32p, f32p, C.c_int32, C.c_int32]),
     "pmmc_distance_map": (C.c_int, [f32p, C.c_int32, f32p]),

@@ -26,7 +26,7 @@ LIB = os.path.join(HERE, "libpmmc_hip.so")
 SOURCES = ["engine.hip", "heatmap_reg.hip", "api.hip", "markarcs.hip", "sweep.hip", "server.hip",
            "spline.hip", "probe.hip", "singleton.hip", "noalloc.hip",
            "arcs_pair.hip", "stress.hip"]
-HOST_SOURCES = ["levels.cpp", "drivers.cpp", "metrics.cpp"]
+HOST_SOURCES = ["levels.cpp", "drivers.cpp", "metrics.cpp", "formats.cpp"]
 CXX = os.environ.get("CXX", "g++")
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wextra",
               f"-I{os.path.join(ROOT, 'include')}"]

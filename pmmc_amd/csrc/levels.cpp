@@ -441,6 +441,8 @@ extern "C" pmmc_status pmmc_tree_create(int32_t n_chr, const pmmc_chromosome *ch
       }
       c.arcs.push_back({in.arc_start[i], in.arc_end[i]});
       c.scores.push_back(in.arc_score[i]);
+      // InteractionArc(start, end, score) (InteractionArc.hpp:103-120)
+      c.arc_info.push_back({-1, -1, in.arc_score[i], -1});
     }
     for (int32_t i = 0; i < in.n_long_arcs; ++i)
       c.long_arcs.push_back({in.long_start[i], in.long_end[i], in.long_score[i]});

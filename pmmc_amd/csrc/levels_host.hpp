@@ -17,6 +17,7 @@ struct TCluster {
   char orientation = 'N';
   float pos[3] = {0.0f, 0.0f, 0.0f};
   double dist_to_next = 0.0;
+  int32_t annotation = 0;  // Cluster::annotation_type
   std::vector<int32_t> children;
   std::vector<int32_t> arcs;  // chromosome-local arc indices touching this cluster
 };
@@ -28,6 +29,9 @@ struct TChr {
   std::vector<char> orient;
   std::vector<std::array<int32_t, 2>> arcs;       // global cluster indices once built
   std::vector<int32_t> scores;
+  // per arc: genomic start, genomic end, effective score, factor
+  // (InteractionArc fields the files carry)
+  std::vector<std::array<int32_t, 4>> arc_info;
   std::vector<std::array<int32_t, 3>> long_arcs;  // bp, bp, count
 };
 
@@ -40,6 +44,7 @@ struct pmmc_tree {
   std::vector<TCluster> cl;
   std::vector<TChr> chrs;
   std::vector<TRegion> regions;
+  std::vector<std::string> factors;  // InteractionArcs::factors
 };
 
 namespace pmmc_host {

@@ -126,6 +126,46 @@ class Tree:
         self.handle = h
         self._keep = []
 
+    @classmethod
+    def from_hcm(cls, path: str, fmt: int = 2) -> "Tree":
+        """pmmc_tree_read_hcm: fmt 0 previous layout, 1 new, 2 the
+        reference's detection rule."""
+        t = cls.__new__(cls)
+        t._keep = []
+        h = C.c_void_p()
+        _check(_lib().pmmc_tree_read_hcm(str(path).encode(), int(fmt), C.byref(h)), "pmmc_tree_read_hcm")
+        t.handle = h
+        return t
+
+    def set_factors(self, names):
+        arr = (C.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
+        _check(_lib().pmmc_tree_set_factors(self.handle, len(names), arr), "pmmc_tree_set_factors")
+
+    def set_arc_info(self, chr_index, arcs):
+        """arcs: [(start, end, genomic_start, genomic_end, score, eff_score,
+        factor)] in the chromosome's arc order."""
+        arr = (A.Arc * max(1, len(arcs)))(*[A.Arc(*map(int, a)) for a in arcs])
+        _check(_lib().pmmc_tree_set_arc_info(self.handle, int(chr_index), len(arcs), arr),
+               "pmmc_tree_set_arc_info")
+
+    def set_annotation(self, ids, types):
+        i = _i32(ids)
+        t = _i32(types)
+        _check(_lib().pmmc_tree_set_annotation(self.handle, len(i), i.ctypes.data_as(_i32p),
+                                               t.ctypes.data_as(_i32p)), "pmmc_tree_set_annotation")
+
+    def write_hcm(self, path, fmt: int = 1):
+        _check(_lib().pmmc_tree_write_hcm(self.handle, str(path).encode(), int(fmt)),
+               "pmmc_tree_write_hcm")
+
+    def write_pdb(self, path, level: int = -1):
+        _check(_lib().pmmc_tree_write_pdb(self.handle, int(level), str(path).encode()),
+               "pmmc_tree_write_pdb")
+
+    def write_cif(self, path, level: int = -1, data_name: str = "cudaMMC"):
+        _check(_lib().pmmc_tree_write_cif(self.handle, int(level), data_name.encode(),
+                                          str(path).encode()), "pmmc_tree_write_cif")
+
     def close(self):
         if self.handle:
             _lib().pmmc_tree_destroy(self.handle)
@@ -280,3 +320,37 @@ def heatmap_level(ctx, tree: Tree, level: int, dist, diagonal: int, p: A.LevelPa
                                          int(diagonal), C.byref(p), C.byref(o), int(seed),
                                          int(segment), C.byref(rep)))
     return _report(rep)
+
+
+# ------------------------------------------------------------------ flat files
+def write_pdb_flat(xyz, chr_name, path):
+    x = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    _check(_lib().pmmc_write_pdb_flat(x.ctypes.data_as(_f32p), len(x), chr_name.encode(),
+                                      str(path).encode()), "pmmc_write_pdb_flat")
+
+
+def write_cif_flat(xyz, genomic_pos, chr_name, path):
+    x = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    g = _i32(genomic_pos)
+    _check(_lib().pmmc_write_cif_flat(x.ctypes.data_as(_f32p), g.ctypes.data_as(_i32p), len(g),
+                                      len(x), chr_name.encode(), str(path).encode()),
+           "pmmc_write_cif_flat")
+
+
+def heat_write(h, path, header=True, zero_diagonal=False, as_integers=False):
+    a = np.ascontiguousarray(h, np.float32)
+    n = a.shape[0] if a.size else 0
+    _check(_lib().pmmc_heat_write(a.ctypes.data_as(_f32p), n, str(path).encode(), int(header),
+                                  int(zero_diagonal), int(as_integers)), "pmmc_heat_write")
+
+
+def heat_read(path):
+    """pmmc_heat_read: (matrix, start or None, resolution or None, diagonal size)."""
+    n = C.c_int32()
+    st, res, diag = C.c_int32(-1), C.c_int32(-1), C.c_int32()
+    _check(_lib().pmmc_heat_read(str(path).encode(), None, 0, C.byref(n), C.byref(st), C.byref(res),
+                                 None), "pmmc_heat_read")
+    out = np.empty((n.value, n.value), np.float32)
+    _check(_lib().pmmc_heat_read(str(path).encode(), out.ctypes.data_as(_f32p), out.size, C.byref(n),
+                                 C.byref(st), C.byref(res), C.byref(diag)), "pmmc_heat_read")
+    return out, (st.value if st.value != -1 else None), (res.value if res.value != -1 else None), diag.value
