@@ -325,8 +325,79 @@ def singleton_leg(ctx, with_cpu=True, cpu_seconds=8.0):
     return out
 
 
+def stress_case(n, seed):
+    """One chromosome of n anchors 4-30 kb apart, arcs spanning up to 6
+    anchors with PET counts 2-6, n/4 long arcs, two segments, and a random
+    walk of start positions."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    gaps = rng.integers(4000, 30000, size=n)
+    lens = rng.integers(500, 3000, size=n)
+    starts = 10_000 + np.cumsum(gaps + lens) - lens
+    anchors = [(int(a), int(a + b)) for a, b in zip(starts, lens)]
+    pairs = sorted({(int(a), int(min(n - 1, a + rng.integers(1, 7))))
+                    for a in rng.integers(0, n - 1, size=n)})
+    arcs = [(a, b, int(rng.integers(2, 7))) for a, b in pairs]
+    longs = []
+    for _ in range(n // 4):
+        a, b = sorted(int(x) for x in rng.integers(0, n, size=2))
+        longs.append((int(rng.integers(anchors[a][0], anchors[a][1] + 1)),
+                      int(rng.integers(anchors[b][0], anchors[b][1] + 1)), int(rng.integers(2, 7))))
+    lo, hi = anchors[0][0] - 1000, anchors[-1][1] + 1000
+    cut = int(rng.integers(lo + 1, hi))
+    case = dict(anchors=anchors, arcs=arcs, long_arcs=longs,
+                splits=[("chr1", lo, cut), ("chr1", cut, hi)])
+    pos = np.cumsum(rng.normal(0, 2.0, size=(n, 3)), 0).astype(np.float32)
+    return case, pos
+
+
+def stress_leg(ctx, with_cpu=True, cpu_seconds=8.0):
+    """LooperSolver::stressRefineAnchors (the global anchor refinement's
+    gradient descent) on one chromosome of 20,000 anchors: pmmc_tree_stress_
+    refine (set-up on the host, every iteration in one cooperative launch) vs
+    the reference's own function on one host thread, same inputs; the
+    positions are compared bit for bit."""
+    import numpy as np
+    from pmmc_amd import tree as T
+    del cpu_seconds
+    n, iters, lr, ks, kq, ka = 20000, 300, 0.01, 0.1, 0.1, 0.1
+    case, pos = stress_case(n, 20000)
+    t = T.Tree([dict(name="chr1", anchors=case["anchors"], arcs=case["arcs"],
+                     long_arcs=case["long_arcs"])], case["splits"])
+    first, _ = t.chromosome(0)
+    ids = list(range(first, first + n))
+    p = T.level_params()
+    runs = []
+    for _ in range(3):
+        t.set_positions(ids, pos)
+        t0 = time.perf_counter()
+        st = T.stress_refine(ctx, t, 0, p, ks, kq, ka, max_iter=iters, lr=lr)[0]
+        runs.append(time.perf_counter() - t0)
+    wall = min(runs)
+    got = t.positions(ids)
+    leg = {"workload": f"stress refinement of {n} anchors ({len(case['arcs'])} arcs, "
+                       f"{len(case['long_arcs'])} long arcs), max {iters} iterations, step {lr}",
+           "iterations": st["iterations"], "stop": st["stop"], "ms_per_call": wall * 1e3,
+           "iterations_per_s": st["iterations"] / wall, "energy": st["total"]}
+    if with_cpu:
+        from oracle import refdriver
+        if refdriver.available():
+            params = {k: getattr(p, {"stretching": "heat_stretching"}.get(k, k))
+                      for k in refdriver.PARAM_KEYS}
+            ms, _, want, _ = refdriver.stress(case, params, ks, kq, ka, iters, lr, pos)
+            leg["identical_to_reference"] = bool(np.array_equal(got.view(np.uint32),
+                                                                want.view(np.uint32)))
+            leg["cpu_reference"] = {
+                "value": st["iterations"] / (ms * 1e-3), "unit": "iterations/s", "cores": 1,
+                "kind": "reference", "ms_per_call": ms,
+                "sample": "LooperSolver::stressRefineAnchors on the same chromosome and start, "
+                          "one host thread (set-up included, as the call runs)"}
+            leg["speedup_vs_reference"] = ms * 1e-3 / wall
+    return leg
+
+
 LEGS = {"heatmap_64": heatmap_leg, "heatmap_64_sweep": heatmap_sweep_leg,
-        "smooth_19951": smooth_leg, "singleton_20M": singleton_leg}
+        "smooth_19951": smooth_leg, "singleton_20M": singleton_leg, "stress_20k": stress_leg}
 # legs runnable alone with --only (not part of `levels`)
 ONLY = dict(LEGS, batched=lambda ctx, with_cpu=True: batched(ctx),
             served=lambda ctx, with_cpu=True: served(ctx))

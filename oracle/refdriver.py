@@ -3,8 +3,8 @@
 oracle/_ref/levels_ref is the reference's LooperSolver compiled in place from
 its sources by oracle/Makefile (target `ref`; only where the reference
 sources exist, the binary then travels with the tree).  This module formats
-its `anneal` / `time_anneal` / `singleton_clusters` input and reads its
-output.  Used by tests/golden/make_anneal_fixtures.py (fixtures) and
+its `anneal` / `time_anneal` / `singleton_clusters` / `stress` input and
+reads its output.  Used by tests/golden/make_anneal_fixtures.py (fixtures) and
 bench.py's CPU legs (the reference's CPU annealing, and its singleton
 heatmap construction, timed on the host's cores).
 """
@@ -114,3 +114,37 @@ def singleton_time(chrs, clusters, rec, diag=1, long_scale=10.0, long_power=2.0)
     out = run("".join(parts), "@@singleton_time")
     _, secs, cnt = out[0].split()
     return float(secs), int(cnt)
+
+
+PARAM_KEYS = ["count_a", "count_scale", "count_shift", "count_base", "heat_scale", "heat_power",
+              "heat_scale_inter", "heat_power_inter", "stretching", "glen_base", "glen_scale",
+              "glen_power"]
+
+
+def stress(case: dict, params: dict, ks: float, kq: float, ka: float, iters: int, lr: float, pos):
+    """LooperSolver::stressRefineAnchors run once by the reference on one
+    chromosome: case {anchors [(start, end)], arcs [(a, b, count)] with local
+    anchor indices, long_arcs [(start, end, count)], splits [(chr, start,
+    end)]} named chr1, the anchors' start positions pos [n, 3].  Returns
+    (milliseconds inside the call, sorted anchor order, positions [n, 3],
+    the function's progress lines)."""
+    f32 = lambda x: repr(float(np.float32(x)))  # noqa: E731
+    c = case
+    text = "stress " + " ".join(f32(params[k]) for k in PARAM_KEYS) + "\n"
+    text += f"1\nchr1 {len(c['anchors'])} {len(c['arcs'])}\n"
+    text += "".join(f"{s} {e} N\n" for s, e in c["anchors"])
+    text += "".join(f"{a} {b} {sc}\n" for a, b, sc in c["arcs"])
+    text += f"{len(c['splits'])}\n" + "".join(f"{n} {s} {e}\n" for n, s, e in c["splits"])
+    text += f"chr1\n{f32(ks)} {f32(kq)} {f32(ka)}\n{iters} {f32(lr)}\n{len(c['long_arcs'])}\n"
+    text += "".join(f"{s} {e} {sc}\n" for s, e, sc in c["long_arcs"])
+    text += " ".join(float(v).hex() for v in np.asarray(pos, np.float32).reshape(-1)) + "\n"
+    r = subprocess.run([REF], input=text, capture_output=True, text=True, check=True)
+    lines = r.stdout.splitlines()
+    k = next(i for i, ln in enumerate(lines) if ln.startswith("@@stress"))
+    n = len(c["anchors"])
+    ms = float(lines[k].split()[2])
+    order = [int(x) for x in lines[k + 1].split()]
+    out = np.array([[float.fromhex(v) for v in ln.split()] for ln in lines[k + 2:k + 2 + n]],
+                   np.float32)
+    log = [ln for ln in lines[:k] if ln.startswith("[stressRefine]")]
+    return ms, order, out, log

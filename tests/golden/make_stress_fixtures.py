@@ -17,13 +17,13 @@ exist:
 """
 import json
 import os
-import subprocess
+import sys
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
-BIN = os.path.join(ROOT, "oracle", "_ref", "levels_ref")
+sys.path.insert(0, ROOT)
 
 PARAMS = dict(count_a=0.5, count_scale=20.0, count_shift=1.0, count_base=0.01,
               heat_scale=100.0, heat_power=-0.333, heat_scale_inter=100.0,
@@ -33,9 +33,6 @@ PARAMS_POWER = dict(PARAMS, count_a=-0.333, count_scale=2.5, count_shift=-2.0, c
                     glen_base=0.1, glen_scale=0.02, glen_power=0.6)
 # pair targets below 1e-6: no pairs at all (only springs and bending)
 PARAMS_NOPAIRS = dict(PARAMS, count_base=0.0, count_scale=1e-9)
-PKEYS = ["count_a", "count_scale", "count_shift", "count_base", "heat_scale", "heat_power",
-         "heat_scale_inter", "heat_power_inter", "stretching", "glen_base", "glen_scale",
-         "glen_power"]
 
 # (seed, anchors, iterations, lr, k_stretch, k_squeeze, k_angular, long arcs, params)
 CASES = [
@@ -93,26 +90,8 @@ def make_case(seed, n, n_long):
 
 
 def run(case, pos, params, ks, kq, ka, iters, lr):
-    c = case
-    text = "stress " + " ".join(repr(f32(params[k])) for k in PKEYS) + "\n"
-    text += f"1\nchr1 {len(c['anchors'])} {len(c['arcs'])}\n"
-    text += "".join(f"{s} {e} N\n" for s, e in c["anchors"])
-    text += "".join(f"{a} {b} {sc}\n" for a, b, sc in c["arcs"])
-    text += f"{len(c['splits'])}\n" + "".join(f"{n} {s} {e}\n" for n, s, e in c["splits"])
-    text += f"chr1\n{f32(ks)!r} {f32(kq)!r} {f32(ka)!r}\n{iters} {f32(lr)!r}\n{len(c['long_arcs'])}\n"
-    text += "".join(f"{s} {e} {sc}\n" for s, e, sc in c["long_arcs"])
-    text += " ".join(float(v).hex() for v in pos.reshape(-1)) + "\n"
-    r = subprocess.run([BIN], input=text, capture_output=True, text=True, timeout=600)
-    if r.returncode != 0:
-        raise RuntimeError(r.stderr[-2000:])
-    lines = r.stdout.splitlines()
-    k = next(i for i, ln in enumerate(lines) if ln.startswith("@@stress"))
-    n = len(c["anchors"])
-    order = [int(x) for x in lines[k + 1].split()]
-    out = np.array([[float.fromhex(v) for v in ln.split()] for ln in lines[k + 2:k + 2 + n]],
-                   np.float32)
-    # the reference's progress lines: iterations run and how it stopped
-    log = [ln for ln in lines[:k] if ln.startswith("[stressRefine]")]
+    from oracle import refdriver
+    _, order, out, log = refdriver.stress(case, params, ks, kq, ka, iters, lr, pos)
     return order, out, log
 
 
