@@ -21,7 +21,10 @@
 // launch runs as many blocks of moves as fit a target of kTargetMs measured
 // on the device (HIP events around the group's launch, the budget rescaled
 // after every step), so small jobs finish in a few steps while large ones
-// advance one block at a time.
+// advance one block at a time.  The groups' launches of a step are spread
+// over kStreams streams (the library's and kStreams - 1 of the server's own,
+// all at the default priority): groups touch disjoint jobs, and a step's
+// small launches would otherwise run one after another on an idle device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,6 +42,7 @@
 namespace pmmc {
 
 constexpr float kTargetMs = 4.0f;  // device time per group launch when pacing adaptively
+constexpr int kStreams = 4;         // streams a step's group launches are spread over
 constexpr int kMaxBlocks = 1 << 16;
 constexpr size_t kGranule = 256;
 
@@ -144,6 +148,12 @@ struct pmmc_server {
   std::vector<pmmc::EnsRef> h_ensrefs;
   std::map<pmmc::SrvKey, pmmc::Pacing> pacing;
   int n_ens = 0;
+  // the server's own streams (besides ctx->stream) and the events that order
+  // them with it: `ready` after the step's uploads, joined[k] after stream
+  // k's launches
+  std::vector<hipStream_t> side;
+  hipEvent_t ready = nullptr;
+  std::vector<hipEvent_t> joined;
 };
 
 using namespace pmmc;
@@ -348,6 +358,20 @@ extern "C" pmmc_status pmmc_server_create(pmmc_ctx *ctx, const pmmc_server_opts 
     return fail(ctx, PMMC_ERR_NOMEM, "server: the HBM budget could not be allocated");
   }
   s->free_list.reset(s->opts.hbm_budget);
+  bool ok = hipEventCreateWithFlags(&s->ready, hipEventDisableTiming) == hipSuccess;
+  for (int k = 1; ok && k < kStreams; ++k) {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+    if (ok) s->side.push_back(st);
+    ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+    if (ok) s->joined.push_back(ev);
+  }
+  if (!ok) {
+    (void)hipGetLastError();
+    pmmc_server_destroy(s);
+    return fail(ctx, PMMC_ERR_HIP, "server: stream or event creation failed");
+  }
   *out = s;
   return PMMC_OK;
 }
@@ -368,6 +392,12 @@ extern "C" void pmmc_server_destroy(pmmc_server *s) {
   if (s->d_jobrefs) (void)hipFree(s->d_jobrefs);
   if (s->d_ensrefs) (void)hipFree(s->d_ensrefs);
   if (s->h_stage) (void)hipHostFree(s->h_stage);
+  for (hipStream_t st : s->side) {
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+  }
+  for (hipEvent_t ev : s->joined) (void)hipEventDestroy(ev);
+  if (s->ready) (void)hipEventDestroy(s->ready);
   delete s;
 }
 
@@ -416,12 +446,23 @@ static pmmc_status init_jobs(pmmc_server *s, const std::vector<SrvJob *> &fresh)
   return PMMC_OK;
 }
 
-// One launch per group; with adaptive pacing each launch is bracketed by the
-// group's events.
+// One launch per group, round-robin over the streams (each side stream first
+// waits for the step's uploads on ctx->stream; ctx->stream then waits for
+// every side stream); with adaptive pacing each launch is bracketed by the
+// group's events on its stream.
 static pmmc_status launch_step(pmmc_server *s) {
   pmmc_ctx *ctx = s->ctx;
   const bool adaptive = s->opts.blocks_per_step == 0;
-  for (const SrvGroup &sg : s->groups) {
+  const int n_side = std::min<int>(static_cast<int>(s->side.size()),
+                                   std::max<int>(0, static_cast<int>(s->groups.size()) - 1));
+  if (n_side > 0) {
+    PMMC_HIP_TRY(ctx, hipEventRecord(s->ready, ctx->stream));
+    for (int k = 0; k < n_side; ++k) PMMC_HIP_TRY(ctx, hipStreamWaitEvent(s->side[k], s->ready, 0));
+  }
+  for (size_t i = 0; i < s->groups.size(); ++i) {
+    const SrvGroup &sg = s->groups[i];
+    const int slot = n_side > 0 ? static_cast<int>(i % static_cast<size_t>(n_side + 1)) : 0;
+    hipStream_t st = slot == 0 ? ctx->stream : s->side[slot - 1];
     int blocks = s->opts.blocks_per_step;
     Pacing *pc = nullptr;
     if (adaptive) {
@@ -431,11 +472,15 @@ static pmmc_status launch_step(pmmc_server *s) {
         PMMC_HIP_TRY(ctx, hipEventCreate(&pc->e1));
       }
       blocks = pc->blocks;
-      PMMC_HIP_TRY(ctx, hipEventRecord(pc->e0, ctx->stream));
+      PMMC_HIP_TRY(ctx, hipEventRecord(pc->e0, st));
     }
     std::vector<LaunchGroup> one(1, sg.g);
-    PMMC_HIP_TRY(ctx, launch_groups(ctx->stream, s->d_probs, s->d_ens, one, blocks));
-    if (pc) PMMC_HIP_TRY(ctx, hipEventRecord(pc->e1, ctx->stream));
+    PMMC_HIP_TRY(ctx, launch_groups(st, s->d_probs, s->d_ens, one, blocks));
+    if (pc) PMMC_HIP_TRY(ctx, hipEventRecord(pc->e1, st));
+  }
+  for (int k = 0; k < n_side; ++k) {
+    PMMC_HIP_TRY(ctx, hipEventRecord(s->joined[k], s->side[k]));
+    PMMC_HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, s->joined[k], 0));
   }
   return PMMC_OK;
 }
