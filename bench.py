@@ -417,6 +417,10 @@ def batched(ctx, chains_per_problem=256, plain_subset=16):
     sched = schedule_preset(2, 1, s, 0.03)
     n = len(probs)
     seeds = [1 + i for i in range(n)]
+    # one untimed call first (other seeds): it grows the library's device
+    # arena to the pool's size, which a caller pays once per context
+    ctx.anneal_batch(probs, sched, seeds=[100_000 + s for s in seeds], n_chains=chains_per_problem,
+                     use_graph=True)
     # batched: launch shape chosen by the library's tuner (chains per
     # wavefront, blocks per step, steps per graph), timed with its probes
     shape = {}

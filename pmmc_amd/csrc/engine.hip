@@ -286,19 +286,21 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
       // pick stream, independent of the chains' state, so its CSR row and
       // the positions it reads are fetched before move m is evaluated (their
       // HBM latency overlaps move m's arithmetic); after move m's decision
-      // the fetched copies of the bead move m changed are patched.  Same
-      // draws, values and arithmetic as the plain loop below.
-      U4 rbn = draw(seed, m0 >> 2, group, segment, LEVEL, kSubBead);
+      // the fetched copies of the bead move m changed are patched.  The
+      // picks and row indices come from per-64-move windows (ArcsWindow).
+      // Same draws, values and arithmetic as the plain loop below.
+      ArcsWindow w;
+      arcs_window(P, seed, group, segment, m0, w);
       ArcsFetch f;
-      arcs_fetch(P, X, cst(P.free_list)[pick(pick_word(rbn, m0), static_cast<uint32_t>(nfree))], f);
+      arcs_fetch_win(X, w, 0, f);
       for (int k = 0; k < S.block; ++k) {
         const uint32_t m = m0 + static_cast<uint32_t>(k);
         const ArcsFetch c = f;
         const bool more = k + 1 < S.block;
         if (more) {
-          const uint32_t mn = m + 1;
-          if ((mn & 3u) == 0) rbn = draw(seed, mn >> 2, group, segment, LEVEL, kSubBead);
-          arcs_fetch(P, X, cst(P.free_list)[pick(pick_word(rbn, mn), static_cast<uint32_t>(nfree))], f);
+          const int kn = k + 1;
+          if ((kn & (kLanes - 1)) == 0) arcs_window(P, seed, group, segment, m + 1, w);
+          arcs_fetch_win(X, w, kn & (kLanes - 1), f);
         }
         const U4 rc = draw(seed, m, chain, segment, LEVEL, kSubChain);
         const float dx = disp(rc.x, step), dy = disp(rc.y, step);

@@ -341,6 +341,52 @@ __device__ __forceinline__ void arcs_fetch(const DevProblem &P, const View &X, i
   }
 }
 
+// The bead picks of 64 consecutive moves mw .. mw + 63 and the first chunk of
+// their CSR rows, gathered one move per lane (lane k holds move mw + k) with
+// vector loads.  A move's fetch then reads them back with readlane, so the
+// per-move chain of dependent scalar loads (free-list entry, row offsets,
+// partner indices) becomes one gather per 64 moves.  Lanes past the block's
+// last move gather picks nobody reads; every index stays in its table (the
+// pick is below nfree, p below n, and the entry index is clamped to the row,
+// or to entry 0 of a table holding at least one entry when the row is empty).
+struct ArcsWindow {
+  int p, e0, e1;
+  int j[kArcsChunk];
+};
+
+__device__ __forceinline__ void arcs_window(const DevProblem &P, uint64_t seed, uint32_t group,
+                                            uint32_t segment, uint32_t mw, ArcsWindow &w) {
+  const uint32_t m = mw + static_cast<uint32_t>(threadIdx.x & (kLanes - 1));
+  const U4 rb = draw(seed, m >> 2, group, segment, PMMC_LEVEL_ARCS, kSubBead);
+  w.p = P.free_list[pick(pick_word(rb, m), static_cast<uint32_t>(P.nfree))];
+  w.e0 = P.off[w.p];
+  w.e1 = P.off[w.p + 1];
+#pragma unroll
+  for (int u = 0; u < kArcsChunk; ++u) w.j[u] = P.ej[max(min(w.e0 + u, w.e1 - 1), 0)] & 0x7fffffff;
+}
+
+// arcs_fetch of move mw + k from its window
+template <class View>
+__device__ __forceinline__ void arcs_fetch_win(const View &X, const ArcsWindow &w, int k,
+                                               ArcsFetch &f) {
+  f.p = __builtin_amdgcn_readlane(w.p, k);
+  f.e0 = __builtin_amdgcn_readlane(w.e0, k);
+  f.e1 = __builtin_amdgcn_readlane(w.e1, k);
+  f.px = X.get(f.p, 0);
+  f.py = X.get(f.p, 1);
+  f.pz = X.get(f.p, 2);
+  if (f.e1 > f.e0) {
+#pragma unroll
+    for (int u = 0; u < kArcsChunk; ++u) f.j[u] = __builtin_amdgcn_readlane(w.j[u], k);
+#pragma unroll
+    for (int u = 0; u < kArcsChunk; ++u) {
+      f.xj[u] = X.get(f.j[u], 0);
+      f.yj[u] = X.get(f.j[u], 1);
+      f.zj[u] = X.get(f.j[u], 2);
+    }
+  }
+}
+
 // The prefetched move reads positions as they were before the current move;
 // when the current move changed bead q (accepted per lane: ok), the copies of
 // q among them take its new position.  Selects, no branches: a branch per
