@@ -30,17 +30,23 @@ struct ArcsFetch2 {
   float xj[kPairChunk], yj[kPairChunk], zj[kPairChunk];
 };
 
-__device__ __forceinline__ void arcs_fetch2(const DevProblem &P, const PosView &X, int p, int half,
+// Move mw + k of its window (kernels.hpp, ArcsWindow: lane k gathered the
+// move's pick and row indices); this lane takes entries half, half + 2, ...
+__device__ __forceinline__ void arcs_fetch2(const PosView &X, const ArcsWindow &w, int k, int half,
                                             ArcsFetch2 &f) {
-  f.p = p;
-  f.e0 = cst(P.off)[p];
-  f.e1 = cst(P.off)[p + 1];
-  f.px = X.get(p, 0);
-  f.py = X.get(p, 1);
-  f.pz = X.get(p, 2);
+  f.p = __builtin_amdgcn_readlane(w.p, k);
+  f.e0 = __builtin_amdgcn_readlane(w.e0, k);
+  f.e1 = __builtin_amdgcn_readlane(w.e1, k);
+  f.px = X.get(f.p, 0);
+  f.py = X.get(f.p, 1);
+  f.pz = X.get(f.p, 2);
   if (f.e1 > f.e0) {
 #pragma unroll
-    for (int u = 0; u < kPairChunk; ++u) f.j[u] = P.ej[min(f.e0 + half + 2 * u, f.e1 - 1)] & 0x7fffffff;
+    for (int u = 0; u < kPairChunk; ++u) {
+      const int even = __builtin_amdgcn_readlane(w.j[2 * u], k);
+      const int odd = __builtin_amdgcn_readlane(w.j[2 * u + 1], k);
+      f.j[u] = half ? odd : even;
+    }
 #pragma unroll
     for (int u = 0; u < kPairChunk; ++u) {
       f.xj[u] = X.get(f.j[u], 0);
@@ -106,7 +112,6 @@ __global__ __launch_bounds__(128) void k_anneal_arcs2(const DevProblem *__restri
   const pmmc_schedule S = P.sched;
   const uint64_t seed = P.seed;
   const uint32_t segment = P.segment;
-  const int nfree = P.nfree;
   const float r = S.sphere_radius;
   const double offset = *P.offset;
 
@@ -128,17 +133,18 @@ __global__ __launch_bounds__(128) void k_anneal_arcs2(const DevProblem *__restri
     const uint32_t m0 = readlane_u32(iters, __builtin_ctzll(am));
     // software-pipelined as the one-lane path: move m+1's row and positions
     // are fetched before move m is evaluated, then patched
-    U4 rbn = draw(seed, m0 >> 2, group, segment, PMMC_LEVEL_ARCS, kSubBead);
+    ArcsWindow w;
+    arcs_window(P, seed, group, segment, m0, w);
     ArcsFetch2 f;
-    arcs_fetch2(P, X, cst(P.free_list)[pick(pick_word(rbn, m0), static_cast<uint32_t>(nfree))], half, f);
+    arcs_fetch2(X, w, 0, half, f);
     for (int k = 0; k < S.block; ++k) {
       const uint32_t m = m0 + static_cast<uint32_t>(k);
       const ArcsFetch2 c = f;
       const bool more = k + 1 < S.block;
       if (more) {
-        const uint32_t mn = m + 1;
-        if ((mn & 3u) == 0) rbn = draw(seed, mn >> 2, group, segment, PMMC_LEVEL_ARCS, kSubBead);
-        arcs_fetch2(P, X, cst(P.free_list)[pick(pick_word(rbn, mn), static_cast<uint32_t>(nfree))], half, f);
+        const int kn = k + 1;
+        if ((kn & (kLanes - 1)) == 0) arcs_window(P, seed, group, segment, m + 1, w);
+        arcs_fetch2(X, w, kn & (kLanes - 1), half, f);
       }
       const U4 rc = draw(seed, m, chain, segment, PMMC_LEVEL_ARCS, kSubChain);
       const float dx = disp(rc.x, step), dy = disp(rc.y, step);

@@ -75,7 +75,10 @@ __device__ __forceinline__ float move_coord(float x, float d) {
   return x + d;
 }
 
+// Rounded as written (no contraction): the arc kernels inline it in
+// different surroundings and must give the same bits.
 __device__ __forceinline__ float sphere_penalty(float x, float y, float z, float r) {
+#pragma clang fp contract(off)
   if (r <= 0.0f) return 0.0f;
   const float d = sqrtf(x * x + y * y + z * z);
   if (d <= r) return 0.0f;

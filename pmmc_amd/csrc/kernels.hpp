@@ -250,15 +250,20 @@ __device__ __forceinline__ float heatmap_delta(const DevProblem &P, const View &
 // kernel that splits a chain's partners over two lanes by entry parity
 // forms the same two sums, so every arc kernel gives the same bits.
 // Term of one entry: (d1 - c)^2 k1 - (d0 - c)^2 k0, with d0 / d1 the
-// distances from the old / new position of p to the partner.
+// distances from the old / new position of p to the partner, rounded as
+// fma((d1 - c)^2, k1, -((d0 - c)^2 k0)) with no further contraction: the
+// rounding is fixed in the source, not left to where the compiler fuses a
+// product into a sum, which can differ between the kernels that inline it.
 __device__ __forceinline__ float arcs_term(float c, float ka, float kb, float px, float py,
                                            float pz, float nx, float ny, float nz, float xj,
                                            float yj, float zj) {
+#pragma clang fp contract(off)
   const float t0 = dist3(px, py, pz, xj, yj, zj) - c;
   const float t1 = dist3(nx, ny, nz, xj, yj, zj) - c;
   const float k0 = t0 >= 0.f ? ka : kb;
   const float k1 = t1 >= 0.f ? ka : kb;
-  return t1 * t1 * k1 - t0 * t0 * k0;
+  const float s0 = t0 * t0, s1 = t1 * t1;
+  return fmaf(s1, k1, -(s0 * k0));
 }
 
 // One chunk (kArcsChunk entries, an even offset into the row) of bead p's
@@ -320,26 +325,6 @@ struct ArcsFetch {
   float px, py, pz;
   float xj[kArcsChunk], yj[kArcsChunk], zj[kArcsChunk];
 };
-
-template <class View>
-__device__ __forceinline__ void arcs_fetch(const DevProblem &P, const View &X, int p, ArcsFetch &f) {
-  f.p = p;
-  f.e0 = cst(P.off)[p];
-  f.e1 = cst(P.off)[p + 1];
-  f.px = X.get(p, 0);
-  f.py = X.get(p, 1);
-  f.pz = X.get(p, 2);
-  if (f.e1 > f.e0) {
-#pragma unroll
-    for (int u = 0; u < kArcsChunk; ++u) f.j[u] = cst(P.ej)[min(f.e0 + u, f.e1 - 1)] & 0x7fffffff;
-#pragma unroll
-    for (int u = 0; u < kArcsChunk; ++u) {
-      f.xj[u] = X.get(f.j[u], 0);
-      f.yj[u] = X.get(f.j[u], 1);
-      f.zj[u] = X.get(f.j[u], 2);
-    }
-  }
-}
 
 // The bead picks of 64 consecutive moves mw .. mw + 63 and the first chunk of
 // their CSR rows, gathered one move per lane (lane k holds move mw + k) with
@@ -408,7 +393,8 @@ __device__ __forceinline__ void arcs_patch(ArcsFetch &f, int q, bool ok, float q
 }
 
 // ΔE of a fetched move: the first chunk from the fetched positions (its
-// coefficients read now), any further chunks (beads with more than
+// coefficients read now: fetching them ahead as well costs 48 VGPRs, a wave
+// per SIMD, and measured slower), any further chunks (beads with more than
 // kArcsChunk entries) read now; same terms and order as arcs_delta.
 template <class View>
 __device__ __forceinline__ float arcs_delta_fetched(const DevProblem &P, const View &X,
