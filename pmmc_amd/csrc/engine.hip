@@ -327,17 +327,21 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
       // kSmoothDepth moves deep: the windows of moves m+1..m+kSmoothDepth
       // are in flight while move m is evaluated; after move m's decision
       // every fetched window is patched.
+      // The picks and rest lengths come from per-64-move windows
+      // (SmoothWindow); move m0 + i is lane i & 63 of window i >> 6.
       constexpr int D = kSmoothDepth;
       const int blk = S.block;
       SmoothFetch f[D];
-      uint32_t mf = m0;  // the next move to fetch
-      U4 rbn = draw(seed, mf >> 2, group, segment, LEVEL, kSubBead);
+      int kf = 0;  // the next move to fetch, relative to m0
+      SmoothWindow w;
+      smooth_window(P, seed, group, segment, m0, w);
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         if (d < blk) {
-          if (d > 0 && (mf & 3u) == 0) rbn = draw(seed, mf >> 2, group, segment, LEVEL, kSubBead);
-          smooth_fetch(P, X, cst(P.free_list)[pick(pick_word(rbn, mf), static_cast<uint32_t>(nfree))], f[d]);
-          ++mf;
+          if (kf > 0 && (kf & (kLanes - 1)) == 0)
+            smooth_window(P, seed, group, segment, m0 + static_cast<uint32_t>(kf), w);
+          smooth_fetch_win(P, X, w, kf & (kLanes - 1), f[d]);
+          ++kf;
         }
       }
       for (int k = 0; k < blk; ++k) {
@@ -346,9 +350,10 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
 #pragma unroll
         for (int d = 0; d + 1 < D; ++d) f[d] = f[d + 1];
         if (k + D < blk) {
-          if ((mf & 3u) == 0) rbn = draw(seed, mf >> 2, group, segment, LEVEL, kSubBead);
-          smooth_fetch(P, X, cst(P.free_list)[pick(pick_word(rbn, mf), static_cast<uint32_t>(nfree))], f[D - 1]);
-          ++mf;
+          if ((kf & (kLanes - 1)) == 0)
+            smooth_window(P, seed, group, segment, m0 + static_cast<uint32_t>(kf), w);
+          smooth_fetch_win(P, X, w, kf & (kLanes - 1), f[D - 1]);
+          ++kf;
         }
         const float px = c.qx[2], py = c.qy[2], pz = c.qz[2];
         const U4 rc = draw(seed, m, chain, segment, LEVEL, kSubChain);

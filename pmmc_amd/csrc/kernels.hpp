@@ -530,8 +530,10 @@ struct SmoothFetch {
   float d0m, d0p;
 };
 
+// the 5-bead window's positions (outside the chain: 0)
 template <class View>
-__device__ __forceinline__ void smooth_fetch(const DevProblem &P, const View &X, int p, SmoothFetch &f) {
+__device__ __forceinline__ void smooth_fetch_pos(const DevProblem &P, const View &X, int p,
+                                                 SmoothFetch &f) {
   const int n = P.n;
   f.p = p;
 #pragma unroll
@@ -542,8 +544,36 @@ __device__ __forceinline__ void smooth_fetch(const DevProblem &P, const View &X,
     f.qy[o] = in ? X.get(idx, 1) : 0.f;
     f.qz[o] = in ? X.get(idx, 2) : 0.f;
   }
-  f.d0m = p > 0 ? fmaxf(cst(P.dtn)[p - 1], 1e-6f) : 1.f;
-  f.d0p = p < n - 1 ? fmaxf(cst(P.dtn)[p], 1e-6f) : 1.f;
+}
+
+// The bead picks of 64 consecutive smooth moves mw .. mw + 63 and their rest
+// lengths, gathered one move per lane as ArcsWindow does for arc moves (the
+// free-list entry and the two rest lengths were dependent scalar loads in
+// every move's fetch).  dist_to_next holds n entries, so both clamped
+// indices stay in it.
+struct SmoothWindow {
+  int p;
+  float d0m, d0p;
+};
+
+__device__ __forceinline__ void smooth_window(const DevProblem &P, uint64_t seed, uint32_t group,
+                                              uint32_t segment, uint32_t mw, SmoothWindow &w) {
+  const int n = P.n;
+  const uint32_t m = mw + static_cast<uint32_t>(threadIdx.x & (kLanes - 1));
+  const U4 rb = draw(seed, m >> 2, group, segment, PMMC_LEVEL_SMOOTH, kSubBead);
+  const int p = P.free_list[pick(pick_word(rb, m), static_cast<uint32_t>(P.nfree))];
+  w.p = p;
+  w.d0m = p > 0 ? fmaxf(P.dtn[max(p - 1, 0)], 1e-6f) : 1.f;
+  w.d0p = p < n - 1 ? fmaxf(P.dtn[p], 1e-6f) : 1.f;
+}
+
+// smooth_fetch of move mw + k from its window
+template <class View>
+__device__ __forceinline__ void smooth_fetch_win(const DevProblem &P, const View &X,
+                                                 const SmoothWindow &w, int k, SmoothFetch &f) {
+  smooth_fetch_pos(P, X, __builtin_amdgcn_readlane(w.p, k), f);
+  f.d0m = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w.d0m), k));
+  f.d0p = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w.d0p), k));
 }
 
 // The fetched window holds positions from before the current move; when the
