@@ -78,12 +78,23 @@ __device__ __forceinline__ float arcs_delta2(const DevProblem &P, const PosView 
                                              const ArcsFetch2 &f, int half, float nx, float ny,
                                              float nz) {
   float acc = 0.f;
+  if (f.e1 > f.e0) {
+    // the lane's entries of the first chunk, loaded before any term (the
+    // tables carry kArcsChunk entries of padding past the last row)
+    float c[kPairChunk], ka[kPairChunk], kb[kPairChunk];
 #pragma unroll
-  for (int u = 0; u < kPairChunk; ++u) {
-    const int e = f.e0 + half + 2 * u;
-    if (e < f.e1)
-      acc += arcs_term(P.ec[e], P.eka[e], P.ekb[e], f.px, f.py, f.pz, nx, ny, nz, f.xj[u],
-                       f.yj[u], f.zj[u]);
+    for (int u = 0; u < kPairChunk; ++u) {
+      const int e = f.e0 + half + 2 * u;
+      c[u] = P.ec[e];
+      ka[u] = P.eka[e];
+      kb[u] = P.ekb[e];
+    }
+#pragma unroll
+    for (int u = 0; u < kPairChunk; ++u) {
+      if (f.e0 + half + 2 * u < f.e1)
+        acc += arcs_term(c[u], ka[u], kb[u], f.px, f.py, f.pz, nx, ny, nz, f.xj[u], f.yj[u],
+                         f.zj[u]);
+    }
   }
   // entries past the first chunk (beads of more than kArcsChunk entries)
   for (int e = f.e0 + kArcsChunk + half; e < f.e1; e += 2) {

@@ -289,18 +289,31 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
       // the fetched copies of the bead move m changed are patched.  The
       // picks and row indices come from per-64-move windows (ArcsWindow).
       // Same draws, values and arithmetic as the plain loop below.
+      constexpr int D = kArcsDepth;
+      const int blk = S.block;
+      ArcsFetch f[D];
+      int kf = 0;  // the next move to fetch, relative to m0
       ArcsWindow w;
       arcs_window(P, seed, group, segment, m0, w);
-      ArcsFetch f;
-      arcs_fetch_win(X, w, 0, f);
-      for (int k = 0; k < S.block; ++k) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (d < blk) {
+          if (kf > 0 && (kf & (kLanes - 1)) == 0)
+            arcs_window(P, seed, group, segment, m0 + static_cast<uint32_t>(kf), w);
+          arcs_fetch_win(X, w, kf & (kLanes - 1), f[d]);
+          ++kf;
+        }
+      }
+      for (int k = 0; k < blk; ++k) {
         const uint32_t m = m0 + static_cast<uint32_t>(k);
-        const ArcsFetch c = f;
-        const bool more = k + 1 < S.block;
-        if (more) {
-          const int kn = k + 1;
-          if ((kn & (kLanes - 1)) == 0) arcs_window(P, seed, group, segment, m + 1, w);
-          arcs_fetch_win(X, w, kn & (kLanes - 1), f);
+        const ArcsFetch c = f[0];
+#pragma unroll
+        for (int d = 0; d + 1 < D; ++d) f[d] = f[d + 1];
+        if (k + D < blk) {
+          if ((kf & (kLanes - 1)) == 0)
+            arcs_window(P, seed, group, segment, m0 + static_cast<uint32_t>(kf), w);
+          arcs_fetch_win(X, w, kf & (kLanes - 1), f[D - 1]);
+          ++kf;
         }
         const U4 rc = draw(seed, m, chain, segment, LEVEL, kSubChain);
         const float dx = disp(rc.x, step), dy = disp(rc.y, step);
@@ -319,7 +332,9 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
           ++acc;
           best = E < best ? E : best;
         }
-        if (more) arcs_patch(f, c.p, ok, nx, ny, nz);
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+          if (k + 1 + d < blk) arcs_patch(f[d], c.p, ok, nx, ny, nz);
         if (active && !S.cool_per_block) T *= S.dt;
       }
     } else if constexpr (LEVEL == PMMC_LEVEL_SMOOTH && !STAGE) {
@@ -782,9 +797,11 @@ static void size_static(Sizer &z, const HostProblem &h) {
   z.add<int32_t>(h.off.size());
   z.add<ArcEntry>(std::max<size_t>(1, h.ent.size()));
   z.add<int32_t>(std::max<size_t>(1, h.ej.size()));
-  z.add<float>(std::max<size_t>(1, h.ec.size()));
-  z.add<float>(std::max<size_t>(1, h.eka.size()));
-  z.add<float>(std::max<size_t>(1, h.ekb.size()));
+  // coefficient tables padded by one chunk: a move loads its first chunk's
+  // coefficients unconditionally (kernels.hpp, arcs_delta_fetched)
+  z.add<float>(h.ec.size() + kArcsChunk);
+  z.add<float>(h.eka.size() + kArcsChunk);
+  z.add<float>(h.ekb.size() + kArcsChunk);
   z.add<PairEntry>(std::max<size_t>(1, h.pairs.size()));
   z.add<int32_t>(std::max<size_t>(1, h.zpairs.size()));
   z.add<float>(h.dtn.size());
@@ -835,9 +852,9 @@ static void fill_static(StageCarver &sc, const HostProblem &h, DevProblem &P) {
   P.off = sc.put(h.off);
   P.ent = sc.put(h.ent, 1);
   P.ej = sc.put(h.ej, 1);
-  P.ec = sc.put(h.ec, 1);
-  P.eka = sc.put(h.eka, 1);
-  P.ekb = sc.put(h.ekb, 1);
+  P.ec = sc.put(h.ec, h.ec.size() + kArcsChunk);
+  P.eka = sc.put(h.eka, h.eka.size() + kArcsChunk);
+  P.ekb = sc.put(h.ekb, h.ekb.size() + kArcsChunk);
   P.pairs = sc.put(h.pairs, 1);
   P.zpairs = sc.put(h.zpairs, 1);
   P.dtn = sc.put(h.dtn);
@@ -984,11 +1001,19 @@ pmmc_status batch_setup(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &b
   bd.hprobs.resize(hp.size());
   bd.hens.clear();
   bd.groups.clear();
-  // order ensembles by launch group so each group is contiguous
+  // order ensembles by launch group so each group is contiguous, and within
+  // a group the largest problems first: the dispatcher hands the first
+  // workgroups of a launch to the CUs one per SIMD before it doubles up, so
+  // the longest-running ensembles get a SIMD each and share it only with
+  // small problems' ensembles, which stop early.  Results do not depend on
+  // where an ensemble sits in the grid.
   std::vector<int> order(hp.size());
   for (size_t i = 0; i < hp.size(); ++i) order[i] = static_cast<int>(i);
   auto key = [&](int i) { return group_key(group_of(hp[i])); };
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const auto ka = key(a), kb = key(b);
+    return ka < kb || (ka == kb && hp[a].n > hp[b].n);
+  });
   for (int oi : order) {
     HostProblem &h = hp[oi];
     const LaunchGroup g = group_of(h);

@@ -21,6 +21,16 @@ namespace pmmc {
 // arc CSR entries evaluated per round trip of the move's partner loop
 constexpr int kArcsChunk = PMMC_ARCS_CHUNK;
 
+#ifndef PMMC_ARCS_DEPTH
+#define PMMC_ARCS_DEPTH 1
+#endif
+// arc-level moves whose rows and positions are in flight ahead of the one
+// evaluated.  Depth 2 (140 VGPRs, 3 waves per SIMD) measured slower on every
+// form: 400-anchor solve 5.91 vs 5.98e10 moves/s, 512-problem pool 2,639 vs
+// 2,840 solves/s, lone plain solves 12.9 vs 14.5 per second.
+constexpr int kArcsDepth = PMMC_ARCS_DEPTH;
+static_assert(kArcsDepth >= 1, "at least the next move is fetched ahead");
+
 #ifndef PMMC_SMOOTH_DEPTH
 #define PMMC_SMOOTH_DEPTH 1
 #endif
@@ -392,22 +402,33 @@ __device__ __forceinline__ void arcs_patch(ArcsFetch &f, int q, bool ok, float q
   }
 }
 
-// ΔE of a fetched move: the first chunk from the fetched positions (its
-// coefficients read now: fetching them ahead as well costs 48 VGPRs, a wave
-// per SIMD, and measured slower), any further chunks (beads with more than
-// kArcsChunk entries) read now; same terms and order as arcs_delta.
+// ΔE of a fetched move: the first chunk from the fetched positions, any
+// further chunks (beads with more than kArcsChunk entries) read now; same
+// terms and order as arcs_delta.  The first chunk's coefficients are read
+// with scalar loads issued together before any term, entries past the row
+// included (the tables carry kArcsChunk entries of padding): a load inside
+// each entry's `u < cnt` branch made every entry wait for its own load.
+// (Fetching them a move ahead instead costs 48 VGPRs, a wave per SIMD, and
+// measured slower.)
 template <class View>
 __device__ __forceinline__ float arcs_delta_fetched(const DevProblem &P, const View &X,
                                                     const ArcsFetch &f, float nx, float ny,
                                                     float nz) {
   float acc[2] = {0.f, 0.f};
   const int cnt = min(kArcsChunk, f.e1 - f.e0);
+  if (cnt > 0) {
+    float c[kArcsChunk], ka[kArcsChunk], kb[kArcsChunk];
 #pragma unroll
-  for (int u = 0; u < kArcsChunk; ++u) {
-    if (u < cnt) {
-      const int e = f.e0 + u;
-      acc[u & 1] += arcs_term(cst(P.ec)[e], cst(P.eka)[e], cst(P.ekb)[e], f.px, f.py, f.pz, nx, ny,
-                              nz, f.xj[u], f.yj[u], f.zj[u]);
+    for (int u = 0; u < kArcsChunk; ++u) {
+      c[u] = cst(P.ec)[f.e0 + u];
+      ka[u] = cst(P.eka)[f.e0 + u];
+      kb[u] = cst(P.ekb)[f.e0 + u];
+    }
+#pragma unroll
+    for (int u = 0; u < kArcsChunk; ++u) {
+      if (u < cnt)
+        acc[u & 1] += arcs_term(c[u], ka[u], kb[u], f.px, f.py, f.pz, nx, ny, nz, f.xj[u], f.yj[u],
+                                f.zj[u]);
     }
   }
   for (int e = f.e0 + kArcsChunk; e < f.e1; e += kArcsChunk) {
