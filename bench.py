@@ -502,7 +502,8 @@ def _alg_bytes_per_move(level, n, deg, acc_rate):
     return 60.0 + 8.0 / 64.0 + 12.0 * acc_rate
 
 
-def served(ctx, seconds=20.0, rate=200.0, seed=0, plain_sample=24, window=5.0):
+def served(ctx, seconds=20.0, rate=200.0, seed=0, plain_sample=24, window=5.0, launch_ms=0.0,
+           streams=0):
     """Config 4: Poisson arrivals of mixed jobs through the server, `seconds`
     of arrivals at `rate` per second; latency percentiles per `window` of
     arrival time show whether the server keeps up."""
@@ -539,7 +540,8 @@ def served(ctx, seconds=20.0, rate=200.0, seed=0, plain_sample=24, window=5.0):
         ens = int(rng.integers(1, 17))
         kind = "heatmap" if u < 0.4 else ("arcs" if u < 0.8 else "smooth")
         reqs.append((float(t), pool[kind][int(rng.integers(0, 48))], ens))
-    srv = ctx.server(hbm_budget=64 << 30, max_jobs=4096, blocks_per_step=0)
+    srv = ctx.server(hbm_budget=64 << 30, max_jobs=4096, blocks_per_step=0, launch_ms=launch_ms,
+                     streams=streams)
     t0 = time.perf_counter()
     jobs = {}
     done_at = {}

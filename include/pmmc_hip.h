@@ -443,7 +443,11 @@ typedef struct {
   int32_t max_jobs;          /* resident job slots                              */
   int32_t blocks_per_step;   /* blocks of moves per chain per step launch; 0 =
                                 paced per launch group (kernel and size class)
-                                to about 4 ms of device time per launch        */
+                                to about launch_ms of device time per launch   */
+  float launch_ms;           /* pacing target per group launch; 0 = 12 ms      */
+  int32_t streams;           /* streams a step's group launches are spread
+                                over (1-16, all at the default priority);
+                                0 = 4                                          */
 } pmmc_server_opts;
 
 pmmc_status pmmc_server_create(pmmc_ctx *ctx, const pmmc_server_opts *opts, pmmc_server **out);

@@ -141,7 +141,7 @@ class Arc(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_jobs", C.c_int32),
-                ("blocks_per_step", C.c_int32)]
+                ("blocks_per_step", C.c_int32), ("launch_ms", C.c_float), ("streams", C.c_int32)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/pmmc_hip.h

@@ -382,8 +382,9 @@ class Context:
         return res, nonm[:len(chromosomes)].copy()
 
     # ---- served ----
-    def server(self, hbm_budget=8 << 30, max_jobs=256, blocks_per_step=4):
-        return Server(self, hbm_budget, max_jobs, blocks_per_step)
+    def server(self, hbm_budget=8 << 30, max_jobs=256, blocks_per_step=4, launch_ms=0.0,
+               streams=0):
+        return Server(self, hbm_budget, max_jobs, blocks_per_step, launch_ms, streams)
 
     # ---- batched ----
     def anneal_batch(self, problems, schedules, seeds, segments=None, n_chains=0,
@@ -421,11 +422,12 @@ class Context:
 class Server:
     """Continuous-batching server (pmmc_server_*): submit jobs, step, collect."""
 
-    def __init__(self, ctx: Context, hbm_budget: int, max_jobs: int, blocks_per_step: int):
+    def __init__(self, ctx: Context, hbm_budget: int, max_jobs: int, blocks_per_step: int,
+                 launch_ms: float = 0.0, streams: int = 0):
         self.ctx = ctx
         self.lib = ctx.lib
         h = C.c_void_p()
-        opts = A.ServerOpts(hbm_budget, max_jobs, blocks_per_step)
+        opts = A.ServerOpts(hbm_budget, max_jobs, blocks_per_step, launch_ms, streams)
         ctx._check(self.lib.pmmc_server_create(ctx.h, C.byref(opts), C.byref(h)))
         self.h = h
         self.jobs = {}

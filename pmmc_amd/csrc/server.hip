@@ -18,12 +18,13 @@
 //
 // Pacing: resident jobs are grouped by kernel (level, register/LDS variant)
 // and by size class (floor(log2 n)).  With blocks_per_step = 0 each group's
-// launch runs as many blocks of moves as fit a target of kTargetMs measured
+// launch runs as many blocks of moves as fit a target (launch_ms, default
+// kTargetMs) measured
 // on the device (HIP events around the group's launch, the budget rescaled
 // after every step), so small jobs finish in a few steps while large ones
 // advance one block at a time.  The groups' launches of a step are spread
-// over kStreams streams (the library's and kStreams - 1 of the server's own,
-// all at the default priority): groups touch disjoint jobs, and a step's
+// over `streams` streams (default kStreams: the library's and the rest the
+// server's own, all at the default priority): groups touch disjoint jobs, and a step's
 // small launches would otherwise run one after another on an idle device.
 #include <hip/hip_runtime.h>
 
@@ -41,8 +42,16 @@
 
 namespace pmmc {
 
-constexpr float kTargetMs = 4.0f;  // device time per group launch when pacing adaptively
-constexpr int kStreams = 4;         // streams a step's group launches are spread over
+// Default device time per group launch when pacing adaptively.  On the
+// served mix (Poisson 200 requests/s of heatmap, arc and smooth jobs), 8 s
+// runs: p50 / p99 latency 206 / 515 ms at 4 ms, 177 / 424 at 8, 145 / 397 at
+// 12, 143 / 428 at 16, 160 / 545 at 24, 199 / 378 at 48; 1 and 2 ms: 513 /
+// 1,529 and 272 / 952.  Longer launches mean fewer steps (each pays the
+// host's admission, synchronisation and retirement), until small jobs wait
+// behind large launches.
+constexpr float kTargetMs = 12.0f;
+constexpr int kStreams = 4;         // default number of streams a step's group launches use
+constexpr int kMaxStreams = 16;
 constexpr int kMaxBlocks = 1 << 16;
 constexpr size_t kGranule = 256;
 
@@ -350,16 +359,21 @@ extern "C" pmmc_status pmmc_server_create(pmmc_ctx *ctx, const pmmc_server_opts 
   if (opts->max_jobs <= 0 || opts->hbm_budget == 0 || opts->blocks_per_step < 0)
     return fail(ctx, PMMC_ERR_INVALID,
                 "server: max_jobs and hbm_budget must be > 0, blocks_per_step >= 0");
+  if (!(opts->launch_ms >= 0.f) || opts->launch_ms > 1000.f || opts->streams < 0 ||
+      opts->streams > kMaxStreams)
+    return fail(ctx, PMMC_ERR_INVALID, "server: launch_ms must be in [0, 1000], streams in [0, 16]");
   pmmc_server *s = new pmmc_server();
   s->ctx = ctx;
   s->opts = *opts;
+  if (s->opts.launch_ms == 0.f) s->opts.launch_ms = kTargetMs;
+  if (s->opts.streams == 0) s->opts.streams = kStreams;
   if (hipMalloc(reinterpret_cast<void **>(&s->pool), s->opts.hbm_budget) != hipSuccess) {
     delete s;
     return fail(ctx, PMMC_ERR_NOMEM, "server: the HBM budget could not be allocated");
   }
   s->free_list.reset(s->opts.hbm_budget);
   bool ok = hipEventCreateWithFlags(&s->ready, hipEventDisableTiming) == hipSuccess;
-  for (int k = 1; ok && k < kStreams; ++k) {
+  for (int k = 1; ok && k < s->opts.streams; ++k) {
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;
     ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
@@ -486,14 +500,14 @@ static pmmc_status launch_step(pmmc_server *s) {
 }
 
 // After the step's synchronization: rescale each group's block budget toward
-// kTargetMs of device time.
+// launch_ms of device time.
 static void repace(pmmc_server *s) {
   if (s->opts.blocks_per_step != 0) return;
   for (const SrvGroup &sg : s->groups) {
     Pacing &pc = s->pacing[sg.key];
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, pc.e0, pc.e1) != hipSuccess || !(ms > 0.f)) continue;
-    const double scale = static_cast<double>(kTargetMs) / ms;
+    const double scale = static_cast<double>(s->opts.launch_ms) / ms;
     double nb = pc.blocks * std::min(4.0, std::max(0.25, scale));
     pc.blocks = static_cast<int>(std::min<double>(kMaxBlocks, std::max(1.0, std::floor(nb))));
   }
