@@ -301,7 +301,13 @@ def test_sweep_matches_oracle(ctx, n, seed):
 
 
 # ---------------- served form ----------------
-def test_server_alone_equals_mixed(ctx):
+# (blocks_per_step, launch_ms, streams): fixed blocks per step; adaptive
+# pacing on one stream with short launches; adaptive on 16 streams
+SERVER_SHAPES = [(2, 0.0, 0), (0, 0.5, 1), (0, 30.0, 16)]
+
+
+@pytest.mark.parametrize("bps,launch_ms,streams", SERVER_SHAPES)
+def test_server_alone_equals_mixed(ctx, bps, launch_ms, streams):
     s = settings(max_temp=5.0, dt_temp=0.9999, improvement=0.975, steps=2048, min_successes=100)
     asched = schedule_preset(2, 1, s, 0.3)
     asched.max_moves = 2048
@@ -312,7 +318,8 @@ def test_server_alone_equals_mixed(ctx):
     jobs.append((synth.heatmap_problem(n=30, seed=4)[0], hsched, 64))
     jobs.append((synth.smooth_problem(n_anchors=12, add=9, seed=2), ssched, 64))
     alone = [ctx.anneal(p, sc, seed=5, segment=i, n_chains=nc) for i, (p, sc, nc) in enumerate(jobs)]
-    srv = ctx.server(hbm_budget=1 << 30, max_jobs=3, blocks_per_step=2)
+    srv = ctx.server(hbm_budget=1 << 30, max_jobs=3, blocks_per_step=bps, launch_ms=launch_ms,
+                     streams=streams)
     ids = [srv.submit(p, sc, seed=5, segment=i, n_chains=nc) for i, (p, sc, nc) in enumerate(jobs)]
     max_res, steps = 0, 0
     while not all(srv.done(j) for j in ids):
@@ -342,6 +349,9 @@ def test_server_budget_admission(ctx):
     with pytest.raises(Exception):
         srv2 = ctx.server(hbm_budget=1024, max_jobs=4)
         srv2.submit(probs[0], sched, n_chains=64)   # larger than the budget
+    for bad in (dict(streams=17), dict(streams=-1), dict(launch_ms=-1.0), dict(launch_ms=2000.0)):
+        with pytest.raises(Exception, match="launch_ms"):
+            ctx.server(hbm_budget=1 << 20, max_jobs=1, blocks_per_step=0, **bad)
     srv.close()
 
 
