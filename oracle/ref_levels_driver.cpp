@@ -52,6 +52,12 @@
 //             given path, then Heatmap::fromFile (:702-732) of a given path
 //   stress    tree, then LooperSolver::stressRefineAnchors (:5638-5915) over
 //             one chromosome's anchors with given positions and long arcs
+//   flat      PdbWriter::writeFlat (PdbWriter.hpp:296-335) and
+//             CifWriter::writeFlat (CifWriter.hpp:300-333, with its header,
+//             genomic-metadata and atom-site helpers :66-124) of given
+//             positions to given paths.  The writers' model forms (write)
+//             need a HierarchicalChromosome, whose members live in a CUDA
+//             source file, so they are neither called nor emitted here.
 //   drive_heatmap
 //             tree, then reconstructClustersHeatmapSingleLevel(level)
 //             (:1258-1422) on a given distance heatmap, with the chromosome
@@ -68,6 +74,9 @@
 #undef private
 #undef protected
 #include <Static.hpp>
+
+#include <PdbWriter.hpp>
+#include <CifWriter.hpp>
 
 #include "../pmmc_amd/integration/reference_glue.h"
 
@@ -784,6 +793,21 @@ static void mode_heat() {
   }
 }
 
+// flat: pdb_path cif_path chr n n_pos, 3n position floats, n_pos genomic
+// positions
+static void mode_flat() {
+  const std::string pdb = tok(), cif = tok(), chr = tok();
+  const int n = geti(), n_pos = geti();
+  std::vector<vector3> pos(n);
+  for (int i = 0; i < n; ++i) {
+    const float x = getf(), y = getf(), z = getf();
+    pos[i] = vector3(x, y, z);
+  }
+  std::vector<int> gpos(n_pos);
+  for (int i = 0; i < n_pos; ++i) gpos[i] = geti();
+  if (!PdbWriter::writeFlat(pos, chr, pdb) || !CifWriter::writeFlat(pos, gpos, chr, cif)) exit(3);
+}
+
 // stress: tree, chromosome name, springs "ks kq ka", "iterations lr", long
 // arcs ("n" then "start end count" each), then the positions of the
 // chromosome's anchors in cluster order.  Prints the refined active region
@@ -936,6 +960,8 @@ int main() {
   } else if (mode == "stress") {
     read_tree(s);
     mode_stress(s);
+  } else if (mode == "flat") {
+    mode_flat();
   } else if (mode == "drive_heatmap") {
     read_tree(s);
     mode_drive_heatmap(s);

@@ -11,13 +11,24 @@ compiled from the reference sources in place by oracle/Makefile:
   heat      Heatmap::toFile (Heatmap.hpp:679-700) for every flag combination,
             and Heatmap::fromFile (:702-732) of the 1-field and 3-field
             headers
-The HCM framing lines (HierarchicalChromosome::toFile, a CUDA source file
-the image cannot compile) and the PDB / mmCIF writers (whose header pulls in
-a CUDA header the image lacks) have no compiled reference here; their tests
-check structure and round trips instead.  Run where the reference sources
-exist:
+  flat      PdbWriter::writeFlat (PdbWriter.hpp:296-335) and
+            CifWriter::writeFlat (CifWriter.hpp:300-333, with the header,
+            genomic-metadata and atom-site helpers :66-124 it shares with
+            the model writer) on special values (signed zero, rounding
+            halves, NaN and infinities, values wider than the PDB columns),
+            genomic positions shorter than the bead list, and a
+            100,050-bead chain past the PDB resSeq wrap (9,999) and serial
+            cap (99,999); that case's inputs come from a seed and its
+            outputs are kept as SHA-256 digests with their first and last
+            lines
+The HCM framing lines (HierarchicalChromosome::toFile) and the model forms
+of the PDB / mmCIF writers (PdbWriter::write, CifWriter::write) need a
+HierarchicalChromosome, whose members live in a CUDA source file the image
+cannot compile; their tests check structure and round trips instead.  Run
+where the reference sources exist:
     make -C oracle ref && python tests/golden/make_formats_fixtures.py
 """
+import hashlib
 import json
 import os
 import subprocess
@@ -111,8 +122,63 @@ def run_heat(mat, flags, read_text):
     return written, dict(size=size, start=start, resolution=res, diagonal=diag, values=hx(vals))
 
 
+FLAT_SPECIAL = [[-0.0, 0.0005, -0.0005], [0.00049999997, 1234.5675, -1234.5675],
+                [1e10, -99999.9, 1e-30], [float("nan"), float("inf"), float("-inf")],
+                [3.4e38, -3.4e38, 7.0], [12.25, -0.125, 99.9995], [1.0, 2.0, 3.0]]
+
+
+def flat_inputs(case):
+    """Positions [n, 3] float32 and genomic positions of a flat case."""
+    if case["kind"] == "special":
+        pos = np.array(FLAT_SPECIAL, np.float32)
+        return pos, np.array([0, 5, -5, 2_000_000_000, 123_456_789], np.int32)
+    rng = np.random.default_rng(case["seed"])
+    n = case["n"]
+    pos = rng.normal(0.0, case["scale"], size=(n, 3)).astype(np.float32)
+    gpos = np.cumsum(rng.integers(1, 50_000, size=case["n_pos"])).astype(np.int32)
+    return pos, gpos
+
+
+def run_flat(pos, gpos, chr_name):
+    with tempfile.TemporaryDirectory() as d:
+        pdb, cif = os.path.join(d, "f.pdb"), os.path.join(d, "f.cif")
+        text = f"flat {HEADER}\n{pdb} {cif} {chr_name} {len(pos)} {len(gpos)}\n"
+        text += " ".join(hx(pos)) + "\n" + " ".join(str(int(g)) for g in gpos) + "\n"
+        r = subprocess.run([BIN], input=text, capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, r.stderr
+        with open(pdb, "rb") as f:
+            a = f.read()
+        with open(cif, "rb") as f:
+            b = f.read()
+    return a, b
+
+
+def summary(data):
+    lines = data.decode().splitlines(keepends=True)
+    return dict(sha256=hashlib.sha256(data).hexdigest(), size=len(data), head=lines[:12],
+                tail=lines[-4:])
+
+
+def flat_cases():
+    cases = [dict(kind="special", chr="chr1"),
+             dict(kind="random", seed=11, n=200, n_pos=200, scale=30.0, chr="chrX"),
+             dict(kind="random", seed=12, n=40, n_pos=25, scale=800.0, chr="chr22"),
+             dict(kind="random", seed=13, n=100_050, n_pos=100_050, scale=20.0, chr="chr2")]
+    out = []
+    for c in cases:
+        pos, gpos = flat_inputs(c)
+        a, b = run_flat(pos, gpos, c["chr"])
+        if len(pos) <= 1000:
+            c = dict(c, pdb=a.decode(), cif=b.decode())
+        else:
+            c = dict(c, pdb_summary=summary(a), cif_summary=summary(b))
+        out.append(c)
+    return out
+
+
 def main():
     fx = dict(records=records_case())
+    fx["flat"] = flat_cases()
     fx["records"]["expected"] = run_records(fx["records"])
     rng = np.random.default_rng(9)
     heat = []

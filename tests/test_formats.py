@@ -12,13 +12,22 @@ structure files, .heat heatmaps.  Host code only; no GPU.
 * .heat: the written text equals Heatmap::toFile's for every flag
   combination, and reading equals Heatmap::fromFile (values, size, diagonal
   band, start / resolution as its init leaves them), byte / bit exact.
-* PDB and mmCIF: the reference writers' header includes a CUDA header the
-  image lacks, so byte parity is unpinned; checked instead: the PDB fixed
-  columns parse back to the scaled coordinates (centroid 0, farthest bead at
-  50), levels, annotation B-factors, chains, TER / HELIX / RESIDUE records,
-  residue numbers wrapping past 9,999, and the mmCIF coordinates equal the
-  PDB ones (the reference rounds them through the PDB column).
+* PDB and mmCIF flat writers: byte-exact against the reference's own
+  PdbWriter::writeFlat / CifWriter::writeFlat compiled in place (the mmCIF
+  header, genomic-metadata and atom-site helpers included, which the model
+  writer shares), on special values, short genomic-position lists and a
+  100,050-bead chain (SHA-256 of the reference's output).
+* PDB and mmCIF model writers: the reference's PdbWriter::write /
+  CifWriter::write need a HierarchicalChromosome, whose members live in a
+  CUDA source file the image cannot compile, so their byte parity is
+  unpinned; checked instead: the PDB fixed columns parse back to the scaled
+  coordinates (centroid 0, farthest bead at 50), levels, annotation
+  B-factors, chains, TER / HELIX / RESIDUE records, residue numbers wrapping
+  past 9,999, and the mmCIF coordinates equal the PDB ones (the reference
+  rounds them through the PDB column).
 """
+import hashlib
+import importlib.util
 import json
 import os
 
@@ -286,3 +295,33 @@ def test_pdb_residue_numbers_wrap(tmp_path):
     rows = [ln.split() for ln in open(q) if ln.startswith("ATOM ")]
     assert [r[-2:] for r in rows] == [["100", "chr7"], ["2000000", "chr7"], ["2", "chr7"]]
     assert rows[1][-3] == "2.00"
+
+
+def _golden_script():
+    path = os.path.join(HERE, "golden", "make_formats_fixtures.py")
+    spec = importlib.util.spec_from_file_location("make_formats_fixtures", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.mark.parametrize("k", range(len(FX["flat"])))
+def test_flat_writers_match_reference(tmp_path, k):
+    """pmmc_write_pdb_flat / pmmc_write_cif_flat against the reference's
+    PdbWriter::writeFlat / CifWriter::writeFlat on the same inputs."""
+    case = FX["flat"][k]
+    pos, gpos = _golden_script().flat_inputs(case)
+    p, q = tmp_path / "f.pdb", tmp_path / "f.cif"
+    T.write_pdb_flat(pos, case["chr"], p)
+    T.write_cif_flat(pos, gpos, case["chr"], q)
+    a, b = p.read_bytes(), q.read_bytes()
+    if "pdb" in case:
+        assert a.decode() == case["pdb"]
+        assert b.decode() == case["cif"]
+    else:
+        for data, want in ((a, case["pdb_summary"]), (b, case["cif_summary"])):
+            lines = data.decode().splitlines(keepends=True)
+            assert lines[:len(want["head"])] == want["head"]
+            assert lines[-len(want["tail"]):] == want["tail"]
+            assert len(data) == want["size"]
+            assert hashlib.sha256(data).hexdigest() == want["sha256"]
