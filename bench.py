@@ -351,6 +351,57 @@ def stress_case(n, seed):
     return case, pos
 
 
+def arcs_level_leg(ctx, with_cpu=True, cpu_seconds=8.0):
+    """A whole arc level (pmmc_arcs_level) over a synthetic genome: 4
+    chromosomes x 600 anchors in 82 interaction blocks, default.ini's driver
+    settings (one restart per block and stage, loop_density 5, noise 0.01 /
+    5.0, [simulation_arcs] / [simulation_arcs_smooth] on the GPU-kernel
+    presets), 256 chains per restart.  The driver's batched form (every
+    block of a stage in one launch) against the same jobs issued one plain
+    call each, in order, as the reference's reconstructClustersArcsDistances
+    dispatches them (LooperSolver.hpp:4027-4138, :4408-4463); both give the
+    same tree bit for bit.  Wall time includes the host's job preparation and
+    densification."""
+    import numpy as np
+    from pmmc_amd import settings, synth
+    from pmmc_amd import tree as T
+    g = synth.genome(seed=5, n_chr=4, n_anchors=600)
+    p = T.level_params(loop_density=5, noise_anchor=0.01, noise_subanchor=5.0,
+                       restarts_anchor=1, restarts_subanchor=1)
+    s_arcs = arcs_settings()
+    s_smooth = settings(max_temp=5.0, dt_temp=0.9999, improvement=0.99, tjs=50.0, tjc=20.0,
+                        min_successes=50, steps=50000, milestone_fails=3)
+
+    def run(one_call):
+        t = T.Tree(g["chrs"], g["splits"])
+        rng = np.random.default_rng(7)
+        for c in range(len(g["chrs"])):
+            ibs = t.level(c, 2)
+            t.set_positions(ibs, np.cumsum(rng.normal(0, 3.0, size=(len(ibs), 3)), 0).astype(np.float32))
+        po = T.arcs_level_opts(arcs=s_arcs, smooth=s_smooth, chains_per_restart=256,
+                               one_call_per_job=one_call)
+        t0 = time.perf_counter()
+        rep = T.arcs_level(ctx, t, -1, p, po, seed=1, segment=0)
+        return t, rep, time.perf_counter() - t0
+
+    run(0)                                  # grows the device arena once
+    tb, rb, wb = run(0)
+    ts, rs, ws = run(1)
+    n = len(tb)
+    pb, ps = tb.positions(list(range(n))), ts.positions(list(range(n)))
+    return {"workload": "arc level of a synthetic genome: 4 chromosomes x 600 anchors, "
+                        "82 interaction blocks, loop_density 5, one restart per block and stage, "
+                        "256 chains per restart, GPU-kernel presets of default.ini",
+            "beads_after": n, "jobs": int(rb["n_jobs"]), "moves": int(rb["moves"]),
+            "batched_s": wb, "batched_ms_anchor": rb["ms_anchor"], "batched_ms_smooth": rb["ms_smooth"],
+            "one_call_per_job_s": ws, "one_call_ms_anchor": rs["ms_anchor"],
+            "one_call_ms_smooth": rs["ms_smooth"], "speedup": ws / wb,
+            "batched_moves_per_s": rb["moves"] / wb,
+            "identical": bool(np.array_equal(pb.view(np.uint32), ps.view(np.uint32))
+                              and rb["score_anchor"] == rs["score_anchor"]
+                              and rb["score_smooth"] == rs["score_smooth"])}
+
+
 def stress_leg(ctx, with_cpu=True, cpu_seconds=8.0):
     """LooperSolver::stressRefineAnchors (the global anchor refinement's
     gradient descent) on one chromosome of 20,000 anchors: pmmc_tree_stress_
@@ -397,7 +448,8 @@ def stress_leg(ctx, with_cpu=True, cpu_seconds=8.0):
 
 
 LEGS = {"heatmap_64": heatmap_leg, "heatmap_64_sweep": heatmap_sweep_leg,
-        "smooth_19951": smooth_leg, "singleton_20M": singleton_leg, "stress_20k": stress_leg}
+        "smooth_19951": smooth_leg, "singleton_20M": singleton_leg, "stress_20k": stress_leg,
+        "arcs_level": arcs_level_leg}
 # legs runnable alone with --only (not part of `levels`)
 ONLY = dict(LEGS, batched=lambda ctx, with_cpu=True: batched(ctx),
             served=lambda ctx, with_cpu=True: served(ctx))

@@ -268,6 +268,9 @@ typedef struct {
   int32_t densify;            /* 0: stop after the anchor stage                */
   int32_t global_refine;      /* useGlobalAnchorRefinement (default 0)         */
   pmmc_stress_opts stress;    /* stressRefineIterations / LR of the global pass */
+  int32_t one_call_per_job;   /* 1: each block restart as its own plain call, in
+                                 order, as the reference dispatches them (same
+                                 results; for comparison); 0: one batch per stage */
 } pmmc_arcs_level_opts;
 
 void pmmc_arcs_level_opts_default(pmmc_arcs_level_opts *o);
@@ -320,6 +323,7 @@ typedef struct {
   pmmc_level_settings heatmap;  /* [simulation_heatmap] + [cuda] settings       */
   int32_t preset;               /* pmmc_preset                                  */
   int32_t chains_per_restart;   /* 0 = share the library default chain pool     */
+  int32_t one_call_per_job;     /* as in pmmc_arcs_level_opts                   */
 } pmmc_heatmap_level_opts;
 
 void pmmc_heatmap_level_opts_default(pmmc_heatmap_level_opts *o);

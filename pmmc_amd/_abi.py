@@ -208,11 +208,13 @@ class ArcsLevelOpts(C.Structure):
                 ("k_squeeze_arcs", C.c_float), ("k_stretch", C.c_float), ("k_squeeze", C.c_float),
                 ("k_angular", C.c_float), ("w_dist", C.c_float), ("w_angle", C.c_float),
                 ("chains_per_restart", C.c_int32), ("densify", C.c_int32),
-                ("global_refine", C.c_int32), ("stress", StressOpts)]
+                ("global_refine", C.c_int32), ("stress", StressOpts),
+                ("one_call_per_job", C.c_int32)]
 
 
 class HeatmapLevelOpts(C.Structure):
-    _fields_ = [("heatmap", LevelSettings), ("preset", C.c_int32), ("chains_per_restart", C.c_int32)]
+    _fields_ = [("heatmap", LevelSettings), ("preset", C.c_int32), ("chains_per_restart", C.c_int32),
+                ("one_call_per_job", C.c_int32)]
 
 
 SIGNATURES = {

@@ -98,15 +98,34 @@ int32_t chains_for(pmmc_ctx *ctx, int32_t requested, size_t n_jobs, int32_t leve
   return static_cast<int32_t>(ens) * 64;
 }
 
-// runs the stage's jobs as one batch; returns moves / accepted
+// runs the stage's jobs as one batch, or (one_call) each as its own plain
+// call in order, as the reference dispatches them; the results are the same
+// bit for bit.  Adds moves / accepted / jobs to the report.
 pmmc_status run_stage(pmmc_ctx *ctx, std::vector<std::unique_ptr<StageJob>> &jobs,
-                      pmmc_level_report *rep) {
+                      pmmc_level_report *rep, bool one_call) {
   if (jobs.empty()) return PMMC_OK;
-  std::vector<pmmc_job> js;
-  js.reserve(jobs.size());
-  for (auto &j : jobs) js.push_back(j->job);
-  pmmc_batch_opts bo{};
-  const pmmc_status st = pmmc_anneal_batch(ctx, js.data(), (int32_t)js.size(), &bo);
+  pmmc_status st = PMMC_OK;
+  if (one_call) {
+    for (auto &j : jobs) {
+      const pmmc_job &b = j->job;
+      if (b.level == PMMC_LEVEL_HEATMAP)
+        st = pmmc_heatmap_anneal(ctx, static_cast<const pmmc_heatmap_problem *>(b.problem),
+                                 &b.schedule, &b.run, b.pos_out, b.stats);
+      else if (b.level == PMMC_LEVEL_ARCS)
+        st = pmmc_arcs_anneal(ctx, static_cast<const pmmc_arcs_problem *>(b.problem), &b.schedule,
+                              &b.run, b.pos_out, b.stats);
+      else
+        st = pmmc_smooth_anneal(ctx, static_cast<const pmmc_smooth_problem *>(b.problem),
+                                &b.schedule, &b.run, b.pos_out, b.stats);
+      if (st != PMMC_OK) break;
+    }
+  } else {
+    std::vector<pmmc_job> js;
+    js.reserve(jobs.size());
+    for (auto &j : jobs) js.push_back(j->job);
+    pmmc_batch_opts bo{};
+    st = pmmc_anneal_batch(ctx, js.data(), (int32_t)js.size(), &bo);
+  }
   if (st != PMMC_OK) return fail(std::string("level driver: ") + pmmc_last_error(ctx), st);
   for (auto &j : jobs) {
     rep->moves += j->st.moves;
@@ -344,7 +363,7 @@ pmmc_status global_stage(pmmc_ctx *ctx, pmmc_tree &t, const std::vector<int32_t>
       jobs.push_back(std::move(j));
     }
   }
-  pmmc_status st = run_stage(ctx, jobs, &rep);
+  pmmc_status st = run_stage(ctx, jobs, &rep, o->one_call_per_job != 0);
   if (st != PMMC_OK) return st;
   {
     std::vector<Best> best(todo.size());
@@ -539,7 +558,7 @@ extern "C" pmmc_status pmmc_arcs_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t chr,
     }
   }
   double t0 = now_ms();
-  pmmc_status st = run_stage(ctx, jobs, &rep);
+  pmmc_status st = run_stage(ctx, jobs, &rep, o->one_call_per_job != 0);
   if (st != PMMC_OK) return st;
   rep.ms_anchor = now_ms() - t0;
   {
@@ -620,7 +639,7 @@ extern "C" pmmc_status pmmc_arcs_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t chr,
     }
   }
   t0 = now_ms();
-  st = run_stage(ctx, jobs, &rep);
+  st = run_stage(ctx, jobs, &rep, o->one_call_per_job != 0);
   if (st != PMMC_OK) return st;
   rep.ms_smooth = now_ms() - t0;
   {
@@ -723,7 +742,7 @@ extern "C" pmmc_status pmmc_heatmap_level(pmmc_ctx *ctx, pmmc_tree *t, int32_t l
   }
   rep.n_blocks = 1;
   const double t0 = now_ms();
-  st = run_stage(ctx, jobs, &rep);
+  st = run_stage(ctx, jobs, &rep, o->one_call_per_job != 0);
   if (st != PMMC_OK) return st;
   rep.ms_anchor = now_ms() - t0;
   Best best;

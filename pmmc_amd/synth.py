@@ -11,6 +11,11 @@ arcs_problem:     one segment of n anchors with ~3 arcs per anchor, separation
                   clamped >= 4, duplicate pairs merged, expected distance by the
                   power-law branch of LooperSolver::freqToDistance
                   (LooperSolver.hpp:3956-3962): base + scale * (count + shift)^a.
+genome:           chromosomes of anchors 4-30 kb apart, grouped into
+                  interaction blocks of 10-60 anchors with arcs (PET counts
+                  2-79) only inside a block, so LooperSolver::findGaps
+                  (LooperSolver.hpp:2119-2180) finds the blocks' borders;
+                  predefined segment splits per chromosome.
 smooth_problem:   anchors fixed, `add` interpolated sub-anchor beads between
                   consecutive anchors (LooperSolver::densifyActiveRegion,
                   LooperSolver.hpp:3836-3898), dist_to_next = 1.
@@ -154,6 +159,35 @@ def batch_arcs_problems(n_segments: int = 32, n_seeds: int = 16, seed: int = 0,
             probs.append(Arcs(pos=pos.astype(np.float32), exp_dist=base.exp_dist))
             meta.append((s, e, n))
     return probs, meta
+
+
+def genome(seed: int = 0, n_chr: int = 4, n_anchors: int = 600, block: tuple = (10, 60),
+           span: int = 4, splits_per_chr: int = 4):
+    """Input of pmmc_amd.tree.Tree: {"chrs": [{name, anchors [(start, end)],
+    arcs [(a, b, count)]}], "splits": [(name, start, end)]}."""
+    rng = np.random.default_rng(seed)
+    chrs, splits = [], []
+    for c in range(n_chr):
+        n = n_anchors
+        gaps = rng.integers(4000, 30000, size=n)
+        lens = rng.integers(500, 3000, size=n)
+        starts = 10_000 + np.cumsum(gaps + lens) - lens
+        anchors = [(int(a), int(a + ln)) for a, ln in zip(starts, lens)]
+        pairs = set()
+        b0 = 0
+        while b0 < n:
+            b1 = min(n, b0 + int(rng.integers(block[0], block[1] + 1)))
+            for a in range(b0, b1 - 1):   # a chain of arcs spanning the block
+                pairs.add((a, min(b1 - 1, a + int(rng.integers(1, span + 1)))))
+            b0 = b1
+        arcs = [(a, b, int(rng.integers(2, 80))) for a, b in sorted(pairs)]
+        name = f"chr{c + 1}"
+        chrs.append(dict(name=name, anchors=anchors, arcs=arcs))
+        lo, hi = anchors[0][0] - 1000, anchors[-1][1] + 1000
+        cuts = sorted(int(x) for x in rng.integers(lo + 1, hi, size=splits_per_chr - 1))
+        edges = [lo] + cuts + [hi]
+        splits += [(name, edges[i], edges[i + 1]) for i in range(splits_per_chr)]
+    return dict(chrs=chrs, splits=splits)
 
 
 def heatmap_avg(h: np.ndarray) -> float:

@@ -265,3 +265,46 @@ def test_heatmap_level_matches_oracle(ctx, level):
     assert rep["n_jobs"] == orep["n_jobs"]
     slack = assert_tree_positions_match(pt, ot)
     assert rep["score_anchor"] == pytest.approx(orep["score_anchor"], rel=1e-9, abs=slack)
+
+
+def _tree_bits(t):
+    return t.positions(list(range(len(t)))).view(np.uint32)
+
+
+def test_level_drivers_one_call_per_job_equals_batched(ctx):
+    """one_call_per_job (each block restart as its own plain call, the
+    reference's dispatch) gives the batched drivers' trees bit for bit."""
+    case, rng = genome(33, n_chr=2, n_anchors=(30, 26), span=3, splits_per_chr=3)
+    ib_pos = rng.normal(0, 2.0, size=(64, 3)).astype(np.float32)
+    po, _ = arcs_opts(steps=1024, chains=64)
+    out = []
+    for one_call in (0, 1):
+        pt = T.Tree(case["chrs"], case["splits"])
+        ibs = [ib for c in range(2) for ib in pt.level(c, 2)]
+        pt.set_positions(ibs, ib_pos[:len(ibs)])
+        po.one_call_per_job = one_call
+        print(f"arcs_level one_call_per_job={one_call}", flush=True)
+        rep = T.arcs_level(ctx, pt, -1, product_params(PARAMS), po, seed=4, segment=11)
+        out.append((_tree_bits(pt), rep))
+    assert out[0][1]["n_jobs"] == out[1][1]["n_jobs"] > 4
+    assert np.array_equal(out[0][0], out[1][0])
+    for k in ("score_anchor", "score_smooth", "moves", "accepted"):
+        assert out[0][1][k] == out[1][1][k], k
+    # heatmap level 1 on the same genome
+    n = sum(len(T.Tree(case["chrs"], case["splits"]).level(c, 1)) for c in range(2))
+    idx = np.arange(n)
+    f = (1.0 / (np.abs(idx[:, None] - idx[None, :]) ** 3 + 0.1)).astype(np.float32)
+    s = api.settings(max_temp=5.0, dt_temp=0.9999, improvement=0.99, steps=2048, milestone_fails=3)
+    res = []
+    for one_call in (0, 1):
+        pt = T.Tree(case["chrs"], case["splits"])
+        dist, _ = T.distance_heatmap(f, 1, False, product_params(PARAMS))
+        roots = [pt.chromosome(c)[1] for c in range(2)]
+        pt.set_positions(roots, np.array([[0, 0, 0], [3, 1, 0]], np.float32))
+        ho = T.heatmap_level_opts(heatmap=s, preset=1, chains_per_restart=64,
+                                  one_call_per_job=one_call)
+        print(f"heatmap_level one_call_per_job={one_call}", flush=True)
+        rep = T.heatmap_level(ctx, pt, 1, dist, 1, product_params(PARAMS), ho, seed=9, segment=3)
+        res.append((_tree_bits(pt), rep))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert res[0][1]["score_anchor"] == res[1][1]["score_anchor"]
