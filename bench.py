@@ -22,6 +22,12 @@ Beside it, in the same line:
              smooth GPU kernel's schedule): moves/s, final energy, and for the heatmap the
              distance-map Pearson / SCC / RMSD to the ground-truth polymer,
              each beside the reference's CPU annealer on the host's cores;
+  matched_energy -- the headline segment annealed to completion under the
+             reference CPU annealer's own schedule: complete anneals per
+             second on the GPU against the CPU baseline's, with each side's
+             median final energy of one chain (the headline's GPU-kernel
+             schedule caps a chain at 50,176 moves, so its solves end at a
+             higher energy than the CPU's complete anneals);
   cpu_baseline -- the reference's own CPU arc annealer (LooperSolver::
              MonteCarloArcs, compiled in place into oracle/_ref) timed on the
              host's cores; the oracle's restatement when that build is absent.
@@ -127,9 +133,11 @@ def cpu_reference(level, prob, step, settings_dict, seconds, sample_moves=0):
     sd = dict(settings_dict)
     if sample_moves > 0:
         sd.update(steps=sample_moves, improvement=0.0, min_successes=2**31 - 1)
-    moves, secs, runs = R.time_anneal(level, prob, step, sd, seconds, threads)
+    moves, secs, runs, scores = R.time_anneal(level, prob, step, sd, seconds, threads)
     return {"value": moves / secs, "unit": "moves/s", "cores": threads, "kind": "reference",
             "per_core": moves / secs / threads, "runs": runs, "seconds": secs,
+            "runs_per_s": runs / secs,
+            "final_energy_median": float(sorted(scores)[len(scores) // 2]),
             "sample": (f"{runs} runs of {sample_moves} moves" if sample_moves > 0 else
                        f"{runs} complete anneals") + f" on {threads} host threads"}
 
@@ -452,6 +460,7 @@ LEGS = {"heatmap_64": heatmap_leg, "heatmap_64_sweep": heatmap_sweep_leg,
         "arcs_level": arcs_level_leg}
 # legs runnable alone with --only (not part of `levels`)
 ONLY = dict(LEGS, batched=lambda ctx, with_cpu=True: batched(ctx),
+            matched_energy=lambda ctx, with_cpu=True: matched_energy(ctx),
             served=lambda ctx, with_cpu=True: served(ctx))
 
 
@@ -536,6 +545,37 @@ def batched(ctx, chains_per_problem=256, plain_subset=16):
         "pearson_to_targets_max_diff_vs_plain": float(p_diff),
         "final_energy_median": float(np.median([r[1]["energy"] for r in res])),
     }
+
+
+def matched_energy(ctx, sample_chains=256):
+    """The headline segment annealed to completion under the reference CPU
+    annealer's own schedule (LooperSolver::MonteCarloArcs, the CPU preset:
+    milestones every 50,000 moves, stop at the first one that fails), so that
+    every GPU chain is the Markov chain the CPU baseline runs and a chain is
+    one complete anneal, the CPU baseline's unit of work.  One plain call with
+    the default chain count gives the throughput; `sample_chains`
+    single-chain jobs give the distribution of one chain's final energy, to
+    set against the CPU runs' (cpu_baseline.final_energy_median)."""
+    import numpy as np
+    from pmmc_amd import schedule_preset, settings, synth
+    prob, _, step, _ = workload()
+    sched = schedule_preset(2, 0, settings(**synth.CONFIG_SETTINGS[2]), step)
+    t0 = time.perf_counter()
+    _, st = ctx.anneal(prob, sched, seed=7, segment=0)
+    wall = time.perf_counter() - t0
+    res = ctx.anneal_batch([prob] * sample_chains, sched, seeds=[11] * sample_chains,
+                           segments=[0] * sample_chains,
+                           chain0s=[64 * k for k in range(sample_chains)], n_chains=1)
+    e1 = np.array([r[1]["energy"] for r in res])
+    return {"workload": "the headline segment (400 anchors) annealed to completion under "
+                        "LooperSolver::MonteCarloArcs' schedule (default.ini [simulation_arcs])",
+            "chains": st["n_chains"], "anneals_per_s": st["n_chains"] / wall,
+            "solve_s": wall, "moves_per_chain_mean": st["moves"] / st["n_chains"],
+            "best_chain_energy": st["energy"],
+            "single_chain_sample": sample_chains,
+            "single_chain_energy_median": float(np.median(e1)),
+            "single_chain_energy_p10_p90": [float(np.percentile(e1, 10)),
+                                            float(np.percentile(e1, 90))]}
 
 
 def _alg_bytes_per_move(level, n, deg, acc_rate):
@@ -771,6 +811,7 @@ def main(argv=None):
             extra["batched"] = batched(ctx)
         if not a.no_served:
             extra["served"] = served(ctx)
+        extra["matched_energy"] = matched_energy(ctx)
         if not a.no_levels:
             extra["levels"] = levels(ctx, with_cpu=not a.no_cpu)
     value = moves / wall
@@ -814,6 +855,16 @@ def main(argv=None):
     line.update(extra)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(prob, s, step)
+        cb = line["cpu_baseline"]
+        if cb.get("kind") == "reference":
+            # complete anneals on both sides: the GPU's per second over the
+            # host's, with each side's median final energy of one chain
+            me = line["matched_energy"]
+            me["cpu_anneals_per_s"] = cb["runs_per_s"]
+            me["cpu_final_energy_median"] = cb["final_energy_median"]
+            me["anneals_speedup_vs_cpu"] = me["anneals_per_s"] / cb["runs_per_s"]
+            me["median_energy_rel_diff_vs_cpu"] = (me["single_chain_energy_median"] /
+                                                   cb["final_energy_median"] - 1.0)
     print(json.dumps(line))
     return 0
 

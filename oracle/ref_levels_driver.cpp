@@ -381,25 +381,29 @@ static void mode_anneal(LooperSolver &s) {
 // time_anneal: `seconds`, then the anneal input.  Runs the annealer from the
 // same initial positions with seeds seed, seed+1, ... until `seconds` of
 // annealing have passed (at least once); prints the runs, the moves and the
-// seconds spent inside the annealer (steady_clock).
+// seconds spent inside the annealer (steady_clock), then the score each run
+// returned.
 static void mode_time_anneal(LooperSolver &s) {
   const double budget = std::stod(tok());
   const AnnealInput in = read_anneal(s);
   long long moves = 0;
   double secs = 0.0;
   int runs = 0;
+  std::vector<double> scores;
   do {
     for (int i = 0; i < in.n; ++i)
       s.clusters[i].pos.set(in.pos[3 * i], in.pos[3 * i + 1], in.pos[3 * i + 2]);
     srand(in.seed + runs);
     const long long before = s.total_mc_steps;
     const auto t0 = std::chrono::steady_clock::now();
-    run_annealer(s, in);
+    scores.push_back(run_annealer(s, in));
     secs += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     moves += s.total_mc_steps - before;
     ++runs;
   } while (secs < budget);
-  printf("\n@@time %d %lld %.9f\n", runs, moves, secs);
+  printf("\n@@time %d %lld %.9f\n@@scores", runs, moves, secs);
+  for (double v : scores) printf(" %a", v);
+  printf("\n");
 }
 
 // metrics: n h, two structures (n xyz each), two n*n maps, a step for

@@ -69,7 +69,8 @@ def anneal(level: int, prob, step: float, settings: dict, seed: int):
 def time_anneal(level: int, prob, step: float, settings: dict, seconds: float, threads: int):
     """`threads` processes each annealing repeatedly (seeds 1, 2, ...) for about
     `seconds` of annealing time.  Returns (moves, annealing seconds per
-    process (max), runs): whole-host moves/s = moves / seconds."""
+    process (max), runs, the score each run returned): whole-host moves/s =
+    moves / seconds."""
     text = f"time_anneal {HEADER}\n{seconds}\n"
     procs = []
     for t in range(threads):
@@ -79,17 +80,19 @@ def time_anneal(level: int, prob, step: float, settings: dict, seconds: float, t
         p.stdin.write(inp)
         p.stdin.close()
         procs.append(p)
-    moves, secs, runs = 0, 0.0, 0
+    moves, secs, runs, scores = 0, 0.0, 0, []
     for p in procs:
         out = p.stdout.read()
         if p.wait() != 0:
             raise RuntimeError("reference annealer failed")
-        line = [x for x in out.splitlines() if x.startswith("@@time")][0]
+        lines = out.splitlines()
+        line = [x for x in lines if x.startswith("@@time")][0]
         _, r, m, s = line.split()
         runs += int(r)
         moves += int(m)
         secs = max(secs, float(s))
-    return moves, secs, runs
+        scores += [float.fromhex(t) for t in [x for x in lines if x.startswith("@@scores")][0].split()[1:]]
+    return moves, secs, runs, scores
 
 
 def singleton_time(chrs, clusters, rec, diag=1, long_scale=10.0, long_power=2.0) -> tuple[float, int]:

@@ -73,15 +73,16 @@ __global__ __launch_bounds__(64) void k_probe_moves(float *__restrict__ buf, uin
 #pragma unroll
           for (int k = 0; k < CB * 3; ++k) a[k % 3] += v[u][r][k];
       float *w = buf + static_cast<size_t>(first[u]) * 3 * kProbeLanes;
-      if constexpr (MODE == 0) {
-        acc += a[0] + a[1] + a[2];
-      } else {
+      // every move's loads feed acc, so the compiler cannot sink them into
+      // the write-back branch (where they would be issued only when a lane
+      // writes, and waited for at once)
+      acc += a[0] + a[1] + a[2];
+      if constexpr (MODE != 0) {
         const bool take = MODE == 2 ||
                           (MODE == 1 && (probe_hash(seed * 31u + it + u + lane * 977u) & 1u)) ||
                           (MODE == 3 && lane == (probe_hash(seed + it + u) & 127u));
         // the written value equals what was read (the sums are scaled to 0):
-        // the buffer keeps its contents.  Every mode's condition depends on
-        // run-time values, so no move's loads are dead code.
+        // the buffer keeps its contents
 #pragma unroll
         for (int k = 0; k < 3; ++k)
           if (take) w[k * kProbeLanes + lane] = fmaf(a[k], 0.f, v[u][0][k]);
