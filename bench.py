@@ -640,18 +640,24 @@ def served(ctx, seconds=20.0, rate=200.0, seed=0, plain_sample=24, window=5.0, l
     pending = set()
     nxt = 0
     steps = 0
+    t_submit = t_step = 0.0
     while len(done_at) < len(reqs):
         now = time.perf_counter() - t0
         while nxt < len(reqs) and reqs[nxt][0] <= now:
             t, (p, sc, _, _, _), ens = reqs[nxt]
-            jobs[nxt] = [srv.submit(p, sc, seed=1 + e, segment=nxt, n_chains=64) for e in range(ens)]
+            ts = time.perf_counter()
+            jobs[nxt] = srv.submit_many(p, sc, seeds=[1 + e for e in range(ens)], segment=nxt,
+                                        n_chains=64)
+            t_submit += time.perf_counter() - ts
             pending.add(nxt)
             nxt += 1
         if not pending:
             if nxt < len(reqs):
                 time.sleep(max(0.0, min(0.001, reqs[nxt][0] - now)))
             continue
+        ts = time.perf_counter()
         n_done, _, _ = srv.step()
+        t_step += time.perf_counter() - ts
         steps += 1
         if n_done:
             now = time.perf_counter() - t0
@@ -698,6 +704,7 @@ def served(ctx, seconds=20.0, rate=200.0, seed=0, plain_sample=24, window=5.0, l
     achieved = alg_bytes / wall / 1e9
     return {"requests": len(reqs), "arrival_rate_per_s": rate, "arrival_seconds": seconds,
             "solves": solves, "steps": steps, "wall_s": wall,
+            "host_s": {"submit": t_submit, "step": t_step},
             "solves_per_s": solves / wall, "requests_per_s": len(reqs) / wall,
             "moves_per_s": moves / wall, "accepted_moves_per_s": acc_served / wall,
             "p50_latency_ms": float(np.percentile(lat, 50)),

@@ -430,7 +430,9 @@ typedef struct {
  * 1,024 moves per chain; the chains advance meanwhile, so no work is
  * repeated): chains_per_wave by the higher moves/s, blocks_per_step for
  * about 20 ms of device time per step, steps_per_graph for about 100 ms per
- * graph replay; the values used are written back to *opts. */
+ * graph replay; the values used are written back to *opts.  Jobs that name
+ * the same level and problem struct as an earlier job share its host
+ * preparation (the tables are built once and copied). */
 pmmc_status pmmc_anneal_batch(pmmc_ctx *ctx, const pmmc_job *jobs, int32_t n_jobs,
                               pmmc_batch_opts *opts);
 
@@ -454,6 +456,14 @@ pmmc_status pmmc_server_create(pmmc_ctx *ctx, const pmmc_server_opts *opts, pmmc
 void pmmc_server_destroy(pmmc_server *srv);
 /* Queue a job (inputs are copied).  *job_id identifies it in poll. */
 pmmc_status pmmc_server_submit(pmmc_server *srv, const pmmc_job *job, uint64_t *job_id);
+/* Queue n_jobs jobs (no reference counterpart: for requests of several
+ * solves, e.g. an ensemble of seeds on one problem).  Jobs that name the same
+ * level and problem struct as an earlier job of the list share its host
+ * preparation (the tables are built once and copied); each job's results are
+ * those pmmc_server_submit of it alone gives.  job_ids[k] identifies job k
+ * (nullable).  On error nothing is queued. */
+pmmc_status pmmc_server_submit_many(pmmc_server *srv, const pmmc_job *jobs, int32_t n_jobs,
+                                    uint64_t *job_ids);
 /* Admit queued jobs that fit the budget, run one launch per launch group over
  * every resident chain, retire finished jobs (results written to the job's
  * pos_out / stats).  *n_done = jobs finished in this step; *n_resident and

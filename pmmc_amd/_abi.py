@@ -249,6 +249,8 @@ SIGNATURES = {
     "pmmc_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
     "pmmc_server_destroy": (None, [C.c_void_p]),
     "pmmc_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_uint64)]),
+    "pmmc_server_submit_many": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int32,
+                                          C.POINTER(C.c_uint64)]),
     "pmmc_server_step": (C.c_int, [C.c_void_p, i32p, i32p, i32p]),
     "pmmc_server_done": (C.c_int, [C.c_void_p, C.c_uint64]),
     "pmmc_singleton_heatmap": (C.c_int, [C.c_void_p, C.POINTER(SingletonInput), f32p,

@@ -396,10 +396,13 @@ class Context:
         autotuned; opts_out (a dict, optional) receives the values used."""
         k = len(problems)
         jobs = (A.Job * k)()
-        keep, outs, stats = [], [], []
+        keep, outs, stats = {}, [], []
         for i, p in enumerate(problems):
-            cp = p.c()
-            keep.append(cp)
+            # one problem struct per problem object: the library prepares a
+            # problem that several jobs name once
+            if id(p) not in keep:
+                keep[id(p)] = (p, p.c())
+            cp = keep[id(p)][1]
             o = _f32(p.pos).reshape(-1, 3).copy()
             s = A.Stats()
             outs.append(o)
@@ -442,6 +445,26 @@ class Server:
         self.ctx._check(self.lib.pmmc_server_submit(self.h, C.byref(job), C.byref(jid)))
         self.jobs[jid.value] = (prob, cp, out, st)
         return jid.value
+
+    def submit_many(self, prob, sched: A.Schedule, seeds, segment=0, n_chains=0):
+        """One job per seed on the same problem (pmmc_server_submit_many: the
+        problem is prepared once); returns the job ids."""
+        cp = prob.c()
+        k = len(seeds)
+        jobs = (A.Job * k)()
+        outs = []
+        for i, sd in enumerate(seeds):
+            out = _f32(prob.pos).reshape(-1, 3).copy()
+            st = A.Stats()
+            jobs[i] = A.Job(prob.level, C.cast(C.pointer(cp), C.c_void_p), sched,
+                            A.Run(int(sd), segment, 0, n_chains), out.ctypes.data_as(A.f32p),
+                            C.pointer(st))
+            outs.append((out, st))
+        ids = (C.c_uint64 * k)()
+        self.ctx._check(self.lib.pmmc_server_submit_many(self.h, jobs, k, ids))
+        for i in range(k):
+            self.jobs[ids[i]] = (prob, cp, outs[i][0], outs[i][1])
+        return list(ids)
 
     def step(self):
         d, r, q = C.c_int32(), C.c_int32(), C.c_int32()

@@ -228,38 +228,46 @@ static pmmc_status prep_heatmap(pmmc_ctx *ctx, const pmmc_heatmap_problem *p, Ho
   if (p->diagonal < 0 || p->n_boundaries < 0 || (p->n_boundaries > 0 && !p->boundaries))
     return fail(ctx, PMMC_ERR_INVALID, "heatmap: bad diagonal or boundaries");
   const size_t nn = static_cast<size_t>(n) * n;
-  for (size_t i = 0; i < nn; ++i)
-    if (std::isnan(p->dist[i])) return fail(ctx, PMMC_ERR_INVALID, "heatmap: NaN in dist");
+  bool nan = false;
+  for (size_t i = 0; i < nn; ++i) nan |= std::isnan(p->dist[i]);
+  if (nan) return fail(ctx, PMMC_ERR_INVALID, "heatmap: NaN in dist");
   std::vector<int> st(n), en(n);
   for (int m = 0; m < n; ++m) block_of(p->boundaries, p->n_boundaries, n, m, st[m], en[m]);
   const float thr = p->skip_below;
-  auto valid = [&](int i, int m) -> bool {  // term i of local(m)
-    if (i < st[m] || i > en[m]) return false;
-    if (i == m) return false;
-    if (std::abs(i - m) < p->diagonal) return false;
-    return p->dist[static_cast<size_t>(i) * n + m] >= thr;
-  };
+  const int diag = p->diagonal;
   // rows padded with zeros to 32 / 64 entries for small heatmaps, so the
   // register-resident kernel reads whole rows with wide scalar loads
   h.ldw = n <= 32 ? 32 : (n <= 64 ? 64 : n);
-  const size_t nl = static_cast<size_t>(n) * h.ldw;
+  const int ldw = h.ldw;
+  const size_t nl = static_cast<size_t>(n) * ldw;
   h.W.assign(nl, 0.f);
   h.Hc.assign(nl, 0.f);
   std::vector<float> Hc2(nl, 0.f);
   h.W2.assign(nl, 0.f);
-  for (int q = 0; q < n; ++q)
+  // W2 / Hc2 row q: the term of bead q in local(j), valid when q lies in j's
+  // chromosome block, off the diagonal band and at or above the threshold
+  // (row q of dist, read in order; selects instead of branches)
+  for (int q = 0; q < n; ++q) {
+    const float *dr = p->dist + static_cast<size_t>(q) * n;
+    float *w2 = h.W2.data() + static_cast<size_t>(q) * ldw;
+    float *c2 = Hc2.data() + static_cast<size_t>(q) * ldw;
     for (int j = 0; j < n; ++j) {
-      if (valid(j, q)) {
-        const float hv = p->dist[static_cast<size_t>(j) * n + q];
-        h.W[static_cast<size_t>(q) * h.ldw + j] = 1.0f / hv;
-        h.Hc[static_cast<size_t>(q) * h.ldw + j] = hv;
-      }
-      if (valid(q, j)) {
-        const float hv = p->dist[static_cast<size_t>(q) * n + j];
-        h.W2[static_cast<size_t>(q) * h.ldw + j] = 1.0f / hv;
-        Hc2[static_cast<size_t>(q) * h.ldw + j] = hv;
-      }
+      const float hv = dr[j];
+      const int dq = q > j ? q - j : j - q;
+      const bool ok = q >= st[j] && q <= en[j] && q != j && dq >= diag && hv >= thr;
+      w2[j] = ok ? 1.0f / hv : 0.f;
+      c2[j] = ok ? hv : 0.f;
     }
+  }
+  // W / Hc row q: the term of bead j in local(q) -- the same value as W2 /
+  // Hc2 at (j, q), so W and Hc are their transposes (32 x 32 tiles)
+  for (int q0 = 0; q0 < n; q0 += 32)
+    for (int j0 = 0; j0 < n; j0 += 32)
+      for (int q = q0; q < std::min(n, q0 + 32); ++q)
+        for (int j = j0; j < std::min(n, j0 + 32); ++j) {
+          h.W[static_cast<size_t>(q) * ldw + j] = h.W2[static_cast<size_t>(j) * ldw + q];
+          h.Hc[static_cast<size_t>(q) * ldw + j] = Hc2[static_cast<size_t>(j) * ldw + q];
+        }
   // The kernels' symmetric form (ΔE = 2 * the row-p terms, re-sums over i < j
   // doubled) is exact only when the term of partner j in local(p) equals the
   // term of p in local(j) for every pair.  That depends on the targets AND
@@ -282,19 +290,29 @@ static pmmc_status prep_arcs(pmmc_ctx *ctx, const pmmc_arcs_problem *p, HostProb
     return fail(ctx, PMMC_ERR_INVALID, "arcs: bad loop arrays");
   const float *ed = p->exp_dist;
   std::vector<std::vector<ArcEntry>> rows(n);
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) {
-      const float e = ed[static_cast<size_t>(i) * n + j];
-      if (std::isnan(e)) return fail(ctx, PMMC_ERR_INVALID, "arcs: NaN in exp_dist");
-      if (j == i) continue;
-      if (e >= 1e-6f) {
-        const float e2 = e * e;
-        rows[i].push_back(ArcEntry{j, e, p->k_stretch / e2, p->k_squeeze / e2});
-      } else if (e >= 0.f && j > i) {
-        h.zpairs.push_back(i);
-        h.zpairs.push_back(j);
+  for (int i = 0; i < n; ++i) {
+    const float *row = ed + static_cast<size_t>(i) * n;
+    for (int j0 = 0; j0 < n; j0 += 64) {
+      const int j1 = std::min(n, j0 + 64);
+      // nearly every entry is negative (no arc): a run of 64 with no
+      // non-negative or NaN entry is passed over after one vector test
+      bool any = false;
+      for (int j = j0; j < j1; ++j) any |= !(row[j] < 0.f);
+      if (!any) continue;
+      for (int j = j0; j < j1; ++j) {
+        const float e = row[j];
+        if (std::isnan(e)) return fail(ctx, PMMC_ERR_INVALID, "arcs: NaN in exp_dist");
+        if (j == i) continue;
+        if (e >= 1e-6f) {
+          const float e2 = e * e;
+          rows[i].push_back(ArcEntry{j, e, p->k_stretch / e2, p->k_squeeze / e2});
+        } else if (e >= 0.f && j > i) {
+          h.zpairs.push_back(i);
+          h.zpairs.push_back(j);
+        }
       }
     }
+  }
   for (int k = 0; k < p->n_loops; ++k) {
     const int a = p->loop_pairs[2 * k], b = p->loop_pairs[2 * k + 1];
     if (a < 0 || a >= n || b < 0 || b >= n) return fail(ctx, PMMC_ERR_INVALID, "arcs: loop index out of range");
@@ -355,6 +373,12 @@ pmmc_status prepare_problem(pmmc_ctx *ctx, int level, const void *problem, const
   h.n = static_cast<int>(h.pos0.size() / 3);
   for (int i = 0; i < h.n; ++i)
     if (!h.fixed[i]) h.free_list.push_back(i);
+  return set_run(ctx, s, run, h);
+}
+
+pmmc_status set_run(pmmc_ctx *ctx, const pmmc_schedule &s, const pmmc_run &run, HostProblem &h) {
+  pmmc_status ps = check_schedule(ctx, s);
+  if (ps != PMMC_OK) return ps;
   h.sched = s;
   h.seed = run.seed ? run.seed : ctx->default_seed;
   h.segment = run.segment;

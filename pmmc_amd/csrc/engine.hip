@@ -13,6 +13,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1272,11 +1273,68 @@ extern "C" pmmc_status pmmc_smooth_anneal(pmmc_ctx *ctx, const pmmc_smooth_probl
   return plain(ctx, PMMC_LEVEL_SMOOTH, p, s, run, pos_out, stats);
 }
 
+namespace pmmc {
+
+// Jobs are prepared on host threads: building the tables (an n x n scan per
+// arc or heatmap problem) is most of a large batch's host time.  Each thread
+// reports errors through its own copy of the context.  Jobs that repeat an
+// earlier job's (level, problem) copy its tables in a second pass; a job
+// whose representative failed prepares alone, so every job gets the error
+// prepare_problem would give it.
+pmmc_status prepare_jobs(pmmc_ctx *ctx, const pmmc_job *jobs, int n_jobs, std::vector<HostProblem> &hp,
+                         int *failed) {
+  hp.assign(n_jobs, HostProblem{});
+  std::vector<int> rep(n_jobs);
+  std::map<std::pair<int, const void *>, int> first;
+  for (int i = 0; i < n_jobs; ++i)
+    rep[i] = first.emplace(std::make_pair(jobs[i].level, jobs[i].problem), i).first->second;
+  std::vector<pmmc_status> pst(n_jobs, PMMC_OK);
+  std::vector<std::string> pmsg(n_jobs);
+  const int nt = std::max(1, std::min({n_jobs / 4, 16, static_cast<int>(std::thread::hardware_concurrency())}));
+  auto pass = [&](bool copies) {
+    auto body = [&](int t) {
+      pmmc_ctx local;
+      local.device = ctx->device;
+      local.cu_count = ctx->cu_count;
+      local.hbm_bytes = ctx->hbm_bytes;
+      local.default_seed = ctx->default_seed;
+      for (int i = t; i < n_jobs; i += nt) {
+        if ((rep[i] != i) != copies) continue;
+        const pmmc_job &j = jobs[i];
+        if (copies && pst[rep[i]] == PMMC_OK) {
+          hp[i] = hp[rep[i]];
+          pst[i] = set_run(&local, j.schedule, j.run, hp[i]);
+        } else {
+          pst[i] = prepare_problem(&local, j.level, j.problem, j.schedule, j.run, hp[i]);
+        }
+        if (pst[i] != PMMC_OK) pmsg[i] = local.last_error;
+      }
+    };
+    if (nt == 1) {
+      body(0);
+    } else {
+      std::vector<std::thread> th;
+      for (int t = 0; t < nt; ++t) th.emplace_back(body, t);
+      for (auto &x : th) x.join();
+    }
+  };
+  pass(false);
+  pass(true);
+  for (int i = 0; i < n_jobs; ++i)
+    if (pst[i] != PMMC_OK) {
+      if (failed) *failed = i;
+      return fail(ctx, pst[i], pmsg[i]);
+    }
+  return PMMC_OK;
+}
+
+}  // namespace pmmc
+
 extern "C" pmmc_status pmmc_anneal_batch(pmmc_ctx *ctx, const pmmc_job *jobs, int32_t n_jobs,
                                          pmmc_batch_opts *opts) {
   if (!ctx || (!jobs && n_jobs > 0) || n_jobs < 0) return fail(ctx, PMMC_ERR_INVALID, "bad batch");
   if (n_jobs == 0) return PMMC_OK;
-  std::vector<HostProblem> hp(n_jobs);
+  std::vector<HostProblem> hp;
   std::vector<float *> outs(n_jobs);
   std::vector<pmmc_anneal_stats *> st(n_jobs);
   for (int i = 0; i < n_jobs; ++i) {
@@ -1284,33 +1342,9 @@ extern "C" pmmc_status pmmc_anneal_batch(pmmc_ctx *ctx, const pmmc_job *jobs, in
     outs[i] = jobs[i].pos_out;
     st[i] = jobs[i].stats;
   }
-  // Prepare the jobs on host threads: building the tables (an n x n scan per
-  // arc problem) is most of a large batch's host time.  Each thread reports
-  // errors through its own copy of the context; the first failing job's
-  // message is kept.
-  std::vector<pmmc_status> pst(n_jobs, PMMC_OK);
-  std::vector<std::string> pmsg(n_jobs);
-  const int nt = std::max(1, std::min({n_jobs / 4, 16, static_cast<int>(std::thread::hardware_concurrency())}));
-  auto prep = [&](int t) {
-    pmmc_ctx local;
-    local.device = ctx->device;
-    local.cu_count = ctx->cu_count;
-    local.hbm_bytes = ctx->hbm_bytes;
-    local.default_seed = ctx->default_seed;
-    for (int i = t; i < n_jobs; i += nt) {
-      pst[i] = prepare_problem(&local, jobs[i].level, jobs[i].problem, jobs[i].schedule, jobs[i].run, hp[i]);
-      if (pst[i] != PMMC_OK) pmsg[i] = local.last_error;
-    }
-  };
-  if (nt == 1) {
-    prep(0);
-  } else {
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t) th.emplace_back(prep, t);
-    for (auto &x : th) x.join();
-  }
-  for (int i = 0; i < n_jobs; ++i)
-    if (pst[i] != PMMC_OK) return fail(ctx, pst[i], "job " + std::to_string(i) + ": " + pmsg[i]);
+  int bad = -1;
+  const pmmc_status pp = prepare_jobs(ctx, jobs, n_jobs, hp, &bad);
+  if (pp != PMMC_OK) return fail(ctx, pp, "job " + std::to_string(bad) + ": " + ctx->last_error);
   BatchShape sh;
   sh.bps = opts ? opts->blocks_per_step : 0;
   sh.spg = opts ? opts->steps_per_graph : 0;

@@ -50,6 +50,17 @@ int pick_shift_of(int level, int n);
 
 pmmc_status prepare_problem(pmmc_ctx *ctx, int level, const void *problem, const pmmc_schedule &s,
                             const pmmc_run &run, HostProblem &out);
+// The schedule and run fields of a prepared problem (the last step of
+// prepare_problem): a copy of a prepared problem given another schedule and
+// run equals prepare_problem of the same problem struct with them.
+pmmc_status set_run(pmmc_ctx *ctx, const pmmc_schedule &s, const pmmc_run &run, HostProblem &h);
+// prepare_problem of every job (on up to 16 host threads).  A job naming the
+// same level and problem struct as an earlier job of the list copies that
+// job's prepared tables and sets its own schedule and run, instead of building
+// the tables again (an n x n scan per arc or heatmap problem).  On failure
+// *failed is the first failing job and ctx's error message is that job's.
+pmmc_status prepare_jobs(pmmc_ctx *ctx, const pmmc_job *jobs, int n_jobs, std::vector<HostProblem> &hp,
+                         int *failed);
 pmmc_status check_schedule(pmmc_ctx *ctx, const pmmc_schedule &s);
 int default_chains(const pmmc_ctx *ctx, const HostProblem &h);
 // ensembles (one wavefront each) of h's annealing kernel one CU holds at once

@@ -415,25 +415,39 @@ extern "C" void pmmc_server_destroy(pmmc_server *s) {
   delete s;
 }
 
+extern "C" pmmc_status pmmc_server_submit_many(pmmc_server *s, const pmmc_job *jobs, int32_t n_jobs,
+                                               uint64_t *job_ids) {
+  if (!s || (!jobs && n_jobs > 0) || n_jobs < 0)
+    return fail(s ? s->ctx : nullptr, PMMC_ERR_INVALID, "server: bad job list");
+  for (int i = 0; i < n_jobs; ++i)
+    if (!jobs[i].pos_out) return fail(s->ctx, PMMC_ERR_INVALID, "server: bad job");
+  // every job is prepared and sized before any is queued, so a failing call
+  // queues nothing
+  std::vector<HostProblem> hp;
+  int bad = -1;
+  pmmc_status ps = prepare_jobs(s->ctx, jobs, n_jobs, hp, &bad);
+  if (ps != PMMC_OK) return ps;
+  std::vector<size_t> bytes(n_jobs);
+  for (int i = 0; i < n_jobs; ++i) {
+    bytes[i] = job_bytes(hp[i]);
+    if (bytes[i] > s->opts.hbm_budget) return fail(s->ctx, PMMC_ERR_NOMEM, "server: job larger than the HBM budget");
+  }
+  for (int i = 0; i < n_jobs; ++i) {
+    SrvJob *j = new SrvJob();
+    j->hp = std::move(hp[i]);
+    j->bytes = bytes[i];
+    j->id = s->next_id++;
+    j->pos_out = jobs[i].pos_out;
+    j->stats = jobs[i].stats;
+    s->queue.push_back(j);
+    if (job_ids) job_ids[i] = j->id;
+  }
+  return PMMC_OK;
+}
+
 extern "C" pmmc_status pmmc_server_submit(pmmc_server *s, const pmmc_job *job, uint64_t *job_id) {
   if (!s || !job || !job->pos_out) return fail(s ? s->ctx : nullptr, PMMC_ERR_INVALID, "server: bad job");
-  SrvJob *j = new SrvJob();
-  pmmc_status ps = prepare_problem(s->ctx, job->level, job->problem, job->schedule, job->run, j->hp);
-  if (ps != PMMC_OK) {
-    delete j;
-    return ps;
-  }
-  j->bytes = job_bytes(j->hp);
-  if (j->bytes > s->opts.hbm_budget) {
-    delete j;
-    return fail(s->ctx, PMMC_ERR_NOMEM, "server: job larger than the HBM budget");
-  }
-  j->id = s->next_id++;
-  j->pos_out = job->pos_out;
-  j->stats = job->stats;
-  s->queue.push_back(j);
-  if (job_id) *job_id = j->id;
-  return PMMC_OK;
+  return pmmc_server_submit_many(s, job, 1, job_id);
 }
 
 // Initial totals and chain state of the jobs admitted in this step: one

@@ -335,6 +335,60 @@ def test_server_alone_equals_mixed(ctx, bps, launch_ms, streams):
     srv.close()
 
 
+def test_server_submit_many_equals_plain(ctx):
+    """Jobs of one pmmc_server_submit_many call that name the same problem
+    share its preparation; each result equals the plain call's, including a
+    job list that interleaves two problems with their own schedules, chain
+    counts and seeds."""
+    hs = schedule_preset(1, 1, settings(steps=512), 1.0)
+    hs.max_moves = 1024
+    asched = schedule_preset(2, 1, settings(steps=2048, improvement=0.975, min_successes=100), 0.3)
+    asched.max_moves = 2048
+    hp = synth.heatmap_problem(n=48, seed=9)[0]
+    ap = synth.arcs_problem(n=120, n_arcs=360, seed=71)
+    srv = ctx.server(hbm_budget=1 << 30, max_jobs=64, blocks_per_step=0)
+    ids_h = srv.submit_many(hp, hs, seeds=[3, 4, 5], segment=7, n_chains=128)
+    ids_a = srv.submit_many(ap, asched, seeds=[6, 7], segment=2, n_chains=64)
+    while not all(srv.done(j) for j in ids_h + ids_a):
+        srv.step()
+    for sd, j in zip([3, 4, 5], ids_h):
+        pa, sa = ctx.anneal(hp, hs, seed=sd, segment=7, n_chains=128)
+        pb, sb = srv.result(j)
+        assert np.array_equal(pa, pb) and sa["energy"] == sb["energy"], sd
+    for sd, j in zip([6, 7], ids_a):
+        pa, sa = ctx.anneal(ap, asched, seed=sd, segment=2, n_chains=64)
+        pb, sb = srv.result(j)
+        assert np.array_equal(pa, pb) and sa["energy"] == sb["energy"], sd
+    # a bad job in the list queues nothing
+    bad = schedule_preset(2, 1, settings(steps=2048), 0.3)
+    bad.block = 0
+    with pytest.raises(Exception):
+        srv.submit_many(ap, bad, seeds=[1, 2])
+    _, r, q = srv.step()
+    assert r == 0 and q == 0
+    srv.close()
+
+
+def test_batch_shared_problem_equals_distinct(ctx):
+    """A batch whose jobs name one problem object (prepared once) equals the
+    same batch over separate copies of it (prepared per job)."""
+    import copy
+    asched = schedule_preset(2, 1, settings(steps=2048, improvement=0.975, min_successes=100), 0.3)
+    asched.max_moves = 2048
+    ap = synth.arcs_problem(n=90, n_arcs=270, seed=72)
+    hp = synth.heatmap_problem(n=40, seed=3)[0]
+    hs = schedule_preset(1, 1, settings(steps=512), 1.0)
+    hs.max_moves = 1024
+    shared = [ap, hp, ap, ap, hp]
+    distinct = [copy.deepcopy(p) for p in shared]
+    scheds = [asched, hs, asched, asched, hs]
+    seeds = [1, 2, 3, 4, 5]
+    a = ctx.anneal_batch(shared, scheds, seeds=seeds, n_chains=64)
+    b = ctx.anneal_batch(distinct, scheds, seeds=seeds, n_chains=64)
+    for (pa, sa), (pb, sb) in zip(a, b):
+        assert np.array_equal(pa, pb) and sa["energy"] == sb["energy"]
+
+
 def test_server_budget_admission(ctx):
     sched = schedule_preset(2, 1, settings(steps=1024), 0.3)
     sched.max_moves = 1024
