@@ -30,10 +30,17 @@ extern "C" pmmc_status pmmc_ctx_create(int device, pmmc_ctx **out) {
   c->device = device;
   c->cu_count = prop.multiProcessorCount;
   c->hbm_bytes = prop.totalGlobalMem;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-      hipEventCreate(&c->ev2) != hipSuccess) {
-    delete c;
+  bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess &&
+            hipEventCreate(&c->ev2) == hipSuccess;
+  for (int k = 0; ok && k < kCtxSideStreams; ++k) {
+    hipStream_t st = nullptr;
+    ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+    if (ok) c->side.push_back(st);
+  }
+  if (!ok) {
+    (void)hipGetLastError();
+    pmmc_ctx_destroy(c);
     return PMMC_ERR_HIP;
   }
   // the allocation-free entry point's scratch (device current); a failure
@@ -51,6 +58,10 @@ extern "C" void pmmc_ctx_destroy(pmmc_ctx *c) {
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->ev2) hipEventDestroy(c->ev2);
+  for (hipStream_t st : c->side) {
+    hipStreamSynchronize(st);
+    hipStreamDestroy(st);
+  }
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }

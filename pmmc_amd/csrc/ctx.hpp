@@ -10,14 +10,25 @@
 
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/pmmc_hip.h"
+
+// side streams a context creates (the server's default of four streams per
+// step: the context's own and these three)
+constexpr int kCtxSideStreams = 3;
 
 struct pmmc_ctx {
   int device = 0;
   int cu_count = 0;
   size_t hbm_bytes = 0;  // the device's global memory
   hipStream_t stream = nullptr;
+  // streams the server spreads a step's group launches over, created with
+  // the context: HIP gives each new stream one of the process's hardware
+  // queues in turn, and a hipGraph launch keeps one for its own stream from
+  // then on, so streams created later can land two on one queue and run
+  // their launches one after another
+  std::vector<hipStream_t> side;
   uint64_t default_seed = 0;
   std::string last_error;
   // grow-only device arena, carved per call

@@ -23,9 +23,12 @@
 // on the device (HIP events around the group's launch, the budget rescaled
 // after every step), so small jobs finish in a few steps while large ones
 // advance one block at a time.  The groups' launches of a step are spread
-// over `streams` streams (default kStreams: the library's and the rest the
-// server's own, all at the default priority): groups touch disjoint jobs, and a step's
-// small launches would otherwise run one after another on an idle device.
+// over `streams` streams (default kStreams, all at the default priority):
+// groups touch disjoint jobs, and a step's small launches would otherwise run
+// one after another on an idle device.  The first is the library's stream,
+// the next three the context's side streams (created with the context, so
+// that each holds a hardware queue of its own even after a hipGraph launch
+// has taken one); a server asked for more creates the rest itself.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -157,10 +160,12 @@ struct pmmc_server {
   std::vector<pmmc::EnsRef> h_ensrefs;
   std::map<pmmc::SrvKey, pmmc::Pacing> pacing;
   int n_ens = 0;
-  // the server's own streams (besides ctx->stream) and the events that order
+  // the streams besides ctx->stream (the context's side streams, then any
+  // the server created itself, which it owns) and the events that order
   // them with it: `ready` after the step's uploads, joined[k] after stream
   // k's launches
   std::vector<hipStream_t> side;
+  std::vector<hipStream_t> own;
   hipEvent_t ready = nullptr;
   std::vector<hipEvent_t> joined;
 };
@@ -376,7 +381,12 @@ extern "C" pmmc_status pmmc_server_create(pmmc_ctx *ctx, const pmmc_server_opts 
   for (int k = 1; ok && k < s->opts.streams; ++k) {
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;
-    ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+    if (static_cast<size_t>(k - 1) < ctx->side.size()) {
+      st = ctx->side[k - 1];
+    } else {
+      ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+      if (ok) s->own.push_back(st);
+    }
     if (ok) s->side.push_back(st);
     ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
     if (ok) s->joined.push_back(ev);
@@ -406,10 +416,8 @@ extern "C" void pmmc_server_destroy(pmmc_server *s) {
   if (s->d_jobrefs) (void)hipFree(s->d_jobrefs);
   if (s->d_ensrefs) (void)hipFree(s->d_ensrefs);
   if (s->h_stage) (void)hipHostFree(s->h_stage);
-  for (hipStream_t st : s->side) {
-    (void)hipStreamSynchronize(st);
-    (void)hipStreamDestroy(st);
-  }
+  for (hipStream_t st : s->side) (void)hipStreamSynchronize(st);
+  for (hipStream_t st : s->own) (void)hipStreamDestroy(st);
   for (hipEvent_t ev : s->joined) (void)hipEventDestroy(ev);
   if (s->ready) (void)hipEventDestroy(s->ready);
   delete s;
