@@ -770,6 +770,7 @@ def main(argv=None):
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-levels", action="store_true")
+    ap.add_argument("--no-matched", action="store_true")
     ap.add_argument("--probe", action="store_true",
                     help="only measure the move kernels' access-pattern bandwidth ceilings")
     ap.add_argument("--only", choices=sorted(ONLY),
@@ -818,7 +819,8 @@ def main(argv=None):
             extra["batched"] = batched(ctx)
         if not a.no_served:
             extra["served"] = served(ctx)
-        extra["matched_energy"] = matched_energy(ctx)
+        if not a.no_matched:
+            extra["matched_energy"] = matched_energy(ctx)
         if not a.no_levels:
             extra["levels"] = levels(ctx, with_cpu=not a.no_cpu)
     value = moves / wall
@@ -863,7 +865,7 @@ def main(argv=None):
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(prob, s, step)
         cb = line["cpu_baseline"]
-        if cb.get("kind") == "reference":
+        if cb.get("kind") == "reference" and "matched_energy" in line:
             # complete anneals on both sides: the GPU's per second over the
             # host's, with each side's median final energy of one chain
             me = line["matched_energy"]
