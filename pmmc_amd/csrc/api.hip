@@ -55,6 +55,7 @@ extern "C" void pmmc_ctx_destroy(pmmc_ctx *c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->arena) hipFree(c->arena);
+  if (c->host_stage) hipHostFree(c->host_stage);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->ev2) hipEventDestroy(c->ev2);

@@ -35,6 +35,11 @@ struct pmmc_ctx {
   char *arena = nullptr;
   size_t arena_cap = 0;
   size_t arena_top = 0;
+  // grow-only pinned host buffer the batch set-up stages the problems'
+  // static tables in (one DMA upload; reused by every call, whose copy has
+  // completed before the call returns)
+  char *host_stage = nullptr;
+  size_t host_stage_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   // the device's static scratch array (pmmc_anneal_noalloc): its device
   // address, looked up when the context is created, or null with the reason

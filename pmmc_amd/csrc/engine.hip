@@ -828,20 +828,46 @@ size_t problem_bytes(const HostProblem &h) {
 }
 
 // Carves device addresses and writes the data at the same offsets of a host
-// staging buffer (both regions 256-byte aligned, offsets as Sizer's).
+// staging buffer (both regions 256-byte aligned, offsets as Sizer's).  Every
+// byte it passes over is written: alignment gaps and the entries a table is
+// padded to (min_count) are zeroed, so the staging buffer need not be
+// cleared beforehand.
 struct StageCarver {
   char *dev;
   char *host;
   size_t top = 0;
+  void align() {
+    const size_t at = (top + 255) & ~size_t(255);
+    if (at > top) std::memset(host + top, 0, at - top);
+    top = at;
+  }
   template <class T>
   T *put(const std::vector<T> &v, size_t min_count = 0) {
-    top = (top + 255) & ~size_t(255);
+    align();
     T *d = reinterpret_cast<T *>(dev + top);
     if (!v.empty()) std::memcpy(host + top, v.data(), sizeof(T) * v.size());
-    top += sizeof(T) * std::max(v.size(), min_count);
+    const size_t n = std::max(v.size(), min_count);
+    if (n > v.size()) std::memset(host + top + sizeof(T) * v.size(), 0, sizeof(T) * (n - v.size()));
+    top += sizeof(T) * n;
     return d;
   }
 };
+
+// The context's pinned staging buffer, grown to at least `bytes`.
+static pmmc_status host_stage_reserve(pmmc_ctx *ctx, size_t bytes) {
+  if (bytes <= ctx->host_stage_cap) return PMMC_OK;
+  PMMC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
+  ctx->host_stage = nullptr;
+  ctx->host_stage_cap = 0;
+  const size_t cap = std::max(bytes, 2 * ctx->host_stage_cap);
+  if (hipHostMalloc(reinterpret_cast<void **>(&ctx->host_stage), cap) != hipSuccess) {
+    ctx->host_stage = nullptr;
+    return fail(ctx, PMMC_ERR_NOMEM, "pinned staging buffer allocation failed");
+  }
+  ctx->host_stage_cap = cap;
+  return PMMC_OK;
+}
 
 static void fill_static(StageCarver &sc, const HostProblem &h, DevProblem &P) {
   P.free_list = sc.put(h.free_list, 1);
@@ -974,6 +1000,10 @@ pmmc_status batch_setup(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &b
     ob += out_block(h);
   }
   const size_t need = batch_bytes(hp);
+  // a caller's span (the allocation-free entry point) allocates nothing: its
+  // tables are staged in pageable memory; otherwise in the context's pinned
+  // buffer, uploaded by DMA without a runtime copy
+  const bool caller_span = span.base != nullptr;
   if (span.base) {
     if (need > span.cap)
       return fail(ctx, PMMC_ERR_NOMEM, "the problem needs " + std::to_string(need) +
@@ -995,8 +1025,16 @@ pmmc_status batch_setup(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &b
   char *sdev = cv.take<char>(sb);
   bd.outs = cv.take<char>(ob);
   bd.outs_bytes = ob;
-  bd.stage.assign(sb, 0);
-  StageCarver sc{sdev, bd.stage.data()};
+  char *host = nullptr;
+  if (caller_span) {
+    bd.stage.resize(sb);
+    host = bd.stage.data();
+  } else {
+    pmmc_status ps = host_stage_reserve(ctx, sb);
+    if (ps != PMMC_OK) return ps;
+    host = ctx->host_stage;
+  }
+  StageCarver sc{sdev, host};
   size_t otop = 0;
   bd.out_off.assign(hp.size(), 0);
   bd.hprobs.resize(hp.size());
@@ -1025,7 +1063,7 @@ pmmc_status batch_setup(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &b
     const size_t before = bd.hens.size();
     DevProblem &P = bd.hprobs[oi];
     std::memset(&P, 0, sizeof(P));
-    sc.top = (sc.top + 255) & ~size_t(255);
+    sc.align();
     fill_static(sc, h, P);
     bd.out_off[oi] = otop;
     float *out_pos = reinterpret_cast<float *>(bd.outs + otop);
@@ -1034,8 +1072,9 @@ pmmc_status batch_setup(pmmc_ctx *ctx, std::vector<HostProblem> &hp, BatchDev &b
     bd.groups.back().n_ens += static_cast<int>(bd.hens.size() - before);
   }
   bd.n_ens_total = static_cast<int>(bd.hens.size());
+  if (sc.top < sb) std::memset(host + sc.top, 0, sb - sc.top);
   // one upload of every problem's static tables, then the descriptors
-  if (sb > 0) err = hipMemcpyAsync(sdev, bd.stage.data(), sb, hipMemcpyHostToDevice, s);
+  if (sb > 0) err = hipMemcpyAsync(sdev, host, sb, hipMemcpyHostToDevice, s);
   if (err == hipSuccess)
     err = hipMemcpyAsync(bd.probs, bd.hprobs.data(), sizeof(DevProblem) * hp.size(),
                          hipMemcpyHostToDevice, s);

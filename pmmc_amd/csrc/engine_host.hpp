@@ -93,7 +93,8 @@ struct BatchDev {
   std::vector<DevEns> hens;
   std::vector<LaunchGroup> groups;
   int n_ens_total = 0;
-  std::vector<char> stage;      // host copy of the static tables (one upload)
+  std::vector<char> stage;      // staging of the static tables for a caller-given span
+                                // (otherwise the context's pinned buffer)
   char *outs = nullptr;         // every problem's result positions
   size_t outs_bytes = 0;
   std::vector<size_t> out_off;  // problem i's positions at outs + out_off[i]
