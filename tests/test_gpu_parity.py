@@ -389,6 +389,32 @@ def test_batch_shared_problem_equals_distinct(ctx):
         assert np.array_equal(pa, pb) and sa["energy"] == sb["energy"]
 
 
+def test_batch_shared_problem_errors_name_the_job(ctx):
+    """Jobs sharing a problem's preparation still fail one by one, as
+    separately prepared jobs would: a bad schedule on the first job (the one
+    whose tables the others copy) or on a later one is reported for that job,
+    and a bad problem for the first job naming it."""
+    from pmmc_amd.api import PmmcError
+    good = schedule_preset(2, 1, settings(steps=1024), 0.3)
+    good.max_moves = 1024
+    bad = schedule_preset(2, 1, settings(steps=1024), 0.3)
+    bad.block = 0
+    ap = synth.arcs_problem(n=50, n_arcs=150, seed=73)
+    with pytest.raises(PmmcError, match="job 0: schedule"):
+        ctx.anneal_batch([ap, ap, ap], [bad, good, good], seeds=[1, 2, 3], n_chains=64)
+    with pytest.raises(PmmcError, match="job 2: schedule"):
+        ctx.anneal_batch([ap, ap, ap], [good, good, bad], seeds=[1, 2, 3], n_chains=64)
+    nan = synth.arcs_problem(n=50, n_arcs=150, seed=73)
+    nan.exp_dist = np.array(nan.exp_dist, np.float32)
+    nan.exp_dist[3, 4] = np.nan
+    with pytest.raises(PmmcError, match="job 1: arcs: NaN"):
+        ctx.anneal_batch([ap, nan, nan], [good, good, good], seeds=[1, 2, 3], n_chains=64)
+    # the context still works afterwards
+    a = ctx.anneal_batch([ap, ap], [good, good], seeds=[1, 2], n_chains=64)
+    b = ctx.anneal_batch([ap, ap], [good, good], seeds=[1, 2], n_chains=64)
+    assert all(np.array_equal(x[0], y[0]) for x, y in zip(a, b))
+
+
 def test_server_budget_admission(ctx):
     sched = schedule_preset(2, 1, settings(steps=1024), 0.3)
     sched.max_moves = 1024
