@@ -500,8 +500,17 @@ def batched(ctx, chains_per_problem=256, plain_subset=16):
                              blocks_per_step=shape["blocks_per_step"],
                              steps_per_graph=shape["steps_per_graph"], chains_per_wave=64)
     t64 = time.perf_counter() - t0
+    # the same batch as one launch that runs every chain to its end
+    # (blocks_per_step -1: no steps, graph or tuner; the workgroup dispatcher
+    # starts ensembles beyond the resident ones as slots free up)
+    t0 = time.perf_counter()
+    res1 = ctx.anneal_batch(probs, sched, seeds=seeds, n_chains=chains_per_problem,
+                            blocks_per_step=-1)
+    t1 = time.perf_counter() - t0
     same_shape = all(np.array_equal(a[0], b[0]) and a[1]["energy"] == b[1]["energy"]
                      for a, b in zip(res, res64))
+    same_shape &= all(np.array_equal(a[0], b[0]) and a[1]["energy"] == b[1]["energy"]
+                      for a, b in zip(res, res1))
     # plain, one problem at a time, on an evenly spread subset
     idx = list(range(0, n, max(1, n // plain_subset)))[:plain_subset]
     t0 = time.perf_counter()
@@ -534,6 +543,7 @@ def batched(ctx, chains_per_problem=256, plain_subset=16):
         "launch_shape": {k: shape[k] for k in ("chains_per_wave", "blocks_per_step",
                                                  "steps_per_graph", "tuned")},
         "one_lane_per_chain_moves_per_s": sum(r[1]["moves"] for r in res64) / t64,
+        "one_launch_solves_per_s": n / t1,
         "results_identical_across_launch_shapes": same_shape,
         "anchors_min_max": [int(min(m[2] for m in meta)), int(max(m[2] for m in meta))],
         "batched_solves_per_s": n / tb, "batched_moves_per_s": mv_b / tb,

@@ -413,7 +413,14 @@ typedef struct {
 } pmmc_job;
 
 typedef struct {
-  int32_t blocks_per_step;   /* blocks of moves per step launch (0 = autotune)  */
+  int32_t blocks_per_step;   /* blocks of moves per step launch (0 = autotune);
+                                -1: one launch per level group that runs every
+                                chain to its end, each workgroup keeping its
+                                ensemble until its last chain stops (the
+                                workgroup dispatcher starts the ensembles
+                                beyond the resident ones as slots free up: a
+                                persistent pool with the dispatcher as its
+                                queue); no steps, graph or tuning           */
   int32_t steps_per_graph;   /* step launches captured per hipGraph (0 = auto)  */
   int32_t use_graph;         /* 1: replay captured step loops                  */
   int32_t chains_per_wave;   /* arc level: 64 (a lane per chain) or 32 (a lane

@@ -1391,10 +1391,21 @@ extern "C" pmmc_status pmmc_anneal_batch(pmmc_ctx *ctx, const pmmc_job *jobs, in
   sh.cpw = opts ? opts->chains_per_wave : 0;
   if (sh.cpw != 0 && sh.cpw != 64 && sh.cpw != 32)
     return fail(ctx, PMMC_ERR_INVALID, "chains_per_wave must be 0, 32 or 64");
-  sh.tune = sh.bps <= 0 || sh.cpw <= 0 || (sh.graph && sh.spg <= 0);
+  if (sh.bps < -1) return fail(ctx, PMMC_ERR_INVALID, "blocks_per_step must be >= -1");
+  const bool one_launch = sh.bps == -1;
+  if (one_launch) {
+    // one launch per level group that runs every chain to its end: nothing
+    // to step, capture or tune but the chains per wavefront (64 if unset)
+    sh.bps = 0;
+    sh.graph = false;
+    sh.tune = false;
+    if (sh.cpw <= 0) sh.cpw = 64;
+  } else {
+    sh.tune = sh.bps <= 0 || sh.cpw <= 0 || (sh.graph && sh.spg <= 0);
+  }
   const pmmc_status ps = run_problems(ctx, hp, outs, st, sh);
   if (ps == PMMC_OK && opts) {
-    opts->blocks_per_step = sh.bps;
+    opts->blocks_per_step = one_launch ? -1 : sh.bps;
     opts->steps_per_graph = sh.graph ? sh.spg : 0;
     opts->chains_per_wave = sh.cpw > 0 ? sh.cpw : 64;
     opts->tuned = sh.tuned;

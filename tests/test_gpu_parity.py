@@ -249,6 +249,30 @@ def test_batch_equals_plain(ctx):
         assert sa["best_chain"] == sb["best_chain"]
 
 
+@pytest.mark.parametrize("cpw", [64, 32])
+def test_batch_one_launch_equals_plain(ctx, cpw):
+    """blocks_per_step = -1: one launch per level group runs every chain to
+    its end (no steps, graph or tuner); same results as plain calls, with one
+    lane or a lane pair per arc chain."""
+    s = settings(max_temp=5.0, dt_temp=0.9999, improvement=0.975, steps=2048, min_successes=100)
+    sched = schedule_preset(2, 1, s, 0.3)
+    sched.max_moves = 3072
+    probs = [synth.arcs_problem(n=n, n_arcs=3 * n, seed=60 + n) for n in (25, 110, 300)]
+    ss = schedule_preset(3, 1, settings(steps=1024, dt_temp=0.9995), 0.5)
+    sm = synth.smooth_problem(n_anchors=12, add=9, seed=5)
+    alone = [ctx.anneal(p, sched, seed=13, segment=i, n_chains=192) for i, p in enumerate(probs)]
+    alone.append(ctx.anneal(sm, ss, seed=13, segment=3, n_chains=64))
+    shape = {}
+    one = ctx.anneal_batch(probs + [sm], [sched] * 3 + [ss], seeds=[13] * 4,
+                           n_chains=[192, 192, 192, 64], blocks_per_step=-1,
+                           chains_per_wave=cpw, opts_out=shape)
+    assert shape["blocks_per_step"] == -1 and shape["chains_per_wave"] == cpw
+    assert shape["tuned"] == 0
+    for (pa, sa), (pb, sb) in zip(alone, one):
+        assert np.array_equal(pa, pb)
+        assert sa["energy"] == sb["energy"] and sa["moves"] == sb["moves"]
+
+
 # ---------------- heatmap sweep (reference GPU heatmap algorithm) ----------------
 # The GPU scores a proposal in FP32 (reciprocal weights, hardware square
 # root, two interleaved accumulators) and keeps each bead's reference score
