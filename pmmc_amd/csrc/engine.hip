@@ -856,11 +856,11 @@ struct StageCarver {
 // The context's pinned staging buffer, grown to at least `bytes`.
 static pmmc_status host_stage_reserve(pmmc_ctx *ctx, size_t bytes) {
   if (bytes <= ctx->host_stage_cap) return PMMC_OK;
+  const size_t cap = std::max(bytes, 2 * ctx->host_stage_cap);
   PMMC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
   ctx->host_stage = nullptr;
   ctx->host_stage_cap = 0;
-  const size_t cap = std::max(bytes, 2 * ctx->host_stage_cap);
   if (hipHostMalloc(reinterpret_cast<void **>(&ctx->host_stage), cap) != hipSuccess) {
     ctx->host_stage = nullptr;
     return fail(ctx, PMMC_ERR_NOMEM, "pinned staging buffer allocation failed");
