@@ -127,9 +127,11 @@ bool read_hcm_new(FILE *f, pmmc_tree &t) {
     if (std::fscanf(f, "%4095s", buf) != 1) return false;
     t.factors.push_back(buf);
   }
-  t.cl.resize(n);
+  // records are appended as they parse (a header count is not trusted with
+  // an allocation of its size)
+  t.cl.reserve(std::min(n, 1 << 20));
   for (int i = 0; i < n; ++i) {
-    TCluster &c = t.cl[i];
+    TCluster &c = t.cl.emplace_back();
     int st, en, cnt;
     char o;
     float x, y, z;
@@ -182,9 +184,9 @@ bool read_hcm_previous(FILE *f, pmmc_tree &t) {
     if (std::fscanf(f, "%4095s", buf) != 1) return false;
     t.factors.push_back(buf);
   }
-  t.cl.resize(n);
+  t.cl.reserve(std::min(n, 1 << 20));
   for (int i = 0; i < n; ++i) {
-    TCluster &c = t.cl[i];
+    TCluster &c = t.cl.emplace_back();
     int gpos, st, en, cnt;
     float x, y, z;
     if (std::fscanf(f, "%d %d %d %f %f %f %d", &gpos, &st, &en, &x, &y, &z, &cnt) != 7 || cnt < 0)

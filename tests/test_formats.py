@@ -157,6 +157,21 @@ def test_hcm_detection_rule(tmp_path):
     assert len(r0) == len(big)
 
 
+@pytest.mark.parametrize("fmt,header", [(1, "2000000000 1 0 1\nchr1\n"), (0, "2000000000 0 0 1\nchr1\n")])
+def test_hcm_header_count_beyond_the_file(tmp_path, fmt, header):
+    """A header promising two billion clusters with one record behind it is a
+    malformed file: rejected as such, without an allocation of the promised
+    size."""
+    import resource
+    p = tmp_path / "bad.hcm"
+    rec = "0 10 N 1 2 3 0\n" if fmt == 1 else "5 0 10 1 2 3 0\n"
+    p.write_text(header + rec)
+    before = resource.getrusage(resource.RUSAGE_SELF).ru_maxrss
+    with pytest.raises(RuntimeError, match="malformed"):
+        T.Tree.from_hcm(p, fmt)
+    assert resource.getrusage(resource.RUSAGE_SELF).ru_maxrss - before < 512 * 1024  # KiB
+
+
 @pytest.mark.parametrize("k", range(len(FX["heat"])))
 def test_heat_write_and_read_match_reference(tmp_path, k):
     c = FX["heat"][k]
