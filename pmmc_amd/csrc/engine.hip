@@ -197,9 +197,12 @@ __global__ __launch_bounds__(256) void k_init_problem_refs(const JobRef *__restr
 // consecutive addresses); workgroup 0 also sets the 64 lanes' state.
 __device__ void init_ens_body(const DevProblem &P, const DevEns &it) {
   const size_t total = static_cast<size_t>(3 * P.n) * kLanes;
+  // element i of the ensemble's positions: SoA rows hold pos0[i / 64];
+  // lane-contiguous coordinates (smooth) hold bead i / 192, axis i % 3
+  const bool xyz = P.level == PMMC_LEVEL_SMOOTH;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
        i += static_cast<size_t>(gridDim.x) * blockDim.x)
-    it.pos[i] = P.pos0[i / kLanes];
+    it.pos[i] = xyz ? P.pos0[3 * (i / (3 * kLanes)) + i % 3] : P.pos0[i / kLanes];
   if (blockIdx.x != 0 || threadIdx.x >= kLanes) return;
   const int lane = threadIdx.x;
   EnsState *st = it.st;
@@ -251,12 +254,13 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
   if (__ballot(status == 0) == 0) return;
 
   const int n = P.n;
-  PosViewT<STAGE> X;
+  typename std::conditional<STAGE, PosViewT<true>, GlobalViewT<LEVEL>>::type X;
   if constexpr (STAGE) {
-    for (int i = 0; i < 3 * n; ++i) lds[i * kLanes + lane] = it.pos[static_cast<size_t>(i) * kLanes + lane];
+    for (int i = 0; i < 3 * n; ++i)
+      lds[i * kLanes + lane] = it.pos[pos_index(kLaneXyz<LEVEL>, i / 3, i % 3, lane)];
     X.p = lds + lane;
   } else {
-    X = global_view(it.pos, lane);
+    X = level_view<LEVEL>(it.pos, lane);
   }
 
   const pmmc_schedule S = P.sched;
@@ -482,7 +486,8 @@ __global__ __launch_bounds__(64) void k_anneal(const DevProblem *__restrict__ pr
     st->status[lane] = status;
   }
   if constexpr (STAGE) {
-    for (int i = 0; i < 3 * n; ++i) it.pos[static_cast<size_t>(i) * kLanes + lane] = X.p[i * kLanes];
+    for (int i = 0; i < 3 * n; ++i)
+      it.pos[pos_index(kLaneXyz<LEVEL>, i / 3, i % 3, lane)] = X.p[i * kLanes];
   }
 }
 
@@ -593,8 +598,9 @@ __device__ void finalize_body(DevProblem &P, const DevEns *__restrict__ ens, Fin
   const int q = bc - static_cast<int>(P.chain0);
   const int be = P.ens0 + q / kLanes, bl = q % kLanes;
   const DevEns &bit = ens[be];
+  const bool xyz = P.level == PMMC_LEVEL_SMOOTH;
   for (int i = t; i < 3 * P.n; i += blockDim.x)
-    P.out_pos[i] = bit.pos[static_cast<size_t>(i) * kLanes + bl];
+    P.out_pos[i] = bit.pos[pos_index(xyz, i / 3, i % 3, static_cast<unsigned>(bl))];
   __syncthreads();
   const float *x = P.out_pos;
   double ret;
@@ -1430,7 +1436,7 @@ __global__ __launch_bounds__(64) void k_delta(const DevProblem *__restrict__ pro
   const int lane = threadIdx.x;
   const int i = blockIdx.x;
   if (i >= n_prop) return;
-  const PosView X = global_view(ens[0].pos, lane);
+  const GlobalViewT<LEVEL> X = level_view<LEVEL>(ens[0].pos, lane);
   const int n = P.n;
   const int p = __builtin_amdgcn_readfirstlane(beads[i]);
   const float px = X.get(p, 0), py = X.get(p, 1), pz = X.get(p, 2);

@@ -10,6 +10,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device.hpp"
 #include "engine.hpp"
 
@@ -117,6 +119,44 @@ __device__ __forceinline__ PosView global_view(float *pos, unsigned lane) {
   v.base = (float PMMC_AS_GLOBAL *)pos;
   v.lane = lane;
   return v;
+}
+
+// An ensemble's positions in HBM are laid out per level.  Heatmap and arc
+// levels: SoA rows, element (bead j, axis k) of lane l at (3j + k) * 64 + l,
+// so each axis of a bead is one 256-byte row.  Smooth level: a lane's three
+// coordinates together, at (64j + l) * 3 + k; a bead is still one 768-byte
+// span, read by one 12-byte load per lane, and a move accepted by one lane
+// (the smooth level's 1.4% acceptance) dirties one cache line instead of
+// three: the move pattern with one lane writing in about every other move
+// sustains 89.7 instead of 81.0 G moves/s.  With about half of the lanes
+// writing (the arc level's 51%) the SoA rows measured 1.6% faster.
+template <int LEVEL>
+constexpr bool kLaneXyz = LEVEL == PMMC_LEVEL_SMOOTH;
+
+__host__ __device__ __forceinline__ size_t pos_index(bool lane_xyz, int j, int k, unsigned lane) {
+  return lane_xyz ? (static_cast<size_t>(j) * kLanes + lane) * 3 + k
+                  : static_cast<size_t>(3 * j + k) * kLanes + lane;
+}
+
+// positions in HBM, lane-contiguous coordinates: base = pos + 3 * lane
+struct PosViewXyz {
+  float PMMC_AS_GLOBAL *base;
+  __device__ __forceinline__ float get(int j, int k) const { return base[j * (3 * kLanes) + k]; }
+  __device__ __forceinline__ void set(int j, int k, float v) const { base[j * (3 * kLanes) + k] = v; }
+};
+
+template <int LEVEL>
+using GlobalViewT = typename std::conditional<kLaneXyz<LEVEL>, PosViewXyz, PosView>::type;
+
+template <int LEVEL>
+__device__ __forceinline__ GlobalViewT<LEVEL> level_view(float *pos, unsigned lane) {
+  if constexpr (kLaneXyz<LEVEL>) {
+    PosViewXyz v;
+    v.base = (float PMMC_AS_GLOBAL *)pos + 3 * lane;
+    return v;
+  } else {
+    return global_view(pos, lane);
+  }
 }
 
 // Pick word of move m from the draw of its group of four moves.
