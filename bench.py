@@ -238,7 +238,7 @@ def smooth_leg(ctx, with_cpu=True, cpu_seconds=8.0):
         "bytes_per_move": bpm, "achieved_GBps": bpm * st["moves"] / kms / 1e9,
         "frac_of_pattern_ceiling": (None if pattern_ceiling("smooth_pattern_ceiling_moves_per_s") is None
                                     else st["moves"] / kms / pattern_ceiling("smooth_pattern_ceiling_moves_per_s")),
-        "bound": "VALU and HBM together (~500 VALU instructions per 64-chain move; "
+        "bound": "VALU and HBM together (~425 VALU instructions per 64-chain move; "
                  "3.8 KB window read per 64-chain move)"}
     if with_cpu:
         out["cpu_reference"] = cpu_reference(3, prob, step, sm, cpu_seconds,

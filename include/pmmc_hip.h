@@ -344,9 +344,10 @@ pmmc_status pmmc_singleton_heatmap(pmmc_ctx *ctx, const pmmc_singleton_input *in
 /* No reference counterpart: measures what the GPU sustains for the move
  * kernels' own access patterns, the ceiling their throughput is read against
  * besides the nominal HBM peak.  Per move of a 64-lane wavefront: `reads`
- * random windows of `chunk_beads` consecutive beads (768 B per bead: three
- * 256-byte SoA rows; chunk_beads 1 with reads <= 8 -- the arc move -- or 5
- * with reads 1 -- the smooth move), then the first window's first bead is
+ * random windows of `chunk_beads` consecutive beads (768 B per bead, laid
+ * out as the level's kernel lays it: chunk_beads 1 with reads <= 8 -- the arc
+ * move, three 256-byte SoA rows per bead -- or 5 with reads 1 -- the smooth
+ * move, each lane's 12 bytes together), then the first window's first bead is
  * written back by no lane (write_mode 0), about half of the lanes (1), every
  * lane (2) or one lane in every other move (3); two moves in flight per
  * wavefront; `waves` wavefronts of

@@ -7,7 +7,9 @@
 // throughput can be set against the ceiling of its own pattern as well as
 // against the nominal peak:
 //   k_probe_moves<CB, MODE>: per move, `reads` random windows of CB consecutive
-//     beads (CB x 768 B), then the first window's first bead written back by
+//     beads (CB x 768 B: SoA rows for the arc move, CB 1; each lane's
+//     coordinates together for the smooth move, CB 5, as the two levels'
+//     kernels lay out positions), then the first window's first bead written back by
 //     no lane (MODE 0), about half of the lanes (1), every lane (2) or one
 //     lane in about every other move (3: a low acceptance rate, as at the
 //     smooth level); two moves in flight per wavefront, as the pipelined move
@@ -59,8 +61,13 @@ __global__ __launch_bounds__(64) void k_probe_moves(float *__restrict__ buf, uin
           const uint32_t b = probe_hash(seed * 13u + static_cast<uint32_t>(it + u) * 8u + r) % n_starts;
           if (r == 0) first[u] = b;
           const float *w = buf + static_cast<size_t>(b) * 3 * kProbeLanes;
+          if constexpr (CB == 1) {
 #pragma unroll
-          for (int k = 0; k < CB * 3; ++k) v[u][r][k] = w[k * kProbeLanes + lane];
+            for (int k = 0; k < 3; ++k) v[u][r][k] = w[k * kProbeLanes + lane];
+          } else {
+#pragma unroll
+            for (int k = 0; k < CB * 3; ++k) v[u][r][k] = w[(k / 3) * 3 * kProbeLanes + 3 * lane + k % 3];
+          }
         }
       }
     }
@@ -85,7 +92,7 @@ __global__ __launch_bounds__(64) void k_probe_moves(float *__restrict__ buf, uin
         // the buffer keeps its contents
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-          if (take) w[k * kProbeLanes + lane] = fmaf(a[k], 0.f, v[u][0][k]);
+          if (take) w[CB == 1 ? k * kProbeLanes + lane : 3 * lane + k] = fmaf(a[k], 0.f, v[u][0][k]);
       }
     }
   }
