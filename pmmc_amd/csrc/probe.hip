@@ -90,9 +90,16 @@ __global__ __launch_bounds__(64) void k_probe_moves(float *__restrict__ buf, uin
                           (MODE == 3 && lane == (probe_hash(seed + it + u) & 127u));
         // the written value equals what was read (the sums are scaled to 0):
         // the buffer keeps its contents
+        // as the kernels write back: non-temporal stores over the arc
+        // level's SoA rows, plain stores of the smooth level's lane triples
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-          if (take) w[CB == 1 ? k * kProbeLanes + lane : 3 * lane + k] = fmaf(a[k], 0.f, v[u][0][k]);
+          if (take) {
+            if constexpr (CB == 1)
+              __builtin_nontemporal_store(fmaf(a[k], 0.f, v[u][0][k]), &w[k * kProbeLanes + lane]);
+            else
+              w[3 * lane + k] = fmaf(a[k], 0.f, v[u][0][k]);
+          }
       }
     }
   }
