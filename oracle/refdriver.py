@@ -1,4 +1,4 @@
-"""Runs the reference's own CPU annealers -- TEST / BASELINE INFRASTRUCTURE ONLY.
+"""Runs the reference's own CPU annealers -- test and CPU-baseline infrastructure only.
 
 oracle/_ref/levels_ref is the reference's LooperSolver compiled in place from
 its sources by oracle/Makefile (target `ref`; only where the reference
