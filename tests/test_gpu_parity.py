@@ -529,3 +529,50 @@ def test_batch_tuner_chooses_and_keeps_results(ctx):
     assert got["blocks_per_step"] >= 1 and 1 <= got["steps_per_graph"] <= 64
     for (pa, sa), (pb, sb) in zip(alone, mixed):
         assert np.array_equal(pa, pb) and sa["energy"] == sb["energy"]
+
+
+# ---------------- full-length chains at the workload sizes ----------------
+# The trajectory tests above cap chains at a few thousand moves.  Here chains
+# of the 64-bead heatmap workload (MonteCarloHeatmap's schedule, ~0.7-1 M
+# moves each) and of the 400-anchor arc segment (MonteCarloArcs' schedule,
+# 450 k moves each) run to their stop rule as single-chain jobs of one batch,
+# and each is compared with the oracle's chain.  A chain whose oracle run
+# made no decision within FP32 error of its threshold (near tie, tie_rel
+# 1e-6) must end bit for bit where the oracle's does; one that did may part
+# after its first near tie.  Long chains nearly all meet one (first after
+# 4-8 k moves), so beyond that: at the heatmap level, where a near tie rarely
+# flips, at least two thirds of the chains must still end bit for bit (12 of
+# 12 measured); at the arc level, whose late moves sit at ~1e-6 relative
+# margins (2 of 12 measured), the chains' final energies must agree with the
+# oracle's within the band the reference comparison uses (|mean log-ratio|
+# <= 1e-3).
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("level,k", [(1, 12), (2, 12)])
+def test_full_length_chains_match_oracle(ctx, level, k):
+    prob, step, _ = synth.config_case(level, 2)
+    sched = schedule_preset(level, 0, settings(**synth.CONFIG_SETTINGS[level]), step)
+    chain0s = [64 * c for c in range(k)]
+    print(f"full-length chains level {level}: {k} single-chain jobs", flush=True)
+    res = ctx.anneal_batch([prob] * k, sched, seeds=[5] * k, segments=[9] * k,
+                           chain0s=chain0s, n_chains=1)
+    P = O.Problem(prob)
+    osched = O.schedule_from(sched)
+    import concurrent.futures as cf
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:   # the oracle's C calls release the GIL
+        orc = list(ex.map(lambda c0: P.anneal(osched, 5, 9, c0, tie_rel=1e-6), chain0s))
+    tie_free = exact = 0
+    for c0, (pos, st), (opos, ost) in zip(chain0s, res, orc):
+        same = (st["moves"] == ost["moves"] and st["accepted"] == ost["accepted"]
+                and np.array_equal(pos, opos))
+        exact += same
+        if ost["near_ties"] == 0:
+            tie_free += 1
+            assert same, f"chain {c0}: no near tie in the oracle's run, yet the chains differ"
+    lr = float(np.mean(np.log([st["energy"] / ost["energy"] for (_, st), (_, ost) in zip(res, orc)])))
+    print(f"level {level}: {exact} of {k} chains bit-exact to their ends "
+          f"({tie_free} without a near tie), "
+          f"{np.mean([r[1]['moves'] for r in res]):.0f} moves per chain on average, "
+          f"energy mean log-ratio {lr:.2e}", flush=True)
+    if level == 1:
+        assert exact >= (2 * k) // 3
+    assert abs(lr) <= 1e-3
