@@ -582,3 +582,33 @@ def test_full_length_chains_match_oracle(ctx, level, k):
     if level == 3:
         assert tie_free >= k // 2
     assert abs(lr) <= 1e-3
+
+
+def test_headline_chains_match_oracle(ctx):
+    """The headline workload as benchmarked: the 400-anchor segment under the
+    GPU-kernel schedule of default.ini (chains capped at 50,176 moves).
+    Twelve chains as single-chain jobs against the oracle's: every chain
+    without a near tie bit for bit, and at least two thirds of all."""
+    import concurrent.futures as cf
+    import bench
+    prob, s, step, _ = bench.workload()
+    sched = schedule_preset(2, 1, s, step)
+    k = 12
+    chain0s = [64 * c for c in range(k)]
+    res = ctx.anneal_batch([prob] * k, sched, seeds=[100] * k, segments=[0] * k,
+                           chain0s=chain0s, n_chains=1)
+    P = O.Problem(prob)
+    osched = O.schedule_from(sched)
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        orc = list(ex.map(lambda c0: P.anneal(osched, 100, 0, c0, tie_rel=1e-6), chain0s))
+    exact = tie_free = 0
+    for c0, (pos, st), (opos, ost) in zip(chain0s, res, orc):
+        same = (st["moves"] == ost["moves"] and st["accepted"] == ost["accepted"]
+                and np.array_equal(pos, opos))
+        exact += same
+        if ost["near_ties"] == 0:
+            tie_free += 1
+            assert same, c0
+    print(f"headline chains: {exact} of {k} bit-exact ({tie_free} without a near tie), "
+          f"{np.mean([r[1]['moves'] for r in res]):.0f} moves each", flush=True)
+    assert exact >= (2 * k) // 3
