@@ -260,11 +260,19 @@ def test_batch_one_launch_equals_plain(ctx, cpw):
     probs = [synth.arcs_problem(n=n, n_arcs=3 * n, seed=60 + n) for n in (25, 110, 300)]
     ss = schedule_preset(3, 1, settings(steps=1024, dt_temp=0.9995), 0.5)
     sm = synth.smooth_problem(n_anchors=12, add=9, seed=5)
+    # a 48-bead heatmap runs the queued register kernel, a 100-bead one the
+    # partner loop over memory
+    hs = schedule_preset(1, 1, settings(steps=512), 1.0)
+    hs.max_moves = 2048
+    h48, _ = synth.heatmap_problem(n=48, seed=6)
+    h100, _ = synth.heatmap_problem(n=100, seed=7)
     alone = [ctx.anneal(p, sched, seed=13, segment=i, n_chains=192) for i, p in enumerate(probs)]
     alone.append(ctx.anneal(sm, ss, seed=13, segment=3, n_chains=64))
+    alone.append(ctx.anneal(h48, hs, seed=13, segment=4, n_chains=128))
+    alone.append(ctx.anneal(h100, hs, seed=13, segment=5, n_chains=64))
     shape = {}
-    one = ctx.anneal_batch(probs + [sm], [sched] * 3 + [ss], seeds=[13] * 4,
-                           n_chains=[192, 192, 192, 64], blocks_per_step=-1,
+    one = ctx.anneal_batch(probs + [sm, h48, h100], [sched] * 3 + [ss, hs, hs], seeds=[13] * 6,
+                           n_chains=[192, 192, 192, 64, 128, 64], blocks_per_step=-1,
                            chains_per_wave=cpw, opts_out=shape)
     assert shape["blocks_per_step"] == -1 and shape["chains_per_wave"] == cpw
     assert shape["tuned"] == 0
