@@ -274,12 +274,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
         g[(3 * p + 1) * kLanes] = ny;
         g[(3 * p + 2) * kLanes] = nz;
       }
+      if (__ballot(ok) != 0) {  // wave-uniform: skipped when every chain rejects
 #pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        const bool hit = ok && j == p;
-        X[j] = hit ? nx : X[j];
-        Y[j] = hit ? ny : Y[j];
-        Z[j] = hit ? nz : Z[j];
+        for (int j = 0; j < NB; ++j) {
+          const bool hit = ok && j == p;
+          X[j] = hit ? nx : X[j];
+          Y[j] = hit ? ny : Y[j];
+          Z[j] = hit ? nz : Z[j];
+        }
       }
       // the next move's prefetched copy was read before this move's store
       const bool same = ok && pn == p;
@@ -484,12 +486,16 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
       }
       const bool mine = ok && (p & 1) == half;
       const int pi = p >> 1;
+      // a wave-uniform branch: most moves are rejected in every chain of the
+      // wave, and then the 32 x (compare + 3 selects) are skipped
+      if (__ballot(mine) != 0) {
 #pragma unroll
-      for (int i = 0; i < 32; ++i) {
-        const bool hit = mine && i == pi;
-        X[i] = hit ? nx : X[i];
-        Y[i] = hit ? ny : Y[i];
-        Z[i] = hit ? nz : Z[i];
+        for (int i = 0; i < 32; ++i) {
+          const bool hit = mine && i == pi;
+          X[i] = hit ? nx : X[i];
+          Y[i] = hit ? ny : Y[i];
+          Z[i] = hit ? nz : Z[i];
+        }
       }
       const bool same = ok && pn == p;
       qx = same ? nx : qx;
