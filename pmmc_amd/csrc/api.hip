@@ -5,6 +5,8 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -369,8 +371,8 @@ static pmmc_status prep_smooth(pmmc_ctx *ctx, const pmmc_smooth_problem *p, Host
   return PMMC_OK;
 }
 
-pmmc_status prepare_problem(pmmc_ctx *ctx, int level, const void *problem, const pmmc_schedule &s,
-                            const pmmc_run &run, HostProblem &h) {
+static pmmc_status prepare_tables(pmmc_ctx *ctx, int level, const void *problem,
+                                  const pmmc_schedule &s, const pmmc_run &run, HostProblem &h) {
   if (!problem) return fail(ctx, PMMC_ERR_INVALID, "null problem");
   pmmc_status ps = check_schedule(ctx, s);
   if (ps != PMMC_OK) return ps;
@@ -386,6 +388,22 @@ pmmc_status prepare_problem(pmmc_ctx *ctx, int level, const void *problem, const
   for (int i = 0; i < h.n; ++i)
     if (!h.fixed[i]) h.free_list.push_back(i);
   return set_run(ctx, s, run, h);
+}
+
+// The host tables of a large problem are several n x n arrays: a failed host
+// allocation becomes PMMC_ERR_NOMEM, never an exception through the C ABI
+// (or out of a preparing thread, which would end the process).
+pmmc_status prepare_problem(pmmc_ctx *ctx, int level, const void *problem, const pmmc_schedule &s,
+                            const pmmc_run &run, HostProblem &h) {
+  try {
+    return prepare_tables(ctx, level, problem, s, run, h);
+  } catch (const std::bad_alloc &) {
+    h = HostProblem{};
+    return fail(ctx, PMMC_ERR_NOMEM, "host memory exhausted while building the problem's tables");
+  } catch (const std::length_error &) {
+    h = HostProblem{};
+    return fail(ctx, PMMC_ERR_INVALID, "problem too large for its host tables");
+  }
 }
 
 pmmc_status set_run(pmmc_ctx *ctx, const pmmc_schedule &s, const pmmc_run &run, HostProblem &h) {

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1347,8 +1348,13 @@ pmmc_status prepare_jobs(pmmc_ctx *ctx, const pmmc_job *jobs, int n_jobs, std::v
         if ((rep[i] != i) != copies) continue;
         const pmmc_job &j = jobs[i];
         if (copies && pst[rep[i]] == PMMC_OK) {
-          hp[i] = hp[rep[i]];
-          pst[i] = set_run(&local, j.schedule, j.run, hp[i]);
+          try {
+            hp[i] = hp[rep[i]];
+            pst[i] = set_run(&local, j.schedule, j.run, hp[i]);
+          } catch (const std::bad_alloc &) {
+            hp[i] = HostProblem{};
+            pst[i] = fail(&local, PMMC_ERR_NOMEM, "host memory exhausted while copying the job's tables");
+          }
         } else {
           pst[i] = prepare_problem(&local, j.level, j.problem, j.schedule, j.run, hp[i]);
         }
