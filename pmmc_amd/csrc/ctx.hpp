@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -54,6 +56,19 @@ namespace pmmc {
 inline pmmc_status fail(pmmc_ctx *c, pmmc_status st, const std::string &msg) {
   if (c) c->last_error = msg;
   return st;
+}
+
+// f() with a failed host allocation turned into PMMC_ERR_NOMEM: no C++
+// exception leaves an entry point of the C ABI.
+template <class F>
+pmmc_status no_throw(pmmc_ctx *c, F &&f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc &) {
+    return fail(c, PMMC_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::length_error &) {
+    return fail(c, PMMC_ERR_INVALID, "size too large for a host array");
+  }
 }
 
 #define PMMC_HIP_TRY(ctx, call)                                                          \

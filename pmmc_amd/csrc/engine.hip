@@ -1381,8 +1381,8 @@ pmmc_status prepare_jobs(pmmc_ctx *ctx, const pmmc_job *jobs, int n_jobs, std::v
 
 }  // namespace pmmc
 
-extern "C" pmmc_status pmmc_anneal_batch(pmmc_ctx *ctx, const pmmc_job *jobs, int32_t n_jobs,
-                                         pmmc_batch_opts *opts) {
+static pmmc_status anneal_batch(pmmc_ctx *ctx, const pmmc_job *jobs, int32_t n_jobs,
+                                pmmc_batch_opts *opts) {
   if (!ctx || (!jobs && n_jobs > 0) || n_jobs < 0) return fail(ctx, PMMC_ERR_INVALID, "bad batch");
   if (n_jobs == 0) return PMMC_OK;
   std::vector<HostProblem> hp;
@@ -1423,6 +1423,11 @@ extern "C" pmmc_status pmmc_anneal_batch(pmmc_ctx *ctx, const pmmc_job *jobs, in
     opts->tuned = sh.tuned;
   }
   return ps;
+}
+
+extern "C" pmmc_status pmmc_anneal_batch(pmmc_ctx *ctx, const pmmc_job *jobs, int32_t n_jobs,
+                                         pmmc_batch_opts *opts) {
+  return no_throw(ctx, [&] { return anneal_batch(ctx, jobs, n_jobs, opts); });
 }
 
 // ------------------------------------------------------------------ //

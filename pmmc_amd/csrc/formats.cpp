@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <string>
 #include <vector>
 
@@ -588,7 +589,13 @@ extern "C" pmmc_status pmmc_tree_read_hcm(const char *path, int32_t format, pmmc
     format = (n > 100 && second < 5) ? PMMC_HCM_NEW : PMMC_HCM_PREVIOUS;
   }
   pmmc_tree *t = new pmmc_tree();
-  const bool ok = format == PMMC_HCM_NEW ? read_hcm_new(f.f, *t) : read_hcm_previous(f.f, *t);
+  bool ok = false;
+  try {  // record counts come from the file: no exception leaves the C ABI
+    ok = format == PMMC_HCM_NEW ? read_hcm_new(f.f, *t) : read_hcm_previous(f.f, *t);
+  } catch (const std::exception &) {
+    delete t;
+    return fail(std::string("pmmc_tree_read_hcm: host memory exhausted reading ") + path, PMMC_ERR_NOMEM);
+  }
   if (!ok || !valid_model(*t)) {
     delete t;
     return fail(std::string("pmmc_tree_read_hcm: malformed file ") + path, PMMC_ERR_INVALID);

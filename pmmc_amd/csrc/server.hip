@@ -36,6 +36,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <set>
 #include <tuple>
 #include <vector>
@@ -423,10 +424,8 @@ extern "C" void pmmc_server_destroy(pmmc_server *s) {
   delete s;
 }
 
-extern "C" pmmc_status pmmc_server_submit_many(pmmc_server *s, const pmmc_job *jobs, int32_t n_jobs,
-                                               uint64_t *job_ids) {
-  if (!s || (!jobs && n_jobs > 0) || n_jobs < 0)
-    return fail(s ? s->ctx : nullptr, PMMC_ERR_INVALID, "server: bad job list");
+static pmmc_status submit_many(pmmc_server *s, const pmmc_job *jobs, int32_t n_jobs,
+                               uint64_t *job_ids) {
   for (int i = 0; i < n_jobs; ++i)
     if (!jobs[i].pos_out) return fail(s->ctx, PMMC_ERR_INVALID, "server: bad job");
   // every job is prepared and sized before any is queued, so a failing call
@@ -440,17 +439,34 @@ extern "C" pmmc_status pmmc_server_submit_many(pmmc_server *s, const pmmc_job *j
     bytes[i] = job_bytes(hp[i]);
     if (bytes[i] > s->opts.hbm_budget) return fail(s->ctx, PMMC_ERR_NOMEM, "server: job larger than the HBM budget");
   }
+  std::vector<std::unique_ptr<SrvJob>> made(n_jobs);
   for (int i = 0; i < n_jobs; ++i) {
-    SrvJob *j = new SrvJob();
-    j->hp = std::move(hp[i]);
-    j->bytes = bytes[i];
-    j->id = s->next_id++;
-    j->pos_out = jobs[i].pos_out;
-    j->stats = jobs[i].stats;
-    s->queue.push_back(j);
-    if (job_ids) job_ids[i] = j->id;
+    made[i].reset(new SrvJob());
+    made[i]->hp = std::move(hp[i]);
+    made[i]->bytes = bytes[i];
+    made[i]->pos_out = jobs[i].pos_out;
+    made[i]->stats = jobs[i].stats;
+  }
+  const size_t queued = s->queue.size();
+  try {
+    for (auto &j : made) s->queue.push_back(j.get());
+  } catch (...) {
+    s->queue.resize(queued);
+    throw;
+  }
+  for (int i = 0; i < n_jobs; ++i) {
+    made[i]->id = s->next_id++;
+    if (job_ids) job_ids[i] = made[i]->id;
+    made[i].release();
   }
   return PMMC_OK;
+}
+
+extern "C" pmmc_status pmmc_server_submit_many(pmmc_server *s, const pmmc_job *jobs, int32_t n_jobs,
+                                               uint64_t *job_ids) {
+  if (!s || (!jobs && n_jobs > 0) || n_jobs < 0)
+    return fail(s ? s->ctx : nullptr, PMMC_ERR_INVALID, "server: bad job list");
+  return no_throw(s->ctx, [&] { return submit_many(s, jobs, n_jobs, job_ids); });
 }
 
 extern "C" pmmc_status pmmc_server_submit(pmmc_server *s, const pmmc_job *job, uint64_t *job_id) {
